@@ -1,0 +1,181 @@
+/* C-ABI of the MI355X device solve for the octree FV curl-curl system
+ *
+ *     (Ce^T Mf^-1 Ce + i*omega*Me(sigma)) e = b
+ *
+ * solved by SSOR-preconditioned BiCGStab with divergence correction.  This is
+ * the drop-in boundary of the forward / adjoint solve path: a host driver
+ * built like ForwardSolverFV (src/forward/forward_solver_fv.cpp) forwards the
+ * bodies of its inner_solve (:266-285), solve_system_ (:727-759),
+ * solve_with_divcorr_ (:778-888) and divcorr_poisson_ (:623-722) here.  No C++
+ * types or exceptions cross this boundary: plain pointers and sizes, int
+ * status codes, and m3d_last_error() for the message.
+ *
+ * Complex vectors are interleaved (re, im) doubles.  Batched calls take
+ * system-major arrays: system s, edge i at index 2*(s*n + i).
+ *
+ * Threading: one context per host thread; calls are blocking from the
+ * caller's view and ordered on the context's stream.  Host buffers stay
+ * caller-owned; the library owns device memory until destroy.
+ */
+#ifndef M3D_DEVICE_H
+#define M3D_DEVICE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define M3D_OK 0
+#define M3D_ERR_ARG 1     /* invalid argument */
+#define M3D_ERR_HIP 2     /* HIP runtime error */
+#define M3D_ERR_STATE 3   /* call order (e.g. no mass set) */
+#define M3D_ERR_ALLOC 4   /* device allocation failed */
+
+/* Solve modes. */
+#define M3D_MODE_INNER 0        /* BiCGStab + dead-edge interpolation: inner_solve (:266-285) */
+#define M3D_MODE_FORWARD 1      /* scattered-field solve_pol (:316-367): rhs_sec = b - A*E_pri,
+                                   BiCGStab from 0, E = E_pri + E_sec, Dirichlet values
+                                   re-imposed, post-hoc DivCorr */
+#define M3D_MODE_SYSTEM 2       /* solve_system_ (:727-759): BiCGStab from x, zeroed if diverged */
+#define M3D_MODE_INTERLEAVED 3  /* adjoint_solve / solve_rhs (:914-940) with
+                                   solve_with_divcorr_ (:778-888); phi0 = G^T b / (i omega) */
+
+/* Inner-solve precision. */
+#define M3D_PREC_FP64 0  /* complex128 BiCGStab (the reference arithmetic) */
+#define M3D_PREC_MIXED 1 /* complex64 inner BiCGStab inside FP64 iterative refinement */
+
+typedef struct m3d_context m3d_context;
+typedef struct m3d_operator m3d_operator;
+
+/* Context per device.  Replaces nothing in the reference (it has no device);
+ * created once per host thread by the driver. */
+int m3d_context_create(int device, m3d_context** out);
+int m3d_context_destroy(m3d_context* ctx);
+/* Message of the last failing call on this host thread. */
+const char* m3d_last_error(void);
+
+/* Frequency-independent operator of one mesh after identity elimination of
+ * the constrained (Dirichlet + dead) edges, in split form:
+ *   A_ij (i != j) = offdiag (real, free rows x free columns only),
+ *   A_ii          = diag_s[i] + i*omega*me[i]  (free rows),
+ *   A_ii          = 1                          (constrained rows).
+ * Replaces the per-frequency SparseMatC A_ of ForwardSolverFV
+ * (forward_solver_fv.h:202; assembled by DiscreteOperators::assemble_system,
+ * operators.cpp:556-837, and modified in place at forward_solver_fv.cpp:224-235). */
+typedef struct {
+    int n;                            /* edge DOFs */
+    const int* rowptr;                /* n+1 */
+    const int* colidx;                /* nnz, no diagonal, no constrained column */
+    const double* values;             /* nnz */
+    const double* diag_s;             /* n: stiffness diagonal */
+    const unsigned char* constrained; /* n: identity rows */
+    const unsigned char* bc;          /* n: Dirichlet rows (value re-imposed after FORWARD) */
+    int n_hanging;                    /* dead coarse edges */
+    const int* hanging;               /* 3*n_hanging: (coarse, child1, child2) */
+} m3d_operator_desc;
+
+int m3d_operator_create(m3d_context* ctx, const m3d_operator_desc* desc, m3d_operator** out);
+int m3d_operator_destroy(m3d_operator* op);
+
+/* Edge mass Me(sigma) (operators.cpp:481-518), n values; call again after
+ * every sigma update (ForwardSolverFV::update_sigma, :53-58). */
+int m3d_operator_set_mass(m3d_operator* op, const double* me);
+
+/* DivCorr data for the current sigma (compute_bc_flags_, :546-610):
+ * node gradient G_node (2 entries per edge, operators.cpp:849-944), the edge
+ * weights Me_sigma, the DivCorr Dirichlet edge / node sets, and the node
+ * Laplacian L = G^T Me_sigma G after its Dirichlet rewrite with the Jacobi
+ * diagonal taken before it. */
+typedef struct {
+    int nn;                           /* nodes */
+    const int* g_col;                 /* 2n: node of each G entry */
+    const double* g_val;              /* 2n */
+    const double* me_sigma;           /* n */
+    const unsigned char* bc_edge;     /* n: edges left uncorrected */
+    const unsigned char* dead_edge;   /* n: excluded from div(sigma E) and phi0 */
+    const unsigned char* bc_node;     /* nn: phi = 0 */
+    const int* L_rowptr;              /* nn+1 */
+    const int* L_colidx;
+    const double* L_values;
+    const double* L_diag;             /* nn */
+} m3d_divcorr_desc;
+
+int m3d_operator_set_divcorr(m3d_operator* op, const m3d_divcorr_desc* desc);
+
+/* Mirrors the solver fields of ForwardParamsFV (forward_solver_fv.h:31-69). */
+typedef struct {
+    double tol;              /* bicgstab_tol */
+    int maxiter;             /* bicgstab_maxiter */
+    int div_correction;      /* div_correction */
+    int div_corr_iters;      /* div_corr_iters */
+    double div_corr_tol;     /* div_corr_tol */
+    int divcorr_outer_max;   /* divcorr_outer_max */
+    int divcorr_iter_per_dc; /* divcorr_iter_per_dc */
+    int precision;           /* M3D_PREC_FP64 or M3D_PREC_MIXED */
+    double inner_tol;        /* MIXED: relative tolerance of each complex64 inner solve */
+} m3d_solve_params;
+
+/* Per-system outcome; mirrors BiCGStabResult (bicgstab.h:23-28) plus what the
+ * outer loops of solve_with_divcorr_ report. */
+typedef struct {
+    int iterations;       /* BiCGStab iterations (summed over bursts / refinement steps) */
+    double residual;      /* recursive relative residual at exit */
+    double true_residual; /* ||A x - b|| / ||b|| in FP64 at exit */
+    int converged;
+    int diverged;         /* solution zeroed because the residual exceeded 1 */
+    int info;             /* 0 converged, 1 early (||s||), 2 breakdown, 3 max iterations */
+    int outer;            /* outer iterations (interleaved bursts / refinement steps) */
+} m3d_solve_result;
+
+/* Solve nsys systems that share this operator, one omega each.
+ *   rhs  [nsys][n] complex: b (FORWARD: rhs_total of build_rhs, :244-255)
+ *   aux  [nsys][n] complex: E_pri for FORWARD, else NULL
+ *   x    [nsys][n] complex: initial guess in (ignored by FORWARD and
+ *                           INTERLEAVED, which start from 0), solution out
+ * Host-memory variant. */
+int m3d_solve_batch(m3d_operator* op, int nsys, int mode, const double* omega,
+                    const double* rhs, const double* aux, double* x,
+                    const m3d_solve_params* params, m3d_solve_result* results);
+
+/* Same with device pointers (HBM-resident inputs and outputs), ordered on the
+ * context's stream. */
+int m3d_solve_batch_device(m3d_operator* op, int nsys, int mode, const double* omega,
+                           const double* d_rhs, const double* d_aux, double* d_x,
+                           const m3d_solve_params* params, m3d_solve_result* results);
+
+/* divcorr_poisson_ (:623-722) on nsys fields in place; phi0 [nsys][nn]
+ * complex or NULL. */
+int m3d_divcorr(m3d_operator* op, int nsys, const double* omega, double* E, const double* phi0,
+                const m3d_solve_params* params);
+
+/* Building blocks, host memory, for parity tests of the kernels. */
+/* y = A x (CSRMatrix::matvec, common.h:52-61). */
+int m3d_spmv(m3d_operator* op, int nsys, const double* omega, const double* x, double* y);
+/* z = M^-1 r with the multicolour symmetric Gauss-Seidel (ssor.h:53-79 in
+ * the colour order returned by m3d_operator_permutation). */
+int m3d_ssor_apply(m3d_operator* op, int nsys, const double* omega, const double* r, double* z);
+/* Device row order: perm[k] = original edge at device row k (-1 = padding
+ * row); colour_start[c] for c = 0..ncolours (device rows).  Either output
+ * may be NULL; sizes via m3d_operator_info. */
+int m3d_operator_permutation(m3d_operator* op, int* perm, int* colour_start);
+/* n_padded, ncolours, stored off-diagonal entries (padding included) and
+ * off-diagonal nonzeros. */
+int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long* stored_entries,
+                      long long* nnz);
+
+/* Measurement (no reference counterpart): average duration in ms of one
+ * launch of a kernel class over `reps` launches, timed with HIP events on the
+ * context stream, and its algorithmic HBM bytes per launch.
+ * kind 0: batched SpMV y = A x; kind 1: one multicolour SSOR apply. */
+int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
+                     double* avg_ms, double* alg_bytes);
+
+/* Stream the library launches on (hipStream_t), for callers that time or
+ * order work around it. */
+void* m3d_context_stream(m3d_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* M3D_DEVICE_H */

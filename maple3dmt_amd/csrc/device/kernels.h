@@ -1,0 +1,73 @@
+// Host-callable launchers of the solve kernels (kernels.hip).
+#pragma once
+
+#include "dev_common.h"
+
+namespace m3d {
+namespace dev {
+
+template <int NS, typename V>
+struct Launch {
+    using C = cvec<V>;
+    static void pack(hipStream_t s, int npad, int n, int nsys, const int* perm, const double2* src,
+                     C* dst);
+    static void unpack(hipStream_t s, int npad, int n, int nsys, const int* perm, const C* src,
+                       double2* dst);
+    static void bicg_init(hipStream_t s, const DevOp<V>& op, const double* omega, const C* b,
+                          const C* x, C* r, C* r0, SysState* st, double* part, unsigned* ctr,
+                          int x_zero);
+    static void sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
+                          bool pupd, const C* in, C* p, const C* v, C* y, const SysState* st);
+    static void sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
+                          const C* y, C* z, const SysState* st);
+    static void spmv_v(hipStream_t s, const DevOp<V>& op, const double* omega, const C* z,
+                       const C* r0, C* v, SysState* st, double* part, unsigned* ctr);
+    static void bicg_s(hipStream_t s, int n, const C* r, const C* v, C* sv, SysState* st,
+                       double* part, unsigned* ctr);
+    static void spmv_t(hipStream_t s, const DevOp<V>& op, const double* omega, const C* z,
+                       const C* sv, C* t, SysState* st, double* part, unsigned* ctr);
+    static void bicg_xr(hipStream_t s, int n, C* x, const C* ph, const C* sh, const C* sv,
+                        const C* t, const C* r0, C* r, SysState* st, double* part, unsigned* ctr);
+    static void spmv(hipStream_t s, const DevOp<V>& op, const double* omega, const C* x, const C* b,
+                     C* y);
+    static void fill_zero(hipStream_t s, size_t count, C* x);
+    static void interp_hanging(hipStream_t s, int npairs, const int* tri, C* x);
+};
+
+template <int NS>
+struct Launch64 {
+    static void true_res(hipStream_t s, const DevOp<double>& op, const double* omega,
+                         const double2* x, const double2* b, double2* r, SysState* st,
+                         double* part, unsigned* ctr);
+    static void zero_systems(hipStream_t s, int npad, const int* flags, double2* x);
+    static void forward_total(hipStream_t s, int npad, const unsigned char* bc,
+                              const double2* epri, const double2* b, double2* x);
+    static void add_correction(hipStream_t s, size_t count, const float2* d, double2* x);
+    static void demote(hipStream_t s, size_t count, const double2* src, const double* scale,
+                       float2* dst);
+    static void add_scaled(hipStream_t s, size_t count, const float2* d, const double* scale,
+                           double2* x);
+    static void phi0(hipStream_t s, int nn, const int* nt_ptr, const int* nt_edge,
+                     const double* nt_val, const unsigned char* dead, const double* omega,
+                     const double2* b, double2* phi0);
+    static void dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, const int* nt_ptr,
+                        const int* nt_edge, const double* nt_val, const double* ms,
+                        const unsigned char* bc_node, const double* ldiag, const double2* E,
+                        const double2* phi0, double2* r, double2* z, double2* p, double2* phi,
+                        SysState* st, double* part, unsigned* ctr);
+    static void dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
+                      const double2* p, double2* ap, SysState* st, double* part, unsigned* ctr);
+    static void dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
+                          const double2* ap, double2* phi, double2* r, double2* z, SysState* st,
+                          double* part, unsigned* ctr);
+    static void dc_p(hipStream_t s, int nn, const double2* z, double2* p, const SysState* st);
+    static void dc_correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
+                           const unsigned char* bc_edge, const double2* phi, double2* E,
+                           const SysState* st);
+};
+
+void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol);
+void launch_dc_begin(hipStream_t s, SysState* st, unsigned mask);
+
+}  // namespace dev
+}  // namespace m3d

@@ -1,0 +1,1023 @@
+// HIP kernels of the batched FP64 / FP32 octree curl-curl solve on CDNA4.
+//
+// Layout: the operator is kept in colour order, padded so every colour starts
+// on a 64-row slice; each slice stores its strictly lower (earlier colours)
+// and strictly upper (later colours) couplings as column-major SELL-64 blocks,
+// so lane k of a wavefront reads entry j of row k with one coalesced load.
+// Vectors of a batch are system-minor, x[row * NS + sys], so the NS lanes of
+// one row read NS consecutive complex values when they gather a column.
+// Thread t of a block works on row (t / NS), system (t % NS).
+#include "dev_common.h"
+#include "kernels.h"
+
+namespace m3d {
+namespace dev {
+
+namespace {
+
+template <int NS>
+__device__ inline int rows_per_block() { return kBlock / NS; }
+
+template <typename V>
+__device__ inline void row_gather(const V* __restrict__ val, const int* __restrict__ col, int off,
+                                  int w, int lane, const cvec<V>* __restrict__ x, int NS, int sys,
+                                  V& ar, V& ai) {
+    for (int k = 0; k < w; ++k) {
+        const int idx = off + k * kSlice + lane;
+        const V a = val[idx];
+        const cvec<V> xv = x[size_t(col[idx]) * NS + sys];
+        ar += a * xv.x;
+        ai += a * xv.y;
+    }
+}
+
+// (A x)_row for one system: lower + upper couplings plus the diagonal
+// (diag_s + i*omega*me) * x_row.
+template <typename V>
+__device__ inline cvec<V> apply_row(const DevOp<V>& op, int row, int NS, int sys, V w,
+                                    const cvec<V>* __restrict__ x) {
+    const int sl = row >> 6, lane = row & 63;
+    V ar = 0, ai = 0;
+    row_gather(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, x, NS, sys, ar, ai);
+    row_gather(op.u_val, op.u_col, op.u_off[sl], op.u_w[sl], lane, x, NS, sys, ar, ai);
+    const V dr = op.diag_s[row], di = w * op.me[row];
+    const cvec<V> xi = x[size_t(row) * NS + sys];
+    cvec<V> y;
+    y.x = ar + (dr * xi.x - di * xi.y);
+    y.y = ai + (dr * xi.y + di * xi.x);
+    return y;
+}
+
+template <typename V>
+__device__ inline cvec<V> to_v(double2 a) {
+    cvec<V> r;
+    r.x = V(a.x);
+    r.y = V(a.y);
+    return r;
+}
+template <typename V>
+__device__ inline double2 to_d(cvec<V> a) {
+    return mk(double(a.x), double(a.y));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Layout conversion: system-major double (caller order) <-> system-minor V
+// (device row order).  perm[row] = caller edge of device row, -1 = padding.
+// ---------------------------------------------------------------------------
+template <int NS, typename V>
+__global__ void k_pack(int npad, int n, int nsys, const int* __restrict__ perm,
+                       const double2* __restrict__ src, cvec<V>* __restrict__ dst) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int row = int(t / NS), sys = int(t % NS);
+        const int e = perm[row];
+        double2 v = mk(0.0, 0.0);
+        if (e >= 0 && sys < nsys && src) v = src[size_t(sys) * n + e];
+        dst[t] = to_v<V>(v);
+    }
+}
+
+template <int NS, typename V>
+__global__ void k_unpack(int npad, int n, int nsys, const int* __restrict__ perm,
+                         const cvec<V>* __restrict__ src, double2* __restrict__ dst) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int row = int(t / NS), sys = int(t % NS);
+        const int e = perm[row];
+        if (e >= 0 && sys < nsys) dst[size_t(sys) * n + e] = to_d<V>(src[t]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab (bicgstab.cpp:15-166), right preconditioned, Hermitian dots.
+// ---------------------------------------------------------------------------
+
+// Start a solve on the systems whose mask bit is set.
+__global__ void k_bicg_begin(SysState* st, unsigned mask, int maxiter, double tol) {
+    const int s = threadIdx.x;
+    if (s >= kMaxSys) return;
+    SysState& S = st[s];
+    if (mask & (1u << s)) {
+        S.status = kRunning;
+        S.iter = 0;
+        S.maxiter = maxiter;
+        S.tol = tol;
+        S.info = 0;
+        S.rho_old = mk(1.0, 0.0);
+        S.alpha = mk(1.0, 0.0);
+        S.omega = mk(1.0, 0.0);
+        S.beta = mk(0.0, 0.0);
+    } else if (S.status == kRunning || S.status == kPendConvEarly || S.status == kPendBreakTT) {
+        S.status = kIdle;
+    }
+}
+
+// r = b - A x (or r = b when x is known to be zero), r0 = r; norms and
+// rho = (r0, r) for the first iteration (bicgstab.cpp:29-46, 51-58).
+template <int NS, typename V>
+__global__ void k_bicg_init(DevOp<V> op, const double* __restrict__ omega,
+                            const cvec<V>* __restrict__ b, const cvec<V>* __restrict__ x,
+                            cvec<V>* __restrict__ r, cvec<V>* __restrict__ r0, SysState* st,
+                            double* part, unsigned* counter, int x_zero) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const bool run = st[sys].status == kRunning;
+    const V w = V(omega[sys]);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int nrb = (op.n + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= op.n || !run) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const cvec<V> bi = b[i];
+        cvec<V> ri = bi;
+        if (!x_zero) {
+            const cvec<V> ax = apply_row(op, row, NS, sys, w, x);
+            ri.x = bi.x - ax.x;
+            ri.y = bi.y - ax.y;
+        }
+        r[i] = ri;
+        r0[i] = ri;
+        const double2 bd = to_d<V>(bi), rd = to_d<V>(ri);
+        acc[0] += cabs2(bd);
+        acc[1] += cabs2(rd);
+        const double2 rr = cconjmul(rd, rd);
+        acc[2] += rr.x;
+    }
+    double out[3];
+    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.status != kRunning) return;
+    double bn = sqrt(out[0]);
+    if (bn < 1e-30) bn = 1.0;
+    S.bnorm = bn;
+    S.rnorm = sqrt(out[1]);
+    if (S.rnorm / bn < S.tol) {
+        S.status = kConverged;
+        S.residual = S.rnorm / bn;
+        S.iter = 0;
+        S.info = 0;
+        return;
+    }
+    S.rho = mk(out[2], 0.0);
+    if (cabs(S.rho) < 1e-30) {
+        S.status = kBreakdown;
+        S.info = 2;
+        S.iter = 1;
+        S.residual = S.rnorm / bn;
+    }
+}
+
+// Forward multicolour Gauss-Seidel sweep over one colour [r0, r1):
+//   y_i = (in_i - sum_{j in earlier colours} a_ij y_j) / d_i   (ssor.h:58-65)
+// With PUPD the input is the search direction, updated first:
+//   p = r (first iteration) or p = r + beta (p - omega v)   (bicgstab.cpp:60-67)
+template <int NS, typename V, bool PUPD>
+__global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
+                            const cvec<V>* __restrict__ in, cvec<V>* __restrict__ p,
+                            const cvec<V>* __restrict__ v, cvec<V>* __restrict__ y,
+                            const SysState* __restrict__ st) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const SysState& S = st[sys];
+    if (S.status != kRunning) return;
+    const V w = V(omega[sys]);
+    const int nrows = r1 - r0;
+    const int nrb = (nrows + rpb - 1) / rpb;
+    const bool first = S.iter == 0;
+    const V br = V(S.beta.x), bi = V(S.beta.y), wr = V(S.omega.x), wi = V(S.omega.y);
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = r0 + rb * rpb + tid / NS;
+        if (row >= r1) continue;
+        const size_t i = size_t(row) * NS + sys;
+        cvec<V> src = in[i];
+        if (PUPD) {
+            if (!first) {
+                const cvec<V> pi = p[i], vi = v[i];
+                // p - omega v
+                const V tr = pi.x - (wr * vi.x - wi * vi.y);
+                const V ti = pi.y - (wr * vi.y + wi * vi.x);
+                src.x = src.x + (br * tr - bi * ti);
+                src.y = src.y + (br * ti + bi * tr);
+            }
+            p[i] = src;
+        }
+        const int sl = row >> 6, lane = row & 63;
+        V ar = 0, ai = 0;
+        row_gather(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, y, NS, sys, ar, ai);
+        V dr, di;
+        cinv(op.diag_s[row], w * op.me[row], dr, di);
+        const V er = src.x - ar, ei = src.y - ai;
+        cvec<V> out;
+        out.x = er * dr - ei * di;
+        out.y = er * di + ei * dr;
+        y[i] = out;
+    }
+}
+
+// Backward sweep over one colour: z_i = y_i - (sum_{j in later colours} a_ij z_j) / d_i
+// (ssor.h:71-78).
+template <int NS, typename V>
+__global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
+                            const cvec<V>* __restrict__ y, cvec<V>* __restrict__ z,
+                            const SysState* __restrict__ st) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    if (st[sys].status != kRunning) return;
+    const V w = V(omega[sys]);
+    const int nrows = r1 - r0;
+    const int nrb = (nrows + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = r0 + rb * rpb + tid / NS;
+        if (row >= r1) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const int sl = row >> 6, lane = row & 63;
+        V ar = 0, ai = 0;
+        row_gather(op.u_val, op.u_col, op.u_off[sl], op.u_w[sl], lane, z, NS, sys, ar, ai);
+        V dr, di;
+        cinv(op.diag_s[row], w * op.me[row], dr, di);
+        const cvec<V> yi = y[i];
+        cvec<V> out;
+        out.x = yi.x - (ar * dr - ai * di);
+        out.y = yi.y - (ar * di + ai * dr);
+        z[i] = out;
+    }
+}
+
+// v = A p_hat and (r0, v); alpha = rho / (r0, v) (bicgstab.cpp:76-87).
+template <int NS, typename V>
+__global__ void k_spmv_v(DevOp<V> op, const double* __restrict__ omega,
+                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ r0,
+                         cvec<V>* __restrict__ v, SysState* st, double* part, unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const bool run = st[sys].status == kRunning;
+    const V w = V(omega[sys]);
+    double acc[2] = {0.0, 0.0};
+    const int nrb = (op.n + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= op.n || !run) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const cvec<V> y = apply_row(op, row, NS, sys, w, z);
+        v[i] = y;
+        const double2 d = cconjmul(to_d<V>(r0[i]), to_d<V>(y));
+        acc[0] += d.x;
+        acc[1] += d.y;
+    }
+    double out[2];
+    if (!grid_reduce<NS, 2>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.status != kRunning) return;
+    const double2 r0v = mk(out[0], out[1]);
+    if (cabs(r0v) < 1e-30) {
+        S.status = kBreakdown;
+        S.info = 2;
+        S.iter = S.iter + 1;
+        S.residual = S.rnorm / S.bnorm;
+        return;
+    }
+    S.alpha = cdiv(S.rho, r0v);
+}
+
+// s = r - alpha v, ||s||; early exit when ||s|| / ||b|| < tol
+// (bicgstab.cpp:89-103); x += alpha p_hat is applied by k_bicg_xr.
+template <int NS, typename V>
+__global__ void k_bicg_s(int n, const cvec<V>* __restrict__ r, const cvec<V>* __restrict__ v,
+                         cvec<V>* __restrict__ s, SysState* st, double* part, unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const bool run = st[sys].status == kRunning;
+    const V ar = V(st[sys].alpha.x), ai = V(st[sys].alpha.y);
+    double acc[1] = {0.0};
+    const int nrb = (n + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= n || !run) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const cvec<V> ri = r[i], vi = v[i];
+        cvec<V> si;
+        si.x = ri.x - (ar * vi.x - ai * vi.y);
+        si.y = ri.y - (ar * vi.y + ai * vi.x);
+        s[i] = si;
+        acc[0] += cabs2(to_d<V>(si));
+    }
+    double out[1];
+    if (!grid_reduce<NS, 1>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.status != kRunning) return;
+    S.snorm = sqrt(out[0]);
+    if (S.snorm / S.bnorm < S.tol) {
+        S.status = kPendConvEarly;
+        S.residual = S.snorm / S.bnorm;
+        S.iter = S.iter + 1;
+        S.info = 1;
+    }
+}
+
+// t = A s_hat, (t,t), (t,s); omega = (t,s)/(t,t) (bicgstab.cpp:112-125).
+template <int NS, typename V>
+__global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
+                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ s,
+                         cvec<V>* __restrict__ t, SysState* st, double* part, unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const bool run = st[sys].status == kRunning;
+    const V w = V(omega[sys]);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int nrb = (op.n + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= op.n || !run) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const cvec<V> y = apply_row(op, row, NS, sys, w, z);
+        t[i] = y;
+        const double2 td = to_d<V>(y);
+        acc[0] += cabs2(td);
+        const double2 ts = cconjmul(td, to_d<V>(s[i]));
+        acc[1] += ts.x;
+        acc[2] += ts.y;
+    }
+    double out[3];
+    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.status != kRunning) return;
+    const double tt = out[0];
+    if (fabs(tt) < 1e-30) {
+        S.status = kPendBreakTT;
+        S.residual = S.snorm / S.bnorm;
+        S.iter = S.iter + 1;
+        S.info = 2;
+        return;
+    }
+    S.omega = cdiv(mk(out[1], out[2]), mk(tt, 0.0));
+}
+
+// x += alpha p_hat + omega s_hat, r = s - omega t, ||r||, (r0, r); then the
+// end-of-iteration checks and the next beta (bicgstab.cpp:127-160).
+template <int NS, typename V>
+__global__ void k_bicg_xr(int n, cvec<V>* __restrict__ x, const cvec<V>* __restrict__ ph,
+                          const cvec<V>* __restrict__ sh, const cvec<V>* __restrict__ s,
+                          const cvec<V>* __restrict__ t, const cvec<V>* __restrict__ r0,
+                          cvec<V>* __restrict__ r, SysState* st, double* part, unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const int status = st[sys].status;
+    const bool run = status == kRunning;
+    const bool pend = status == kPendConvEarly || status == kPendBreakTT;
+    const V ar = V(st[sys].alpha.x), ai = V(st[sys].alpha.y);
+    const V wr = V(st[sys].omega.x), wi = V(st[sys].omega.y);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int nrb = (n + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= n || !(run || pend)) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const cvec<V> pi = ph[i];
+        cvec<V> xi = x[i];
+        if (pend) {
+            xi.x += ar * pi.x - ai * pi.y;
+            xi.y += ar * pi.y + ai * pi.x;
+            x[i] = xi;
+            continue;
+        }
+        const cvec<V> si = sh[i];
+        xi.x += (ar * pi.x - ai * pi.y) + (wr * si.x - wi * si.y);
+        xi.y += (ar * pi.y + ai * pi.x) + (wr * si.y + wi * si.x);
+        x[i] = xi;
+        const cvec<V> ss = s[i], ti = t[i];
+        cvec<V> ri;
+        ri.x = ss.x - (wr * ti.x - wi * ti.y);
+        ri.y = ss.y - (wr * ti.y + wi * ti.x);
+        r[i] = ri;
+        const double2 rd = to_d<V>(ri);
+        acc[0] += cabs2(rd);
+        const double2 d = cconjmul(to_d<V>(r0[i]), rd);
+        acc[1] += d.x;
+        acc[2] += d.y;
+    }
+    double out[3];
+    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.status == kPendConvEarly) { S.status = kConvergedEarly; return; }
+    if (S.status == kPendBreakTT) { S.status = kBreakdown; return; }
+    if (S.status != kRunning) return;
+    const int it = S.iter + 1;
+    S.rnorm = sqrt(out[0]);
+    const double rel = S.rnorm / S.bnorm;
+    if (rel < S.tol) {
+        S.status = kConverged; S.iter = it; S.residual = rel; S.info = 0;
+        return;
+    }
+    if (cabs(S.omega) < 1e-30) {
+        S.status = kBreakdown; S.iter = it; S.residual = rel; S.info = 2;
+        return;
+    }
+    S.rho_old = S.rho;
+    S.iter = it;
+    S.residual = rel;
+    if (it >= S.maxiter) {
+        S.status = kMaxIter; S.info = 3;
+        return;
+    }
+    S.rho = mk(out[1], out[2]);
+    if (cabs(S.rho) < 1e-30) {
+        S.status = kBreakdown; S.iter = it + 1; S.info = 2;
+        return;
+    }
+    S.beta = cmul(cdiv(S.rho, S.rho_old), cdiv(S.alpha, S.omega));
+}
+
+// ---------------------------------------------------------------------------
+// Plain operator application and residuals.
+// ---------------------------------------------------------------------------
+
+// y = A x, or y = b - A x when b != nullptr.
+template <int NS, typename V>
+__global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
+                       const cvec<V>* __restrict__ x, const cvec<V>* __restrict__ b,
+                       cvec<V>* __restrict__ y) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const V w = V(omega[sys]);
+    const int nrb = (op.n + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= op.n) continue;
+        const size_t i = size_t(row) * NS + sys;
+        cvec<V> a = apply_row(op, row, NS, sys, w, x);
+        if (b) {
+            const cvec<V> bi = b[i];
+            a.x = bi.x - a.x;
+            a.y = bi.y - a.y;
+        }
+        y[i] = a;
+    }
+}
+
+// ||A x - b|| / ||b|| in FP64 (forward_solver_fv.cpp:843-848, 786-790); with
+// r != nullptr also stores r = b - A x in FP64 (iterative refinement).
+template <int NS>
+__global__ void k_true_res(DevOp<double> op, const double* __restrict__ omega,
+                           const double2* __restrict__ x, const double2* __restrict__ b,
+                           double2* __restrict__ r, SysState* st, double* part,
+                           unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const double w = omega[sys];
+    double acc[2] = {0.0, 0.0};
+    const int nrb = (op.n + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= op.n) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const double2 ax = apply_row(op, row, NS, sys, w, x);
+        const double2 bi = b[i];
+        const double2 d = mk(bi.x - ax.x, bi.y - ax.y);
+        if (r) r[i] = d;
+        acc[0] += cabs2(d);
+        acc[1] += cabs2(bi);
+    }
+    double out[2];
+    if (!grid_reduce<NS, 2>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    double bn = sqrt(out[1]);
+    if (bn < 1e-30) bn = 1.0;
+    st[tid].true_bnorm = bn;
+    st[tid].true_res = sqrt(out[0]) / bn;
+}
+
+// ---------------------------------------------------------------------------
+// Vector utilities.
+// ---------------------------------------------------------------------------
+template <int NS, typename V>
+__global__ void k_fill_zero(size_t count, cvec<V>* __restrict__ x) {
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < count;
+         t += size_t(gridDim.x) * blockDim.x) {
+        cvec<V> z;
+        z.x = 0;
+        z.y = 0;
+        x[t] = z;
+    }
+}
+
+// Zero the solution of systems whose flag is set (divergence guard,
+// forward_solver_fv.cpp:749-752, 821-826).
+template <int NS>
+__global__ void k_zero_systems(int npad, const int* __restrict__ flags, double2* __restrict__ x) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        if (flags[t % NS]) x[t] = mk(0.0, 0.0);
+    }
+}
+
+// E[coarse] = 0.5 (E[child1] + E[child2]) for one independent level of
+// hanging pairs (forward_solver_fv.cpp:258-261).
+template <int NS, typename V>
+__global__ void k_interp_hanging(int npairs, const int* __restrict__ tri, cvec<V>* __restrict__ x) {
+    const size_t total = size_t(npairs) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int p = int(t / NS), sys = int(t % NS);
+        const int c = tri[3 * p], a = tri[3 * p + 1], b = tri[3 * p + 2];
+        const cvec<V> xa = x[size_t(a) * NS + sys], xb = x[size_t(b) * NS + sys];
+        cvec<V> o;
+        o.x = V(0.5) * (xa.x + xb.x);
+        o.y = V(0.5) * (xa.y + xb.y);
+        x[size_t(c) * NS + sys] = o;
+    }
+}
+
+// E = E_pri + E_sec, then Dirichlet rows take rhs_total (E_bc)
+// (forward_solver_fv.cpp:357-361).
+template <int NS>
+__global__ void k_forward_total(int npad, const unsigned char* __restrict__ bc,
+                                const double2* __restrict__ epri, const double2* __restrict__ b,
+                                double2* __restrict__ x) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int row = int(t / NS);
+        if (bc[row]) {
+            x[t] = b[t];
+        } else {
+            const double2 a = epri[t], c = x[t];
+            x[t] = mk(a.x + c.x, a.y + c.y);
+        }
+    }
+}
+
+// x += d (FP64 accumulate of an FP32 correction).
+template <int NS>
+__global__ void k_add_correction(size_t count, const float2* __restrict__ d, double2* __restrict__ x) {
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < count;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const float2 a = d[t];
+        const double2 c = x[t];
+        x[t] = mk(c.x + double(a.x), c.y + double(a.y));
+    }
+}
+
+// FP32 copy of a (scaled) FP64 vector: dst = src * scale[sys].
+template <int NS>
+__global__ void k_demote(size_t count, const double2* __restrict__ src, const double* __restrict__ scale,
+                         float2* __restrict__ dst) {
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < count;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const double2 a = src[t];
+        const double sc = scale[t % NS];
+        dst[t] = make_float2(float(a.x * sc), float(a.y * sc));
+    }
+}
+
+// x += d * scale[sys] (undo the right-hand-side scaling of a correction).
+template <int NS>
+__global__ void k_add_scaled(size_t count, const float2* __restrict__ d, const double* __restrict__ scale,
+                             double2* __restrict__ x) {
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < count;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const float2 a = d[t];
+        const double sc = scale[t % NS];
+        const double2 c = x[t];
+        x[t] = mk(c.x + double(a.x) * sc, c.y + double(a.y) * sc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Divergence correction (divcorr_poisson_, forward_solver_fv.cpp:623-722),
+// FP64 throughout.  Node n's incident edges are nt_ptr[n]..nt_ptr[n+1] in
+// ascending caller edge order; nt_edge holds device rows, nt_val the G value.
+// ---------------------------------------------------------------------------
+
+// phi0 = G^T b / (i omega) over non-dead edges (compute_phi0, :893-908).
+template <int NS>
+__global__ void k_phi0(int nn, const int* __restrict__ nt_ptr, const int* __restrict__ nt_edge,
+                       const double* __restrict__ nt_val, const unsigned char* __restrict__ dead,
+                       const double* __restrict__ omega, const double2* __restrict__ b,
+                       double2* __restrict__ phi0) {
+    const size_t total = size_t(nn) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int v = int(t / NS), sys = int(t % NS);
+        double2 acc = mk(0.0, 0.0);
+        for (int k = nt_ptr[v]; k < nt_ptr[v + 1]; ++k) {
+            const int e = nt_edge[k];
+            if (dead[e]) continue;
+            const double g = nt_val[k];
+            const double2 bv = b[size_t(e) * NS + sys];
+            acc.x += g * bv.x;
+            acc.y += g * bv.y;
+        }
+        phi0[t] = cdiv(acc, mk(0.0, omega[sys]));
+    }
+}
+
+__global__ void k_dc_begin(SysState* st, unsigned mask) {
+    const int s = threadIdx.x;
+    if (s >= kMaxSys) return;
+    st[s].dc_status = (mask & (1u << s)) ? kRunning : kIdle;
+    st[s].pcg_status = kIdle;
+}
+
+// divJ = G^T (Me_sigma E) - phi0, zero on Dirichlet nodes, ||divJ||; PCG start
+// r = divJ, z = r / diag, p = z, phi = 0, rz = (r, z) (:636-677).
+template <int NS>
+__global__ void k_dc_divj(int nn, int dc_iter, double dc_tol, int cg_max,
+                          const int* __restrict__ nt_ptr, const int* __restrict__ nt_edge,
+                          const double* __restrict__ nt_val, const double* __restrict__ ms,
+                          const unsigned char* __restrict__ bc_node, const double* __restrict__ ldiag,
+                          const double2* __restrict__ E, const double2* __restrict__ phi0,
+                          double2* __restrict__ r, double2* __restrict__ z, double2* __restrict__ p,
+                          double2* __restrict__ phi, SysState* st, double* part, unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const bool run = st[sys].dc_status == kRunning;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int nrb = (nn + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int v = rb * rpb + tid / NS;
+        if (v >= nn || !run) continue;
+        const size_t i = size_t(v) * NS + sys;
+        double2 a = mk(0.0, 0.0);
+        for (int k = nt_ptr[v]; k < nt_ptr[v + 1]; ++k) {
+            const int e = nt_edge[k];
+            const double2 ev = E[size_t(e) * NS + sys];
+            const double m = ms[e];
+            const double g = nt_val[k];
+            a.x += g * (m * ev.x);
+            a.y += g * (m * ev.y);
+        }
+        if (phi0) {
+            const double2 f = phi0[i];
+            a.x -= f.x;
+            a.y -= f.y;
+        }
+        if (bc_node[v]) a = mk(0.0, 0.0);
+        r[i] = a;
+        const double d = ldiag[v];
+        const double2 zi = (fabs(d) > 1e-30) ? mk(a.x / d, a.y / d) : a;
+        z[i] = zi;
+        p[i] = zi;
+        phi[i] = mk(0.0, 0.0);
+        acc[0] += cabs2(a);
+        const double2 rz = cconjmul(a, zi);
+        acc[1] += rz.x;
+        acc[2] += rz.y;
+    }
+    double out[3];
+    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.dc_status != kRunning) return;
+    const double dn = sqrt(out[0]);
+    if (dc_iter == 0) S.div_norm0 = dn;
+    S.div_norm = dn;
+    const double rel = (S.div_norm0 > 1e-30) ? dn / S.div_norm0 : 0.0;
+    if ((rel < dc_tol && dc_iter > 0) || dn < 1e-12) {
+        S.dc_status = kConverged;
+        S.pcg_status = kIdle;
+        return;
+    }
+    S.rz = mk(out[1], out[2]);
+    S.pcg_status = kRunning;
+    S.pcg_iter = 0;
+    S.pcg_max = cg_max;
+}
+
+// Ap = L p, (p, Ap); alpha = rz / pAp (:681-692).
+template <int NS>
+__global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restrict__ lcol,
+                        const double* __restrict__ lval, const double2* __restrict__ p,
+                        double2* __restrict__ ap, SysState* st, double* part, unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const bool run = st[sys].pcg_status == kRunning;
+    double acc[2] = {0.0, 0.0};
+    const int nrb = (nn + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int v = rb * rpb + tid / NS;
+        if (v >= nn || !run) continue;
+        const size_t i = size_t(v) * NS + sys;
+        double2 a = mk(0.0, 0.0);
+        for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+            const double l = lval[k];
+            const double2 pv = p[size_t(lcol[k]) * NS + sys];
+            a.x += l * pv.x;
+            a.y += l * pv.y;
+        }
+        ap[i] = a;
+        const double2 d = cconjmul(p[i], a);
+        acc[0] += d.x;
+        acc[1] += d.y;
+    }
+    double out[2];
+    if (!grid_reduce<NS, 2>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.pcg_status != kRunning) return;
+    const double2 pap = mk(out[0], out[1]);
+    if (cabs(pap) < 1e-30) {
+        S.pcg_status = kConverged;
+        return;
+    }
+    S.pcg_alpha = cdiv(S.rz, pap);
+}
+
+// phi += alpha p, r -= alpha Ap, ||r||; z = r / diag, (r, z);
+// stop when ||r|| < 1e-10 ||divJ||, else beta = rz_new / (rz + 1e-30) (:693-711).
+template <int NS>
+__global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const double2* __restrict__ p,
+                            const double2* __restrict__ ap, double2* __restrict__ phi,
+                            double2* __restrict__ r, double2* __restrict__ z, SysState* st,
+                            double* part, unsigned* counter) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const bool run = st[sys].pcg_status == kRunning;
+    const double2 al = st[sys].pcg_alpha;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int nrb = (nn + rpb - 1) / rpb;
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        const int v = rb * rpb + tid / NS;
+        if (v >= nn || !run) continue;
+        const size_t i = size_t(v) * NS + sys;
+        const double2 pi = p[i], api = ap[i];
+        const double2 ap_ = cmul(al, pi), aap = cmul(al, api);
+        double2 f = phi[i];
+        f.x += ap_.x;
+        f.y += ap_.y;
+        phi[i] = f;
+        double2 ri = r[i];
+        ri.x -= aap.x;
+        ri.y -= aap.y;
+        r[i] = ri;
+        const double d = ldiag[v];
+        const double2 zi = (fabs(d) > 1e-30) ? mk(ri.x / d, ri.y / d) : ri;
+        z[i] = zi;
+        acc[0] += cabs2(ri);
+        const double2 rz = cconjmul(ri, zi);
+        acc[1] += rz.x;
+        acc[2] += rz.y;
+    }
+    double out[3];
+    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
+    if (tid >= NS) return;
+    SysState& S = st[tid];
+    if (S.pcg_status != kRunning) return;
+    S.pcg_iter += 1;
+    S.pcg_rnorm = sqrt(out[0]);
+    if (S.pcg_rnorm < 1e-10 * S.div_norm || S.pcg_iter >= S.pcg_max) {
+        S.pcg_status = kConverged;
+        return;
+    }
+    const double2 rzn = mk(out[1], out[2]);
+    S.pcg_beta = cdiv(rzn, mk(S.rz.x + 1e-30, S.rz.y));
+    S.rz = rzn;
+}
+
+// p = z + beta p.
+template <int NS>
+__global__ void k_dc_p(int nn, const double2* __restrict__ z, double2* __restrict__ p,
+                       const SysState* __restrict__ st) {
+    const size_t total = size_t(nn) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int sys = int(t % NS);
+        if (st[sys].pcg_status != kRunning) continue;
+        const double2 b = st[sys].pcg_beta;
+        const double2 bp = cmul(b, p[t]);
+        const double2 zi = z[t];
+        p[t] = mk(zi.x + bp.x, zi.y + bp.y);
+    }
+}
+
+// E -= G phi on edges outside the DivCorr Dirichlet set (:714-720).
+template <int NS>
+__global__ void k_dc_correct(int npad, const int* __restrict__ g_col, const double* __restrict__ g_val,
+                             const unsigned char* __restrict__ bc_edge,
+                             const double2* __restrict__ phi, double2* __restrict__ E,
+                             const SysState* __restrict__ st) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int row = int(t / NS), sys = int(t % NS);
+        if (st[sys].dc_status != kRunning || bc_edge[row]) continue;
+        const int a = g_col[2 * row], b = g_col[2 * row + 1];
+        if (a < 0) continue;  // padding row
+        const double2 pa = phi[size_t(a) * NS + sys], pb = phi[size_t(b) * NS + sys];
+        const double ga = g_val[2 * row], gb = g_val[2 * row + 1];
+        double2 gr;
+        gr.x = ga * pa.x + gb * pb.x;
+        gr.y = ga * pa.y + gb * pb.y;
+        double2 e = E[t];
+        E[t] = mk(e.x - gr.x, e.y - gr.y);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (explicit instantiation per batch width).
+// ---------------------------------------------------------------------------
+namespace {
+inline int grid_for(size_t work_items, int cap) {
+    const size_t b = (work_items + kBlock - 1) / kBlock;
+    return int(b < size_t(cap) ? (b > 0 ? b : 1) : cap);
+}
+template <int NS>
+inline int row_grid(int rows, int cap = kMaxBlocks) {
+    const int rpb = kBlock / NS;
+    const int b = (rows + rpb - 1) / rpb;
+    return b < cap ? (b > 0 ? b : 1) : cap;
+}
+constexpr int kStreamCap = 4096;
+}  // namespace
+
+template <int NS, typename V>
+void Launch<NS, V>::pack(hipStream_t s, int npad, int n, int nsys, const int* perm,
+                         const double2* src, cvec<V>* dst) {
+    hipLaunchKernelGGL((k_pack<NS, V>), dim3(grid_for(size_t(npad) * NS, kStreamCap)), dim3(kBlock),
+                       0, s, npad, n, nsys, perm, src, dst);
+}
+template <int NS, typename V>
+void Launch<NS, V>::unpack(hipStream_t s, int npad, int n, int nsys, const int* perm,
+                           const cvec<V>* src, double2* dst) {
+    hipLaunchKernelGGL((k_unpack<NS, V>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
+                       dim3(kBlock), 0, s, npad, n, nsys, perm, src, dst);
+}
+template <int NS, typename V>
+void Launch<NS, V>::bicg_init(hipStream_t s, const DevOp<V>& op, const double* omega,
+                              const cvec<V>* b, const cvec<V>* x, cvec<V>* r, cvec<V>* r0,
+                              SysState* st, double* part, unsigned* ctr, int x_zero) {
+    hipLaunchKernelGGL((k_bicg_init<NS, V>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op,
+                       omega, b, x, r, r0, st, part, ctr, x_zero);
+}
+template <int NS, typename V>
+void Launch<NS, V>::sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
+                              int r1, bool pupd, const cvec<V>* in, cvec<V>* p, const cvec<V>* v,
+                              cvec<V>* y, const SysState* st) {
+    const dim3 g(row_grid<NS>(r1 - r0, kStreamCap));
+    if (pupd)
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, true>), g, dim3(kBlock), 0, s, op, omega, r0, r1, in,
+                           p, v, y, st);
+    else
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, false>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
+                           in, p, v, y, st);
+}
+template <int NS, typename V>
+void Launch<NS, V>::sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
+                              int r1, const cvec<V>* y, cvec<V>* z, const SysState* st) {
+    hipLaunchKernelGGL((k_sweep_bwd<NS, V>), dim3(row_grid<NS>(r1 - r0, kStreamCap)), dim3(kBlock),
+                       0, s, op, omega, r0, r1, y, z, st);
+}
+template <int NS, typename V>
+void Launch<NS, V>::spmv_v(hipStream_t s, const DevOp<V>& op, const double* omega,
+                           const cvec<V>* z, const cvec<V>* r0, cvec<V>* v, SysState* st,
+                           double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_spmv_v<NS, V>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op, omega,
+                       z, r0, v, st, part, ctr);
+}
+template <int NS, typename V>
+void Launch<NS, V>::bicg_s(hipStream_t s, int n, const cvec<V>* r, const cvec<V>* v, cvec<V>* sv,
+                           SysState* st, double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_bicg_s<NS, V>), dim3(row_grid<NS>(n)), dim3(kBlock), 0, s, n, r, v, sv,
+                       st, part, ctr);
+}
+template <int NS, typename V>
+void Launch<NS, V>::spmv_t(hipStream_t s, const DevOp<V>& op, const double* omega,
+                           const cvec<V>* z, const cvec<V>* sv, cvec<V>* t, SysState* st,
+                           double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_spmv_t<NS, V>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op, omega,
+                       z, sv, t, st, part, ctr);
+}
+template <int NS, typename V>
+void Launch<NS, V>::bicg_xr(hipStream_t s, int n, cvec<V>* x, const cvec<V>* ph, const cvec<V>* sh,
+                            const cvec<V>* sv, const cvec<V>* t, const cvec<V>* r0, cvec<V>* r,
+                            SysState* st, double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_bicg_xr<NS, V>), dim3(row_grid<NS>(n)), dim3(kBlock), 0, s, n, x, ph, sh,
+                       sv, t, r0, r, st, part, ctr);
+}
+template <int NS, typename V>
+void Launch<NS, V>::spmv(hipStream_t s, const DevOp<V>& op, const double* omega, const cvec<V>* x,
+                         const cvec<V>* b, cvec<V>* y) {
+    hipLaunchKernelGGL((k_spmv<NS, V>), dim3(row_grid<NS>(op.n, kStreamCap)), dim3(kBlock), 0, s,
+                       op, omega, x, b, y);
+}
+template <int NS, typename V>
+void Launch<NS, V>::fill_zero(hipStream_t s, size_t count, cvec<V>* x) {
+    hipLaunchKernelGGL((k_fill_zero<NS, V>), dim3(grid_for(count, kStreamCap)), dim3(kBlock), 0, s,
+                       count, x);
+}
+template <int NS, typename V>
+void Launch<NS, V>::interp_hanging(hipStream_t s, int npairs, const int* tri, cvec<V>* x) {
+    if (npairs <= 0) return;
+    hipLaunchKernelGGL((k_interp_hanging<NS, V>), dim3(grid_for(size_t(npairs) * NS, kStreamCap)),
+                       dim3(kBlock), 0, s, npairs, tri, x);
+}
+
+template <int NS>
+void Launch64<NS>::true_res(hipStream_t s, const DevOp<double>& op, const double* omega,
+                            const double2* x, const double2* b, double2* r, SysState* st,
+                            double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_true_res<NS>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op, omega, x,
+                       b, r, st, part, ctr);
+}
+template <int NS>
+void Launch64<NS>::zero_systems(hipStream_t s, int npad, const int* flags, double2* x) {
+    hipLaunchKernelGGL((k_zero_systems<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
+                       dim3(kBlock), 0, s, npad, flags, x);
+}
+template <int NS>
+void Launch64<NS>::forward_total(hipStream_t s, int npad, const unsigned char* bc,
+                                 const double2* epri, const double2* b, double2* x) {
+    hipLaunchKernelGGL((k_forward_total<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
+                       dim3(kBlock), 0, s, npad, bc, epri, b, x);
+}
+template <int NS>
+void Launch64<NS>::add_correction(hipStream_t s, size_t count, const float2* d, double2* x) {
+    hipLaunchKernelGGL((k_add_correction<NS>), dim3(grid_for(count, kStreamCap)), dim3(kBlock), 0,
+                       s, count, d, x);
+}
+template <int NS>
+void Launch64<NS>::demote(hipStream_t s, size_t count, const double2* src, const double* scale,
+                          float2* dst) {
+    hipLaunchKernelGGL((k_demote<NS>), dim3(grid_for(count, kStreamCap)), dim3(kBlock), 0, s, count,
+                       src, scale, dst);
+}
+template <int NS>
+void Launch64<NS>::add_scaled(hipStream_t s, size_t count, const float2* d, const double* scale,
+                              double2* x) {
+    hipLaunchKernelGGL((k_add_scaled<NS>), dim3(grid_for(count, kStreamCap)), dim3(kBlock), 0, s,
+                       count, d, scale, x);
+}
+template <int NS>
+void Launch64<NS>::phi0(hipStream_t s, int nn, const int* nt_ptr, const int* nt_edge,
+                        const double* nt_val, const unsigned char* dead, const double* omega,
+                        const double2* b, double2* phi0) {
+    hipLaunchKernelGGL((k_phi0<NS>), dim3(grid_for(size_t(nn) * NS, kStreamCap)), dim3(kBlock), 0, s,
+                       nn, nt_ptr, nt_edge, nt_val, dead, omega, b, phi0);
+}
+template <int NS>
+void Launch64<NS>::dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, const int* nt_ptr,
+                           const int* nt_edge, const double* nt_val, const double* ms,
+                           const unsigned char* bc_node, const double* ldiag, const double2* E,
+                           const double2* phi0, double2* r, double2* z, double2* p, double2* phi,
+                           SysState* st, double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_dc_divj<NS>), dim3(row_grid<NS>(nn)), dim3(kBlock), 0, s, nn, it, tol,
+                       cg_max, nt_ptr, nt_edge, nt_val, ms, bc_node, ldiag, E, phi0, r, z, p, phi,
+                       st, part, ctr);
+}
+template <int NS>
+void Launch64<NS>::dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
+                         const double2* p, double2* ap, SysState* st, double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_dc_lp<NS>), dim3(row_grid<NS>(nn)), dim3(kBlock), 0, s, nn, lptr, lcol, lval,
+                       p, ap, st, part, ctr);
+}
+template <int NS>
+void Launch64<NS>::dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
+                             const double2* ap, double2* phi, double2* r, double2* z, SysState* st,
+                             double* part, unsigned* ctr) {
+    hipLaunchKernelGGL((k_dc_update<NS>), dim3(row_grid<NS>(nn)), dim3(kBlock), 0, s, nn, ldiag, p,
+                       ap, phi, r, z, st, part, ctr);
+}
+template <int NS>
+void Launch64<NS>::dc_p(hipStream_t s, int nn, const double2* z, double2* p, const SysState* st) {
+    hipLaunchKernelGGL((k_dc_p<NS>), dim3(grid_for(size_t(nn) * NS, kStreamCap)), dim3(kBlock), 0, s,
+                       nn, z, p, st);
+}
+template <int NS>
+void Launch64<NS>::dc_correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
+                              const unsigned char* bc_edge, const double2* phi, double2* E,
+                              const SysState* st) {
+    hipLaunchKernelGGL((k_dc_correct<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
+                       dim3(kBlock), 0, s, npad, g_col, g_val, bc_edge, phi, E, st);
+}
+
+void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol) {
+    hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(64), 0, s, st, mask, maxiter, tol);
+}
+void launch_dc_begin(hipStream_t s, SysState* st, unsigned mask) {
+    hipLaunchKernelGGL(k_dc_begin, dim3(1), dim3(64), 0, s, st, mask);
+}
+
+#define M3D_INST(NS)                     \
+    template struct Launch<NS, double>;  \
+    template struct Launch<NS, float>;   \
+    template struct Launch64<NS>;
+M3D_INST(1)
+M3D_INST(2)
+M3D_INST(4)
+M3D_INST(8)
+M3D_INST(16)
+M3D_INST(32)
+#undef M3D_INST
+
+}  // namespace dev
+}  // namespace m3d

@@ -1,0 +1,1195 @@
+// Host orchestration of the device solve and the C-ABI declared in
+// include/m3d_device.h.
+//
+// An operator is laid out once per mesh: rows are greedily coloured on the
+// free-free coupling graph, renumbered colour by colour (each colour padded
+// to a 64-row slice), and every slice's lower / upper couplings are packed
+// as SELL-64 blocks.  In that order the natural-order symmetric Gauss-Seidel
+// of ssor.h becomes a multicolour sweep: one launch per colour, every row of
+// a colour independent.  BiCGStab scalars live on the device; the host only
+// polls the per-system status between chunks of iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "m3d_device.h"
+
+using namespace m3d::dev;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Fail : std::runtime_error {
+    int code;
+    Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw Fail(M3D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            throw Fail(M3D_ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+}  // namespace
+
+struct m3d_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+// Per-(batch width, precision) device workspace of one operator.
+template <int NS, typename V>
+struct Work {
+    using C = cvec<V>;
+    DBuf<C> b, x, r, r0, p, v, s, t, y, ph, sh;
+};
+
+template <int NS>
+struct Work64 {
+    DBuf<double2> b, x, aux, tmp, rres;             // FP64 system vectors
+    DBuf<double2> phi0, dr, dz, dp, dphi, dap;      // DivCorr node vectors
+    DBuf<double> omega, scale;
+    DBuf<SysState> st;
+    DBuf<double> part;
+    DBuf<unsigned> ctr;
+    DBuf<int> flags;
+    std::unique_ptr<Work<NS, double>> w64;
+    std::unique_ptr<Work<NS, float>> w32;
+    bool ready = false;
+};
+
+}  // namespace
+
+struct m3d_operator {
+    m3d_context* ctx = nullptr;
+    int n = 0;          // caller edges
+    int npad = 0;       // device rows
+    int ncolours = 0;
+    long long stored = 0, nnz = 0;
+    std::vector<int> perm, iperm, colour_start;
+    std::vector<int> hang_level_start;  // offsets into the hanging triples per level
+    // Operator arrays.
+    DBuf<int> d_perm, l_off, l_w, u_off, u_w, l_col, u_col, hang;
+    DBuf<double> l_val, u_val, diag_s, me;
+    DBuf<float> l_val32, u_val32, diag_s32, me32;
+    DBuf<unsigned char> bc, constrained;
+    std::vector<unsigned char> h_constrained;
+    bool mass_set = false;
+    // DivCorr arrays.
+    bool dc_set = false;
+    int nn = 0;
+    DBuf<int> nt_ptr, nt_edge, g_col, l_ptr, l_colidx;
+    DBuf<double> nt_val, g_val, ms, l_vals, l_diag;
+    DBuf<unsigned char> dc_bc_edge, dc_dead, dc_bc_node;
+    // Workspaces by batch width.
+    std::unique_ptr<Work64<1>> w1;
+    std::unique_ptr<Work64<2>> w2;
+    std::unique_ptr<Work64<4>> w4;
+    std::unique_ptr<Work64<8>> w8;
+    std::unique_ptr<Work64<16>> w16;
+    std::unique_ptr<Work64<32>> w32;
+
+    template <typename V>
+    DevOp<V> view() const;
+};
+
+template <>
+DevOp<double> m3d_operator::view<double>() const {
+    DevOp<double> o;
+    o.n = npad;
+    o.l_off = l_off.p; o.l_w = l_w.p; o.u_off = u_off.p; o.u_w = u_w.p;
+    o.l_val = l_val.p; o.l_col = l_col.p; o.u_val = u_val.p; o.u_col = u_col.p;
+    o.diag_s = diag_s.p; o.me = me.p;
+    return o;
+}
+template <>
+DevOp<float> m3d_operator::view<float>() const {
+    DevOp<float> o;
+    o.n = npad;
+    o.l_off = l_off.p; o.l_w = l_w.p; o.u_off = u_off.p; o.u_w = u_w.p;
+    o.l_val = l_val32.p; o.l_col = l_col.p; o.u_val = u_val32.p; o.u_col = u_col.p;
+    o.diag_s = diag_s32.p; o.me = me32.p;
+    return o;
+}
+
+namespace {
+
+template <int NS>
+Work64<NS>& workspace(m3d_operator& op);
+template <> Work64<1>& workspace<1>(m3d_operator& op) { if (!op.w1) op.w1.reset(new Work64<1>); return *op.w1; }
+template <> Work64<2>& workspace<2>(m3d_operator& op) { if (!op.w2) op.w2.reset(new Work64<2>); return *op.w2; }
+template <> Work64<4>& workspace<4>(m3d_operator& op) { if (!op.w4) op.w4.reset(new Work64<4>); return *op.w4; }
+template <> Work64<8>& workspace<8>(m3d_operator& op) { if (!op.w8) op.w8.reset(new Work64<8>); return *op.w8; }
+template <> Work64<16>& workspace<16>(m3d_operator& op) { if (!op.w16) op.w16.reset(new Work64<16>); return *op.w16; }
+template <> Work64<32>& workspace<32>(m3d_operator& op) { if (!op.w32) op.w32.reset(new Work64<32>); return *op.w32; }
+
+template <int NS>
+Work64<NS>& ready_workspace(m3d_operator& op) {
+    Work64<NS>& W = workspace<NS>(op);
+    if (W.ready) return W;
+    const size_t nv = size_t(op.npad) * NS;
+    W.b.alloc(nv); W.x.alloc(nv); W.aux.alloc(nv); W.tmp.alloc(nv); W.rres.alloc(nv);
+    W.omega.alloc(NS); W.scale.alloc(NS);
+    W.st.alloc(kMaxSys);
+    W.part.alloc(size_t(3) * kMaxSys * kMaxBlocks);
+    W.ctr.alloc(16);
+    W.flags.alloc(NS);
+    HIPCHK(hipMemsetAsync(W.ctr.p, 0, 16 * sizeof(unsigned), op.ctx->stream));
+    HIPCHK(hipMemsetAsync(W.st.p, 0, kMaxSys * sizeof(SysState), op.ctx->stream));
+    W.ready = true;
+    return W;
+}
+
+template <int NS, typename V>
+Work<NS, V>& vec_workspace(m3d_operator& op, Work64<NS>& W);
+template <int NS>
+Work<NS, double>& vec64(m3d_operator& op, Work64<NS>& W) {
+    if (!W.w64) {
+        W.w64.reset(new Work<NS, double>);
+        const size_t nv = size_t(op.npad) * NS;
+        auto& w = *W.w64;
+        w.r.alloc(nv); w.r0.alloc(nv); w.p.alloc(nv); w.v.alloc(nv); w.s.alloc(nv);
+        w.t.alloc(nv); w.y.alloc(nv); w.ph.alloc(nv); w.sh.alloc(nv);
+        hipStream_t st = op.ctx->stream;
+        for (auto* b : {&w.r, &w.r0, &w.p, &w.v, &w.s, &w.t, &w.y, &w.ph, &w.sh})
+            Launch<NS, double>::fill_zero(st, nv, b->p);
+    }
+    return *W.w64;
+}
+template <int NS>
+Work<NS, float>& vec32(m3d_operator& op, Work64<NS>& W) {
+    if (!W.w32) {
+        W.w32.reset(new Work<NS, float>);
+        const size_t nv = size_t(op.npad) * NS;
+        auto& w = *W.w32;
+        w.b.alloc(nv); w.x.alloc(nv);
+        w.r.alloc(nv); w.r0.alloc(nv); w.p.alloc(nv); w.v.alloc(nv); w.s.alloc(nv);
+        w.t.alloc(nv); w.y.alloc(nv); w.ph.alloc(nv); w.sh.alloc(nv);
+        hipStream_t st = op.ctx->stream;
+        for (auto* b : {&w.b, &w.x, &w.r, &w.r0, &w.p, &w.v, &w.s, &w.t, &w.y, &w.ph, &w.sh})
+            Launch<NS, float>::fill_zero(st, nv, b->p);
+    }
+    return *W.w32;
+}
+
+template <int NS>
+void ensure_dc_vectors(m3d_operator& op, Work64<NS>& W) {
+    const size_t nv = size_t(std::max(op.nn, 1)) * NS;
+    if (W.dr.n >= nv) return;
+    W.phi0.alloc(nv); W.dr.alloc(nv); W.dz.alloc(nv); W.dp.alloc(nv); W.dphi.alloc(nv); W.dap.alloc(nv);
+}
+
+std::vector<SysState> read_states(m3d_operator& op, const DBuf<SysState>& st) {
+    std::vector<SysState> h(kMaxSys);
+    HIPCHK(hipMemcpyAsync(h.data(), st.p, kMaxSys * sizeof(SysState), hipMemcpyDeviceToHost,
+                          op.ctx->stream));
+    HIPCHK(hipStreamSynchronize(op.ctx->stream));
+    return h;
+}
+
+// One BiCGStab iteration (bicgstab.cpp:50-160) for every running system.
+template <int NS, typename V>
+void bicg_iteration(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x) {
+    hipStream_t s = op.ctx->stream;
+    const DevOp<V> A = op.view<V>();
+    using L = Launch<NS, V>;
+    const int K = op.ncolours;
+    for (int c = 0; c < K; ++c)
+        L::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], true, w.r.p, w.p.p,
+                     w.v.p, w.y.p, W.st.p);
+    for (int c = K - 1; c >= 0; --c)
+        L::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p, w.ph.p, W.st.p);
+    L::spmv_v(s, A, W.omega.p, w.ph.p, w.r0.p, w.v.p, W.st.p, W.part.p, W.ctr.p);
+    L::bicg_s(s, op.npad, w.r.p, w.v.p, w.s.p, W.st.p, W.part.p, W.ctr.p);
+    for (int c = 0; c < K; ++c)
+        L::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], false, w.s.p,
+                     nullptr, nullptr, w.y.p, W.st.p);
+    for (int c = K - 1; c >= 0; --c)
+        L::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p, w.sh.p, W.st.p);
+    L::spmv_t(s, A, W.omega.p, w.sh.p, w.s.p, w.t.p, W.st.p, W.part.p, W.ctr.p);
+    L::bicg_xr(s, op.npad, x, w.ph.p, w.sh.p, w.s.p, w.t.p, w.r0.p, w.r.p, W.st.p, W.part.p, W.ctr.p);
+}
+
+struct BicgOutcome {
+    int iterations[kMaxSys];
+    double residual[kMaxSys];
+    int status[kMaxSys];
+    int info[kMaxSys];
+};
+
+// Runs BiCGStab on the masked systems until every one has stopped.
+template <int NS, typename V>
+BicgOutcome bicgstab(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, const cvec<V>* b,
+                     cvec<V>* x, unsigned mask, int maxiter, double tol, bool x_zero) {
+    hipStream_t s = op.ctx->stream;
+    launch_bicg_begin(s, W.st.p, mask, maxiter, tol);
+    Launch<NS, V>::bicg_init(s, op.view<V>(), W.omega.p, b, x, w.r.p, w.r0.p, W.st.p, W.part.p,
+                             W.ctr.p, x_zero ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    int launched = 0;
+    int chunk = 4;
+    std::vector<SysState> h = read_states(op, W.st);
+    auto running = [&](const std::vector<SysState>& hs) {
+        for (int i = 0; i < NS; ++i)
+            if ((mask & (1u << i)) && hs[i].status == kRunning) return true;
+        return false;
+    };
+    while (running(h) && launched < maxiter) {
+        const int todo = std::min(chunk, maxiter - launched);
+        for (int k = 0; k < todo; ++k) bicg_iteration<NS, V>(op, W, w, x);
+        HIPCHK(hipGetLastError());
+        launched += todo;
+        chunk = std::min(chunk * 2, 32);
+        h = read_states(op, W.st);
+    }
+    BicgOutcome o{};
+    for (int i = 0; i < NS; ++i) {
+        o.iterations[i] = h[i].iter;
+        o.residual[i] = h[i].residual;
+        o.status[i] = h[i].status;
+        o.info[i] = h[i].info;
+    }
+    return o;
+}
+
+// divcorr_poisson_ on the masked systems of the FP64 field E (device order).
+template <int NS>
+void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, unsigned mask,
+             const m3d_solve_params& prm) {
+    if (!op.dc_set || mask == 0) return;
+    ensure_dc_vectors<NS>(op, W);
+    hipStream_t s = op.ctx->stream;
+    using L = Launch64<NS>;
+    const int cg_max = std::max(500, op.nn);
+    launch_dc_begin(s, W.st.p, mask);
+    for (int it = 0; it < prm.div_corr_iters; ++it) {
+        L::dc_divj(s, op.nn, it, prm.div_corr_tol, cg_max, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p,
+                   op.ms.p, op.dc_bc_node.p, op.l_diag.p, E, phi0, W.dr.p, W.dz.p, W.dp.p, W.dphi.p,
+                   W.st.p, W.part.p, W.ctr.p);
+        HIPCHK(hipGetLastError());
+        std::vector<SysState> h = read_states(op, W.st);
+        bool any_dc = false, any_pcg = false;
+        for (int i = 0; i < NS; ++i) {
+            if (h[i].dc_status == kRunning) any_dc = true;
+            if (h[i].pcg_status == kRunning) any_pcg = true;
+        }
+        if (!any_dc) break;
+        int chunk = 8;
+        while (any_pcg) {
+            for (int k = 0; k < chunk; ++k) {
+                L::dc_lp(s, op.nn, op.l_ptr.p, op.l_colidx.p, op.l_vals.p, W.dp.p, W.dap.p, W.st.p,
+                         W.part.p, W.ctr.p);
+                L::dc_update(s, op.nn, op.l_diag.p, W.dp.p, W.dap.p, W.dphi.p, W.dr.p, W.dz.p, W.st.p,
+                             W.part.p, W.ctr.p);
+                L::dc_p(s, op.nn, W.dz.p, W.dp.p, W.st.p);
+            }
+            HIPCHK(hipGetLastError());
+            chunk = std::min(chunk * 2, 64);
+            h = read_states(op, W.st);
+            any_pcg = false;
+            for (int i = 0; i < NS; ++i)
+                if (h[i].pcg_status == kRunning) any_pcg = true;
+        }
+        L::dc_correct(s, op.npad, op.g_col.p, op.g_val.p, op.dc_bc_edge.p, W.dphi.p, E, W.st.p);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+template <int NS>
+std::vector<double> true_residual(m3d_operator& op, Work64<NS>& W, const double2* x,
+                                  const double2* b, double2* r) {
+    Launch64<NS>::true_res(op.ctx->stream, op.view<double>(), W.omega.p, x, b, r, W.st.p, W.part.p,
+                           W.ctr.p);
+    HIPCHK(hipGetLastError());
+    std::vector<SysState> h = read_states(op, W.st);
+    std::vector<double> out(NS);
+    for (int i = 0; i < NS; ++i) out[i] = h[i].true_res;
+    return out;
+}
+
+template <int NS>
+void interp_all(m3d_operator& op, double2* x) {
+    for (size_t l = 0; l + 1 < op.hang_level_start.size(); ++l) {
+        const int a = op.hang_level_start[l], b = op.hang_level_start[l + 1];
+        Launch<NS, double>::interp_hanging(op.ctx->stream, b - a, op.hang.p + 3 * size_t(a), x);
+    }
+}
+
+void fill_result(m3d_solve_result& r, int iters, double res, int status, int info) {
+    r.iterations = iters;
+    r.residual = res;
+    r.converged = (status == kConverged || status == kConvergedEarly) ? 1 : 0;
+    r.info = info;
+}
+
+// FP64 BiCGStab, or complex64 BiCGStab inside FP64 iterative refinement, on
+// the FP64 system vectors W.b / W.x.  Returns per-system outcome.
+template <int NS>
+BicgOutcome solve_linear(m3d_operator& op, Work64<NS>& W, const double2* b, double2* x,
+                         unsigned mask, int maxiter, const m3d_solve_params& prm, bool x_zero,
+                         int* outer_steps) {
+    if (prm.precision != M3D_PREC_MIXED) {
+        auto& w = vec64<NS>(op, W);
+        *outer_steps = 1;
+        return bicgstab<NS, double>(op, W, w, b, x, mask, maxiter, prm.tol, x_zero);
+    }
+    // Iterative refinement: r = b - A x in FP64, A d = r in complex64 to a
+    // loose relative tolerance, x += d; stop when ||b - A x|| / ||b|| < tol.
+    auto& w = vec32<NS>(op, W);
+    hipStream_t s = op.ctx->stream;
+    const size_t nv = size_t(op.npad) * NS;
+    if (x_zero) Launch<NS, double>::fill_zero(s, nv, x);
+    BicgOutcome total{};
+    unsigned active = mask;
+    const double inner_tol = prm.inner_tol > 0 ? prm.inner_tol : 1e-4;
+    int steps = 0;
+    std::vector<double> scale(NS, 1.0);
+    for (int step = 0; step < 30 && active; ++step) {
+        std::vector<double> tr = true_residual<NS>(op, W, x, b, W.rres.p);
+        std::vector<SysState> h = read_states(op, W.st);
+        for (int i = 0; i < NS; ++i) {
+            if (!(active & (1u << i))) continue;
+            total.residual[i] = tr[i];
+            if (tr[i] < prm.tol) {
+                active &= ~(1u << i);
+                total.status[i] = kConverged;
+                total.info[i] = 0;
+            }
+        }
+        if (!active) break;
+        // Scale each correction system to unit right-hand side before FP32.
+        for (int i = 0; i < NS; ++i) {
+            const double rn = tr[i] * h[i].true_bnorm;
+            scale[i] = rn > 0 ? rn : 1.0;
+        }
+        std::vector<double> inv(NS);
+        for (int i = 0; i < NS; ++i) inv[i] = 1.0 / scale[i];
+        HIPCHK(hipMemcpyAsync(W.scale.p, inv.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
+        Launch64<NS>::demote(s, nv, W.rres.p, W.scale.p, w.b.p);
+        Launch<NS, float>::fill_zero(s, nv, w.x.p);
+        const int left = maxiter - 0;
+        BicgOutcome o = bicgstab<NS, float>(op, W, w, w.b.p, w.x.p, active, left, inner_tol, true);
+        HIPCHK(hipMemcpyAsync(W.scale.p, scale.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
+        Launch64<NS>::add_scaled(s, nv, w.x.p, W.scale.p, x);
+        for (int i = 0; i < NS; ++i)
+            if (active & (1u << i)) total.iterations[i] += o.iterations[i];
+        ++steps;
+        // A correction solve that made no progress ends refinement for it.
+        for (int i = 0; i < NS; ++i) {
+            if (!(active & (1u << i))) continue;
+            if (o.status[i] == kBreakdown && o.iterations[i] <= 1) {
+                active &= ~(1u << i);
+                total.status[i] = kBreakdown;
+                total.info[i] = 2;
+            }
+        }
+    }
+    std::vector<double> tr = true_residual<NS>(op, W, x, b, W.rres.p);
+    for (int i = 0; i < NS; ++i) {
+        if (!(mask & (1u << i))) continue;
+        total.residual[i] = tr[i];
+        if (active & (1u << i)) {
+            total.status[i] = tr[i] < prm.tol ? kConverged : kMaxIter;
+            total.info[i] = tr[i] < prm.tol ? 0 : 3;
+        }
+    }
+    *outer_steps = steps;
+    return total;
+}
+
+template <int NS>
+void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const double2* d_rhs,
+                const double2* d_aux, double2* d_x, const m3d_solve_params& prm,
+                m3d_solve_result* res) {
+    Work64<NS>& W = ready_workspace<NS>(op);
+    hipStream_t s = op.ctx->stream;
+    std::vector<double> om(NS, 0.0);
+    for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
+    const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
+    using L = Launch<NS, double>;
+    using L64 = Launch64<NS>;
+    const size_t nv = size_t(op.npad) * NS;
+    L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_rhs, W.b.p);
+    for (int i = 0; i < nsys; ++i) res[i] = m3d_solve_result{};
+
+    const bool dc = prm.div_correction && op.dc_set;
+    if (mode == M3D_MODE_FORWARD) {
+        L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_aux, W.aux.p);
+        L::spmv(s, op.view<double>(), W.omega.p, W.aux.p, W.b.p, W.tmp.p);  // rhs_sec
+        L::fill_zero(s, nv, W.x.p);
+        int steps = 1;
+        BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, mask, prm.maxiter, prm, true, &steps);
+        interp_all<NS>(op, W.x.p);
+        L64::forward_total(s, op.npad, op.bc.p, W.aux.p, W.b.p, W.x.p);
+        if (dc) divcorr<NS>(op, W, W.x.p, nullptr, mask, prm);
+        for (int i = 0; i < nsys; ++i) {
+            fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
+            res[i].outer = steps;
+        }
+    } else if (mode == M3D_MODE_INNER || mode == M3D_MODE_SYSTEM) {
+        L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
+        int steps = 1;
+        BicgOutcome o = solve_linear<NS>(op, W, W.b.p, W.x.p, mask, prm.maxiter, prm, false, &steps);
+        if (mode == M3D_MODE_INNER) interp_all<NS>(op, W.x.p);
+        std::vector<int> zero(NS, 0);
+        bool any = false;
+        for (int i = 0; i < nsys; ++i) {
+            fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
+            res[i].outer = steps;
+            if (mode == M3D_MODE_SYSTEM && !res[i].converged && res[i].residual > 1.0) {
+                zero[i] = 1;
+                res[i].diverged = 1;
+                any = true;
+            }
+        }
+        if (any) {
+            HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int), hipMemcpyHostToDevice, s));
+            L64::zero_systems(s, op.npad, W.flags.p, W.x.p);
+        }
+    } else if (mode == M3D_MODE_INTERLEAVED) {
+        L::fill_zero(s, nv, W.x.p);
+        bool use_dc = dc;
+        for (int i = 0; i < nsys; ++i)
+            if (!(omega[i] > 0)) use_dc = false;
+        if (!use_dc) {
+            // solve_system_ path (forward_solver_fv.cpp:921-923).
+            int steps = 1;
+            BicgOutcome o = solve_linear<NS>(op, W, W.b.p, W.x.p, mask, prm.maxiter, prm, true, &steps);
+            std::vector<int> zero(NS, 0);
+            bool any = false;
+            for (int i = 0; i < nsys; ++i) {
+                fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
+                res[i].outer = steps;
+                if (!res[i].converged && res[i].residual > 1.0) {
+                    zero[i] = 1; res[i].diverged = 1; any = true;
+                }
+            }
+            if (any) {
+                HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int), hipMemcpyHostToDevice, s));
+                L64::zero_systems(s, op.npad, W.flags.p, W.x.p);
+            }
+        } else {
+            ensure_dc_vectors<NS>(op, W);
+            L64::phi0(s, op.nn, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p, op.dc_dead.p, W.omega.p, W.b.p,
+                      W.phi0.p);
+            divcorr<NS>(op, W, W.x.p, W.phi0.p, mask, prm);
+            unsigned active = mask;
+            std::vector<double> prev(NS, 1e30);
+            std::vector<int> zero(NS, 0);
+            for (int outer = 0; outer < prm.divcorr_outer_max && active; ++outer) {
+                int steps = 1;
+                BicgOutcome o = solve_linear<NS>(op, W, W.b.p, W.x.p, active, prm.divcorr_iter_per_dc,
+                                                 prm, false, &steps);
+                bool any_zero = false;
+                for (int i = 0; i < nsys; ++i) {
+                    if (!(active & (1u << i))) continue;
+                    res[i].iterations += o.iterations[i];
+                    res[i].residual = o.residual[i];
+                    res[i].info = o.info[i];
+                    res[i].outer = outer + 1;
+                    if (o.residual[i] > 1.0) {
+                        std::fill(zero.begin(), zero.end(), 0);
+                        zero[i] = 1;
+                        res[i].diverged = 1;
+                        active &= ~(1u << i);
+                        any_zero = true;
+                        HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int),
+                                              hipMemcpyHostToDevice, s));
+                        L64::zero_systems(s, op.npad, W.flags.p, W.x.p);
+                        HIPCHK(hipStreamSynchronize(s));
+                    } else if (o.status[i] == kConverged || o.status[i] == kConvergedEarly) {
+                        res[i].converged = 1;
+                        active &= ~(1u << i);
+                    }
+                }
+                (void)any_zero;
+                if (!active) break;
+                divcorr<NS>(op, W, W.x.p, W.phi0.p, active, prm);
+                std::vector<double> tr = true_residual<NS>(op, W, W.x.p, W.b.p, nullptr);
+                for (int i = 0; i < nsys; ++i) {
+                    if (!(active & (1u << i))) continue;
+                    if (tr[i] < prm.tol) {
+                        res[i].converged = 1;
+                        active &= ~(1u << i);
+                    } else if (outer > 0 && tr[i] > 0.95 * prev[i]) {
+                        active &= ~(1u << i);  // stagnation: accept (:866-873)
+                    } else {
+                        prev[i] = tr[i];
+                    }
+                }
+            }
+        }
+    } else {
+        throw Fail(M3D_ERR_ARG, "unknown solve mode");
+    }
+    std::vector<double> tr = true_residual<NS>(op, W, W.x.p, W.b.p, nullptr);
+    for (int i = 0; i < nsys; ++i) res[i].true_residual = tr[i];
+    L::unpack(s, op.npad, op.n, nsys, op.d_perm.p, W.x.p, d_x);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+void check_ready(const m3d_operator* op) {
+    if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+    if (!op->mass_set) throw Fail(M3D_ERR_STATE, "m3d_operator_set_mass has not been called");
+}
+
+void check_params(const m3d_solve_params* p) {
+    if (!p) throw Fail(M3D_ERR_ARG, "null params");
+    if (!(p->tol > 0) || p->maxiter <= 0) throw Fail(M3D_ERR_ARG, "tol and maxiter must be positive");
+    if (p->div_correction && (p->div_corr_iters < 0 || p->divcorr_outer_max < 0 || p->divcorr_iter_per_dc <= 0))
+        throw Fail(M3D_ERR_ARG, "invalid DivCorr parameters");
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return M3D_OK;
+    } catch (const Fail& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return M3D_ERR_ARG;
+    }
+}
+
+// Dispatch a batch of any size onto the fixed batch widths (<= 32 per launch).
+void solve_dispatch(m3d_operator* op, int nsys, int mode, const double* omega, const double2* d_rhs,
+                    const double2* d_aux, double2* d_x, const m3d_solve_params& prm,
+                    m3d_solve_result* res) {
+    for (int s0 = 0; s0 < nsys; s0 += kMaxSys) {
+        const int k = std::min(kMaxSys, nsys - s0);
+        const size_t off = size_t(s0) * op->n;
+        const double2* rb = d_rhs + off;
+        const double2* ab = d_aux ? d_aux + off : nullptr;
+        double2* xb = d_x + off;
+        if (k == 1) solve_impl<1>(*op, k, mode, omega + s0, rb, ab, xb, prm, res + s0);
+        else if (k == 2) solve_impl<2>(*op, k, mode, omega + s0, rb, ab, xb, prm, res + s0);
+        else if (k <= 4) solve_impl<4>(*op, k, mode, omega + s0, rb, ab, xb, prm, res + s0);
+        else if (k <= 8) solve_impl<8>(*op, k, mode, omega + s0, rb, ab, xb, prm, res + s0);
+        else if (k <= 16) solve_impl<16>(*op, k, mode, omega + s0, rb, ab, xb, prm, res + s0);
+        else solve_impl<32>(*op, k, mode, omega + s0, rb, ab, xb, prm, res + s0);
+    }
+}
+
+// Greedy first-fit colouring of the coupling graph in caller order.
+std::vector<int> colour_rows(int n, const int* ptr, const int* col, const unsigned char* constrained,
+                             int* ncolours) {
+    std::vector<int> c(n, -1), stamp(64, -1);
+    int nc = 1;
+    for (int i = 0; i < n; ++i) {
+        if (constrained[i] || ptr[i + 1] == ptr[i]) { c[i] = 0; continue; }
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+            const int cj = c[col[k]];
+            if (cj >= 0) {
+                if (cj >= int(stamp.size())) stamp.resize(cj + 1, -1);
+                stamp[cj] = i;
+            }
+        }
+        int pick = 0;
+        while (pick < int(stamp.size()) && stamp[pick] == i) ++pick;
+        if (pick >= int(stamp.size())) stamp.resize(pick + 1, -1);
+        c[i] = pick;
+        nc = std::max(nc, pick + 1);
+    }
+    *ncolours = nc;
+    return c;
+}
+
+void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
+    const int n = d.n;
+    hipStream_t s = op.ctx->stream;
+    op.n = n;
+    op.h_constrained.assign(d.constrained, d.constrained + n);
+    int K = 0;
+    std::vector<int> colour = colour_rows(n, d.rowptr, d.colidx, d.constrained, &K);
+    op.ncolours = K;
+    std::vector<int> count(K, 0);
+    for (int i = 0; i < n; ++i) count[colour[i]]++;
+    op.colour_start.assign(K + 1, 0);
+    for (int c = 0; c < K; ++c) op.colour_start[c + 1] = op.colour_start[c] + ((count[c] + kSlice - 1) / kSlice) * kSlice;
+    op.npad = op.colour_start[K];
+    op.perm.assign(op.npad, -1);
+    op.iperm.assign(n, -1);
+    std::vector<int> fill(op.colour_start.begin(), op.colour_start.end() - 1);
+    for (int i = 0; i < n; ++i) {
+        const int r = fill[colour[i]]++;
+        op.perm[r] = i;
+        op.iperm[i] = r;
+    }
+    // SELL-64 lower / upper blocks.
+    const int nsl = op.npad / kSlice;
+    std::vector<int> loff(nsl), lw(nsl), uoff(nsl), uw(nsl);
+    long long ltot = 0, utot = 0;
+    for (int sl = 0; sl < nsl; ++sl) {
+        int ml = 0, mu = 0;
+        for (int l = 0; l < kSlice; ++l) {
+            const int e = op.perm[sl * kSlice + l];
+            if (e < 0) continue;
+            int cl = 0, cu = 0;
+            for (int k = d.rowptr[e]; k < d.rowptr[e + 1]; ++k)
+                (op.iperm[d.colidx[k]] < sl * kSlice + l ? cl : cu)++;
+            ml = std::max(ml, cl);
+            mu = std::max(mu, cu);
+        }
+        loff[sl] = int(ltot); lw[sl] = ml; ltot += (long long)ml * kSlice;
+        uoff[sl] = int(utot); uw[sl] = mu; utot += (long long)mu * kSlice;
+    }
+    if (ltot > 2147483647LL || utot > 2147483647LL) throw Fail(M3D_ERR_ARG, "operator too large for int32 offsets");
+    std::vector<double> lval(ltot, 0.0), uval(utot, 0.0);
+    std::vector<int> lcol(ltot), ucol(utot);
+    long long nz = 0;
+    for (int sl = 0; sl < nsl; ++sl) {
+        for (int l = 0; l < kSlice; ++l) {
+            const int row = sl * kSlice + l;
+            for (int k = 0; k < lw[sl]; ++k) lcol[loff[sl] + k * kSlice + l] = row;
+            for (int k = 0; k < uw[sl]; ++k) ucol[uoff[sl] + k * kSlice + l] = row;
+            const int e = op.perm[row];
+            if (e < 0) continue;
+            int kl = 0, ku = 0;
+            for (int k = d.rowptr[e]; k < d.rowptr[e + 1]; ++k) {
+                const int j = op.iperm[d.colidx[k]];
+                if (j < row) {
+                    lval[loff[sl] + kl * kSlice + l] = d.values[k];
+                    lcol[loff[sl] + kl * kSlice + l] = j;
+                    ++kl;
+                } else {
+                    uval[uoff[sl] + ku * kSlice + l] = d.values[k];
+                    ucol[uoff[sl] + ku * kSlice + l] = j;
+                    ++ku;
+                }
+                ++nz;
+            }
+        }
+    }
+    op.stored = ltot + utot;
+    op.nnz = nz;
+    std::vector<double> ds(op.npad, 1.0);
+    std::vector<unsigned char> bc(op.npad, 1), cons(op.npad, 1);
+    for (int r = 0; r < op.npad; ++r) {
+        const int e = op.perm[r];
+        if (e < 0) continue;
+        ds[r] = d.constrained[e] ? 1.0 : d.diag_s[e];
+        bc[r] = d.bc[e];
+        cons[r] = d.constrained[e];
+    }
+    op.d_perm.upload(op.perm, s);
+    op.l_off.upload(loff, s); op.l_w.upload(lw, s); op.u_off.upload(uoff, s); op.u_w.upload(uw, s);
+    op.l_val.upload(lval, s); op.u_val.upload(uval, s); op.l_col.upload(lcol, s); op.u_col.upload(ucol, s);
+    op.diag_s.upload(ds, s);
+    op.bc.upload(bc, s);
+    op.constrained.upload(cons, s);
+    std::vector<float> lval32(lval.begin(), lval.end()), uval32(uval.begin(), uval.end()), ds32(ds.begin(), ds.end());
+    op.l_val32.upload(lval32, s); op.u_val32.upload(uval32, s); op.diag_s32.upload(ds32, s);
+
+    // Hanging pairs in dependency levels that keep the sequential semantics
+    // of the caller's list order.
+    std::vector<int> lastw(n, -1), maxr(n, -1), level(d.n_hanging, 0);
+    int nlev = 0;
+    for (int p = 0; p < d.n_hanging; ++p) {
+        const int c = d.hanging[3 * p], a = d.hanging[3 * p + 1], b = d.hanging[3 * p + 2];
+        if (c < 0 || c >= n || a < 0 || a >= n || b < 0 || b >= n) throw Fail(M3D_ERR_ARG, "hanging index out of range");
+        int lv = 0;
+        lv = std::max(lv, lastw[a] + 1);
+        lv = std::max(lv, lastw[b] + 1);
+        lv = std::max(lv, maxr[c] + 1);
+        lv = std::max(lv, lastw[c] + 1);
+        level[p] = lv;
+        maxr[a] = std::max(maxr[a], lv);
+        maxr[b] = std::max(maxr[b], lv);
+        lastw[c] = lv;
+        nlev = std::max(nlev, lv + 1);
+    }
+    std::vector<int> tri;
+    op.hang_level_start.assign(1, 0);
+    for (int l = 0; l < nlev; ++l) {
+        for (int p = 0; p < d.n_hanging; ++p) {
+            if (level[p] != l) continue;
+            tri.push_back(op.iperm[d.hanging[3 * p]]);
+            tri.push_back(op.iperm[d.hanging[3 * p + 1]]);
+            tri.push_back(op.iperm[d.hanging[3 * p + 2]]);
+        }
+        op.hang_level_start.push_back(int(tri.size() / 3));
+    }
+    op.hang.upload(tri, s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+void set_mass(m3d_operator& op, const double* me) {
+    std::vector<double> m(op.npad, 0.0);
+    for (int r = 0; r < op.npad; ++r) {
+        const int e = op.perm[r];
+        if (e >= 0 && !op.h_constrained[e]) m[r] = me[e];
+    }
+    op.me.upload(m, op.ctx->stream);
+    std::vector<float> m32(m.begin(), m.end());
+    op.me32.upload(m32, op.ctx->stream);
+    HIPCHK(hipStreamSynchronize(op.ctx->stream));
+    op.mass_set = true;
+}
+
+void set_divcorr(m3d_operator& op, const m3d_divcorr_desc& d) {
+    const int n = op.n, nn = d.nn;
+    hipStream_t s = op.ctx->stream;
+    if (nn <= 0) throw Fail(M3D_ERR_ARG, "DivCorr needs at least one node");
+    // Node -> incident edges in ascending caller edge order.
+    std::vector<int> ptr(nn + 1, 0);
+    for (int e = 0; e < n; ++e)
+        for (int k = 2 * e; k < 2 * e + 2; ++k) {
+            const int v = d.g_col[k];
+            if (v < 0 || v >= nn) throw Fail(M3D_ERR_ARG, "G column out of range");
+            ptr[v + 1]++;
+        }
+    for (int v = 0; v < nn; ++v) ptr[v + 1] += ptr[v];
+    std::vector<int> edge(ptr[nn]);
+    std::vector<double> val(ptr[nn]);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < n; ++e)
+        for (int k = 2 * e; k < 2 * e + 2; ++k) {
+            const int v = d.g_col[k];
+            const int q = pos[v]++;
+            edge[q] = op.iperm[e];
+            val[q] = d.g_val[k];
+        }
+    std::vector<int> gcol(2 * size_t(op.npad), -1);
+    std::vector<double> gval(2 * size_t(op.npad), 0.0), ms(op.npad, 0.0);
+    std::vector<unsigned char> bce(op.npad, 1), dead(op.npad, 1);
+    for (int r = 0; r < op.npad; ++r) {
+        const int e = op.perm[r];
+        if (e < 0) continue;
+        gcol[2 * r] = d.g_col[2 * e]; gcol[2 * r + 1] = d.g_col[2 * e + 1];
+        gval[2 * r] = d.g_val[2 * e]; gval[2 * r + 1] = d.g_val[2 * e + 1];
+        ms[r] = d.dead_edge[e] ? 0.0 : d.me_sigma[e];
+        bce[r] = d.bc_edge[e];
+        dead[r] = d.dead_edge[e];
+    }
+    op.nn = nn;
+    op.nt_ptr.upload(ptr, s); op.nt_edge.upload(edge, s); op.nt_val.upload(val, s);
+    op.g_col.upload(gcol, s); op.g_val.upload(gval, s); op.ms.upload(ms, s);
+    op.dc_bc_edge.upload(bce, s); op.dc_dead.upload(dead, s);
+    op.dc_bc_node.upload(std::vector<unsigned char>(d.bc_node, d.bc_node + nn), s);
+    const int lnnz = d.L_rowptr[nn];
+    op.l_ptr.upload(std::vector<int>(d.L_rowptr, d.L_rowptr + nn + 1), s);
+    op.l_colidx.upload(std::vector<int>(d.L_colidx, d.L_colidx + lnnz), s);
+    op.l_vals.upload(std::vector<double>(d.L_values, d.L_values + lnnz), s);
+    op.l_diag.upload(std::vector<double>(d.L_diag, d.L_diag + nn), s);
+    HIPCHK(hipStreamSynchronize(s));
+    op.dc_set = true;
+}
+
+// Host staging for the host-memory entry points.
+struct Staging {
+    DBuf<double2> rhs, aux, x;
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* m3d_last_error(void) { return g_last_error.c_str(); }
+
+int m3d_context_create(int device, m3d_context** out) {
+    return guarded([&] {
+        if (!out) throw Fail(M3D_ERR_ARG, "null output pointer");
+        int count = 0;
+        HIPCHK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw Fail(M3D_ERR_ARG, "device index out of range");
+        HIPCHK(hipSetDevice(device));
+        auto* c = new m3d_context;
+        c->device = device;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            throw Fail(M3D_ERR_HIP, "hipStreamCreate failed");
+        }
+        *out = c;
+    });
+}
+
+int m3d_context_destroy(m3d_context* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+void* m3d_context_stream(m3d_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int m3d_operator_create(m3d_context* ctx, const m3d_operator_desc* desc, m3d_operator** out) {
+    return guarded([&] {
+        if (!ctx || !desc || !out) throw Fail(M3D_ERR_ARG, "null argument");
+        if (desc->n <= 0 || !desc->rowptr || !desc->colidx || !desc->values || !desc->diag_s ||
+            !desc->constrained || !desc->bc)
+            throw Fail(M3D_ERR_ARG, "incomplete operator description");
+        if (desc->n_hanging > 0 && !desc->hanging) throw Fail(M3D_ERR_ARG, "hanging triples missing");
+        const int nnz = desc->rowptr[desc->n];
+        for (int k = 0; k < nnz; ++k)
+            if (desc->colidx[k] < 0 || desc->colidx[k] >= desc->n) throw Fail(M3D_ERR_ARG, "column index out of range");
+        HIPCHK(hipSetDevice(ctx->device));
+        std::unique_ptr<m3d_operator> op(new m3d_operator);
+        op->ctx = ctx;
+        build_operator(*op, *desc);
+        *out = op.release();
+    });
+}
+
+int m3d_operator_destroy(m3d_operator* op) {
+    return guarded([&] {
+        if (!op) return;
+        (void)hipSetDevice(op->ctx->device);
+        (void)hipStreamSynchronize(op->ctx->stream);
+        delete op;
+    });
+}
+
+int m3d_operator_set_mass(m3d_operator* op, const double* me) {
+    return guarded([&] {
+        if (!op || !me) throw Fail(M3D_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        set_mass(*op, me);
+    });
+}
+
+int m3d_operator_set_divcorr(m3d_operator* op, const m3d_divcorr_desc* desc) {
+    return guarded([&] {
+        if (!op || !desc || !desc->g_col || !desc->g_val || !desc->me_sigma || !desc->bc_edge ||
+            !desc->dead_edge || !desc->bc_node || !desc->L_rowptr || !desc->L_colidx ||
+            !desc->L_values || !desc->L_diag)
+            throw Fail(M3D_ERR_ARG, "incomplete DivCorr description");
+        const int lnnz = desc->L_rowptr[desc->nn];
+        for (int k = 0; k < lnnz; ++k)
+            if (desc->L_colidx[k] < 0 || desc->L_colidx[k] >= desc->nn) throw Fail(M3D_ERR_ARG, "L column out of range");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        set_divcorr(*op, *desc);
+    });
+}
+
+int m3d_operator_permutation(m3d_operator* op, int* perm, int* colour_start) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        if (perm) std::copy(op->perm.begin(), op->perm.end(), perm);
+        if (colour_start) std::copy(op->colour_start.begin(), op->colour_start.end(), colour_start);
+    });
+}
+
+int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long* stored_entries,
+                      long long* nnz) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        if (n_padded) *n_padded = op->npad;
+        if (ncolours) *ncolours = op->ncolours;
+        if (stored_entries) *stored_entries = op->stored;
+        if (nnz) *nnz = op->nnz;
+    });
+}
+
+int m3d_solve_batch_device(m3d_operator* op, int nsys, int mode, const double* omega,
+                           const double* d_rhs, const double* d_aux, double* d_x,
+                           const m3d_solve_params* params, m3d_solve_result* results) {
+    return guarded([&] {
+        check_ready(op);
+        check_params(params);
+        if (nsys <= 0 || !omega || !d_rhs || !d_x || !results) throw Fail(M3D_ERR_ARG, "null argument");
+        if (mode == M3D_MODE_FORWARD && !d_aux) throw Fail(M3D_ERR_ARG, "FORWARD needs E_pri in aux");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        solve_dispatch(op, nsys, mode, omega, reinterpret_cast<const double2*>(d_rhs),
+                       reinterpret_cast<const double2*>(d_aux), reinterpret_cast<double2*>(d_x),
+                       *params, results);
+    });
+}
+
+int m3d_solve_batch(m3d_operator* op, int nsys, int mode, const double* omega, const double* rhs,
+                    const double* aux, double* x, const m3d_solve_params* params,
+                    m3d_solve_result* results) {
+    return guarded([&] {
+        check_ready(op);
+        check_params(params);
+        if (nsys <= 0 || !omega || !rhs || !x || !results) throw Fail(M3D_ERR_ARG, "null argument");
+        if (mode == M3D_MODE_FORWARD && !aux) throw Fail(M3D_ERR_ARG, "FORWARD needs E_pri in aux");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        const size_t bytes = size_t(nsys) * op->n * sizeof(double2);
+        Staging st;
+        st.rhs.alloc(size_t(nsys) * op->n);
+        st.x.alloc(size_t(nsys) * op->n);
+        HIPCHK(hipMemcpyAsync(st.rhs.p, rhs, bytes, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(st.x.p, x, bytes, hipMemcpyHostToDevice, s));
+        if (aux) {
+            st.aux.alloc(size_t(nsys) * op->n);
+            HIPCHK(hipMemcpyAsync(st.aux.p, aux, bytes, hipMemcpyHostToDevice, s));
+        }
+        solve_dispatch(op, nsys, mode, omega, st.rhs.p, aux ? st.aux.p : nullptr, st.x.p, *params, results);
+        HIPCHK(hipMemcpyAsync(x, st.x.p, bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+template <int NS>
+void spmv_impl(m3d_operator& op, int nsys, const double* omega, const double2* d_x, double2* d_y) {
+    Work64<NS>& W = ready_workspace<NS>(op);
+    hipStream_t s = op.ctx->stream;
+    std::vector<double> om(NS, 0.0);
+    for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
+    Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
+    Launch<NS, double>::spmv(s, op.view<double>(), W.omega.p, W.x.p, nullptr, W.tmp.p);
+    Launch<NS, double>::unpack(s, op.npad, op.n, nsys, op.d_perm.p, W.tmp.p, d_y);
+    HIPCHK(hipGetLastError());
+}
+
+template <int NS>
+void ssor_impl(m3d_operator& op, int nsys, const double* omega, const double2* d_r, double2* d_z) {
+    Work64<NS>& W = ready_workspace<NS>(op);
+    auto& w = vec64<NS>(op, W);
+    hipStream_t s = op.ctx->stream;
+    std::vector<double> om(NS, 0.0);
+    for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
+    const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
+    launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
+    Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_r, W.x.p);
+    const DevOp<double> A = op.view<double>();
+    for (int c = 0; c < op.ncolours; ++c)
+        Launch<NS, double>::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], false,
+                                      W.x.p, nullptr, nullptr, w.y.p, W.st.p);
+    for (int c = op.ncolours - 1; c >= 0; --c)
+        Launch<NS, double>::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p,
+                                      W.tmp.p, W.st.p);
+    Launch<NS, double>::unpack(s, op.npad, op.n, nsys, op.d_perm.p, W.tmp.p, d_z);
+    launch_bicg_begin(s, W.st.p, 0u, 1, 1.0);
+    HIPCHK(hipGetLastError());
+}
+
+template <int NS>
+void divcorr_host_impl(m3d_operator& op, int nsys, const double* omega, double2* d_E,
+                       const double2* d_phi0_sm, const m3d_solve_params& prm) {
+    Work64<NS>& W = ready_workspace<NS>(op);
+    ensure_dc_vectors<NS>(op, W);
+    hipStream_t s = op.ctx->stream;
+    std::vector<double> om(NS, 0.0);
+    for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
+    Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_E, W.x.p);
+    const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
+    divcorr<NS>(op, W, W.x.p, d_phi0_sm, mask, prm);
+    Launch<NS, double>::unpack(s, op.npad, op.n, nsys, op.d_perm.p, W.x.p, d_E);
+    HIPCHK(hipGetLastError());
+}
+
+// Average duration of one launch of a kernel class, timed with HIP events on
+// the context stream, plus its algorithmic HBM bytes (free rows n, stored
+// off-diagonal nonzeros m, k systems):
+//   SpMV        m*(8+4) + n*2*8 + 2*k*n*16
+//   SSOR apply  m*(8+4) + n*4*8 + 4*k*n*16   (lower half forward, upper half backward)
+template <int NS>
+void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int reps, double* avg_ms,
+                double* bytes) {
+    Work64<NS>& W = ready_workspace<NS>(op);
+    auto& w = vec64<NS>(op, W);
+    hipStream_t s = op.ctx->stream;
+    std::vector<double> om(NS, 0.0);
+    for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
+    const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
+    launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
+    const size_t nv = size_t(op.npad) * NS;
+    Launch<NS, double>::fill_zero(s, nv, W.x.p);
+    const DevOp<double> A = op.view<double>();
+    auto one = [&]() {
+        if (kind == 0) {
+            Launch<NS, double>::spmv(s, A, W.omega.p, W.x.p, nullptr, W.tmp.p);
+        } else {
+            for (int c = 0; c < op.ncolours; ++c)
+                Launch<NS, double>::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
+                                              false, W.x.p, nullptr, nullptr, w.y.p, W.st.p);
+            for (int c = op.ncolours - 1; c >= 0; --c)
+                Launch<NS, double>::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
+                                              w.y.p, W.tmp.p, W.st.p);
+        }
+    };
+    one();  // warm
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) one();
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    launch_bicg_begin(s, W.st.p, 0u, 1, 1.0);
+    HIPCHK(hipStreamSynchronize(s));
+    *avg_ms = double(ms) / reps;
+    long long nfree = 0;
+    for (int e = 0; e < op.n; ++e) nfree += op.h_constrained[e] ? 0 : 1;
+    const double m = double(op.nnz), n = double(nfree), k = double(nsys);
+    *bytes = kind == 0 ? m * 12.0 + n * 16.0 + 2.0 * k * n * 16.0
+                       : m * 12.0 + n * 32.0 + 4.0 * k * n * 16.0;
+}
+
+template <typename F>
+void for_width(int nsys, F&& f) {
+    if (nsys <= 0 || nsys > kMaxSys) throw Fail(M3D_ERR_ARG, "nsys must be in 1..32");
+    f(nsys);
+}
+
+}  // namespace
+
+extern "C" {
+
+int m3d_spmv(m3d_operator* op, int nsys, const double* omega, const double* x, double* y) {
+    return guarded([&] {
+        check_ready(op);
+        if (!omega || !x || !y) throw Fail(M3D_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        for_width(nsys, [&](int k) {
+            const size_t bytes = size_t(k) * op->n * sizeof(double2);
+            Staging st;
+            st.x.alloc(size_t(k) * op->n);
+            st.rhs.alloc(size_t(k) * op->n);
+            HIPCHK(hipMemcpyAsync(st.x.p, x, bytes, hipMemcpyHostToDevice, s));
+            if (k == 1) spmv_impl<1>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k == 2) spmv_impl<2>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k <= 4) spmv_impl<4>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k <= 8) spmv_impl<8>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k <= 16) spmv_impl<16>(*op, k, omega, st.x.p, st.rhs.p);
+            else spmv_impl<32>(*op, k, omega, st.x.p, st.rhs.p);
+            HIPCHK(hipMemcpyAsync(y, st.rhs.p, bytes, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        });
+    });
+}
+
+int m3d_ssor_apply(m3d_operator* op, int nsys, const double* omega, const double* r, double* z) {
+    return guarded([&] {
+        check_ready(op);
+        if (!omega || !r || !z) throw Fail(M3D_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        for_width(nsys, [&](int k) {
+            const size_t bytes = size_t(k) * op->n * sizeof(double2);
+            Staging st;
+            st.x.alloc(size_t(k) * op->n);
+            st.rhs.alloc(size_t(k) * op->n);
+            HIPCHK(hipMemcpyAsync(st.x.p, r, bytes, hipMemcpyHostToDevice, s));
+            if (k == 1) ssor_impl<1>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k == 2) ssor_impl<2>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k <= 4) ssor_impl<4>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k <= 8) ssor_impl<8>(*op, k, omega, st.x.p, st.rhs.p);
+            else if (k <= 16) ssor_impl<16>(*op, k, omega, st.x.p, st.rhs.p);
+            else ssor_impl<32>(*op, k, omega, st.x.p, st.rhs.p);
+            HIPCHK(hipMemcpyAsync(z, st.rhs.p, bytes, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        });
+    });
+}
+
+int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
+                     double* avg_ms, double* alg_bytes) {
+    return guarded([&] {
+        check_ready(op);
+        if (!omega || !avg_ms || !alg_bytes || reps <= 0 || (kind != 0 && kind != 1))
+            throw Fail(M3D_ERR_ARG, "invalid benchmark arguments");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        for_width(nsys, [&](int k) {
+            if (k == 1) bench_impl<1>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
+            else if (k == 2) bench_impl<2>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
+            else if (k <= 4) bench_impl<4>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
+            else if (k <= 8) bench_impl<8>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
+            else if (k <= 16) bench_impl<16>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
+            else bench_impl<32>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
+        });
+    });
+}
+
+int m3d_divcorr(m3d_operator* op, int nsys, const double* omega, double* E, const double* phi0,
+                const m3d_solve_params* params) {
+    return guarded([&] {
+        check_ready(op);
+        check_params(params);
+        if (!op->dc_set) throw Fail(M3D_ERR_STATE, "m3d_operator_set_divcorr has not been called");
+        if (!omega || !E) throw Fail(M3D_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        for_width(nsys, [&](int k) {
+            int NS = 1;
+            while (NS < k) NS *= 2;
+            const size_t bytes = size_t(k) * op->n * sizeof(double2);
+            Staging st;
+            st.x.alloc(size_t(k) * op->n);
+            HIPCHK(hipMemcpyAsync(st.x.p, E, bytes, hipMemcpyHostToDevice, s));
+            // phi0 to the system-minor node layout.
+            std::vector<double2> ph;
+            if (phi0) {
+                ph.assign(size_t(op->nn) * NS, make_double2(0.0, 0.0));
+                for (int sy = 0; sy < k; ++sy)
+                    for (int v = 0; v < op->nn; ++v)
+                        ph[size_t(v) * NS + sy] = make_double2(phi0[2 * (size_t(sy) * op->nn + v)],
+                                                               phi0[2 * (size_t(sy) * op->nn + v) + 1]);
+                st.aux.alloc(ph.size());
+                HIPCHK(hipMemcpyAsync(st.aux.p, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+            }
+            const double2* p0 = phi0 ? st.aux.p : nullptr;
+            if (NS == 1) divcorr_host_impl<1>(*op, k, omega, st.x.p, p0, *params);
+            else if (NS == 2) divcorr_host_impl<2>(*op, k, omega, st.x.p, p0, *params);
+            else if (NS == 4) divcorr_host_impl<4>(*op, k, omega, st.x.p, p0, *params);
+            else if (NS == 8) divcorr_host_impl<8>(*op, k, omega, st.x.p, p0, *params);
+            else if (NS == 16) divcorr_host_impl<16>(*op, k, omega, st.x.p, p0, *params);
+            else divcorr_host_impl<32>(*op, k, omega, st.x.p, p0, *params);
+            HIPCHK(hipMemcpyAsync(E, st.x.p, bytes, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        });
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,531 @@
+"""ctypes bindings of ``lib/libm3dmt.so`` (include/m3d_device.h, include/m3d_host.h).
+
+The library is loaded from the package's ``lib/`` folder; nothing here falls
+back to another implementation: a missing or stale library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libm3dmt.so")
+
+MODE_INNER, MODE_FORWARD, MODE_SYSTEM, MODE_INTERLEAVED = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+
+_lib: Optional[C.CDLL] = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+class MeshParamsC(C.Structure):
+    _fields_ = [
+        ("domain", C.c_double * 6),
+        ("min_level", C.c_int), ("max_level", C.c_int),
+        ("station_refine_radius", C.c_double),
+        ("station_refine_level", C.c_int),
+        ("sigma_bg", C.c_double),
+        ("max_interest_depth", C.c_double),
+        ("air_surface_thickness", C.c_double),
+        ("air_mid_thickness", C.c_double),
+        ("air_upper_thickness", C.c_double),
+        ("transition_padding", C.c_double),
+        ("station_bbox", C.c_double * 4),
+        ("n_regions", C.c_int),
+        ("regions", C.POINTER(C.c_double)),
+        ("flat_terrain", C.c_int),
+    ]
+
+
+class ForwardParamsC(C.Structure):
+    _fields_ = [
+        ("bicgstab_tol", C.c_double),
+        ("bicgstab_maxiter", C.c_int),
+        ("air_z_threshold", C.c_double),
+        ("div_correction", C.c_int),
+        ("div_corr_iters", C.c_int),
+        ("div_corr_tol", C.c_double),
+        ("divcorr_outer_max", C.c_int),
+        ("divcorr_iter_per_dc", C.c_int),
+    ]
+
+
+class SolveParamsC(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double),
+        ("maxiter", C.c_int),
+        ("div_correction", C.c_int),
+        ("div_corr_iters", C.c_int),
+        ("div_corr_tol", C.c_double),
+        ("divcorr_outer_max", C.c_int),
+        ("divcorr_iter_per_dc", C.c_int),
+        ("precision", C.c_int),
+        ("inner_tol", C.c_double),
+    ]
+
+
+class SolveResultC(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("residual", C.c_double),
+        ("true_residual", C.c_double),
+        ("converged", C.c_int),
+        ("diverged", C.c_int),
+        ("info", C.c_int),
+        ("outer", C.c_int),
+    ]
+
+
+class OperatorDescC(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("rowptr", C.POINTER(C.c_int)),
+        ("colidx", C.POINTER(C.c_int)),
+        ("values", C.POINTER(C.c_double)),
+        ("diag_s", C.POINTER(C.c_double)),
+        ("constrained", C.POINTER(C.c_ubyte)),
+        ("bc", C.POINTER(C.c_ubyte)),
+        ("n_hanging", C.c_int),
+        ("hanging", C.POINTER(C.c_int)),
+    ]
+
+
+class DivcorrDescC(C.Structure):
+    _fields_ = [
+        ("nn", C.c_int),
+        ("g_col", C.POINTER(C.c_int)),
+        ("g_val", C.POINTER(C.c_double)),
+        ("me_sigma", C.POINTER(C.c_double)),
+        ("bc_edge", C.POINTER(C.c_ubyte)),
+        ("dead_edge", C.POINTER(C.c_ubyte)),
+        ("bc_node", C.POINTER(C.c_ubyte)),
+        ("L_rowptr", C.POINTER(C.c_int)),
+        ("L_colidx", C.POINTER(C.c_int)),
+        ("L_values", C.POINTER(C.c_double)),
+        ("L_diag", C.POINTER(C.c_double)),
+    ]
+
+
+# Every symbol the two public headers declare.
+DEVICE_SYMBOLS = [
+    "m3d_context_create", "m3d_context_destroy", "m3d_last_error", "m3d_operator_create",
+    "m3d_operator_destroy", "m3d_operator_set_mass", "m3d_operator_set_divcorr", "m3d_solve_batch",
+    "m3d_solve_batch_device", "m3d_divcorr", "m3d_spmv", "m3d_ssor_apply",
+    "m3d_operator_permutation", "m3d_operator_info", "m3d_context_stream", "m3d_bench_kernel",
+]
+HOST_SYMBOLS = [
+    "m3dh_last_error", "m3dh_model_create", "m3dh_model_destroy", "m3dh_get_array",
+    "m3dh_get_count", "m3dh_set_sigma", "m3dh_find_stations", "m3dh_primary_field",
+    "m3dh_bc_values", "m3dh_rhs_total", "m3dh_phi0", "m3dh_extract_impedance", "m3dh_adjoint_rhs",
+    "m3dh_sensitivity", "m3dh_solve_frequencies", "m3dh_adjoint_solve", "m3dh_device_operator",
+]
+
+
+def lib() -> C.CDLL:
+    """The loaded native library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError(f"{LIB_PATH} is missing: run maple3dmt_amd.build.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, p = C.c_void_p, C.c_int, C.c_double, C.POINTER
+    L.m3d_last_error.restype = C.c_char_p
+    L.m3dh_last_error.restype = C.c_char_p
+    L.m3d_context_create.argtypes = [i, p(vp)]
+    L.m3d_context_destroy.argtypes = [vp]
+    L.m3d_context_stream.argtypes = [vp]
+    L.m3d_context_stream.restype = vp
+    L.m3d_operator_create.argtypes = [vp, p(OperatorDescC), p(vp)]
+    L.m3d_operator_destroy.argtypes = [vp]
+    L.m3d_operator_set_mass.argtypes = [vp, vp]
+    L.m3d_operator_set_divcorr.argtypes = [vp, p(DivcorrDescC)]
+    L.m3d_solve_batch.argtypes = [vp, i, i, vp, vp, vp, vp, p(SolveParamsC), p(SolveResultC)]
+    L.m3d_solve_batch_device.argtypes = [vp, i, i, vp, vp, vp, vp, p(SolveParamsC), p(SolveResultC)]
+    L.m3d_divcorr.argtypes = [vp, i, vp, vp, vp, p(SolveParamsC)]
+    L.m3d_spmv.argtypes = [vp, i, vp, vp, vp]
+    L.m3d_ssor_apply.argtypes = [vp, i, vp, vp, vp]
+    L.m3d_operator_permutation.argtypes = [vp, vp, vp]
+    L.m3d_operator_info.argtypes = [vp, p(i), p(i), p(C.c_longlong), p(C.c_longlong)]
+    L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d)]
+    L.m3dh_model_create.argtypes = [p(MeshParamsC), vp, i, vp, i, p(ForwardParamsC), vp, p(vp)]
+    L.m3dh_model_destroy.argtypes = [vp]
+    L.m3dh_get_array.argtypes = [vp, C.c_char_p, p(i), p(C.c_longlong), p(vp)]
+    L.m3dh_get_count.argtypes = [vp, C.c_char_p]
+    L.m3dh_get_count.restype = C.c_longlong
+    L.m3dh_set_sigma.argtypes = [vp, vp]
+    L.m3dh_find_stations.argtypes = [vp, vp, i]
+    L.m3dh_primary_field.argtypes = [vp, d, i, vp]
+    L.m3dh_bc_values.argtypes = [vp, d, i, vp]
+    L.m3dh_rhs_total.argtypes = [vp, vp, vp]
+    L.m3dh_phi0.argtypes = [vp, d, vp, vp]
+    L.m3dh_extract_impedance.argtypes = [vp, d, vp, vp, vp]
+    L.m3dh_adjoint_rhs.argtypes = [vp, d, vp, vp, vp]
+    L.m3dh_sensitivity.argtypes = [vp, d, vp, vp, vp]
+    L.m3dh_solve_frequencies.argtypes = [vp, vp, i, p(SolveParamsC), vp, vp, p(SolveResultC)]
+    L.m3dh_adjoint_solve.argtypes = [vp, vp, i, vp, p(SolveParamsC), vp, p(SolveResultC)]
+    L.m3dh_device_operator.argtypes = [vp]
+    L.m3dh_device_operator.restype = vp
+    _lib = L
+    return L
+
+
+def _check_dev(status: int) -> None:
+    if status != 0:
+        raise NativeError(lib().m3d_last_error().decode())
+
+
+def _check_host(status: int) -> None:
+    if status != 0:
+        raise NativeError(lib().m3dh_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _as_c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+# --------------------------------------------------------------------------
+# Parameters
+# --------------------------------------------------------------------------
+@dataclass
+class MeshParams:
+    """RefinementParams (octree_mesh.h:37-90) with the reference defaults."""
+    domain: Sequence[float]
+    min_level: int = 3
+    max_level: int = 8
+    station_refine_radius: float = 5000.0
+    station_refine_level: int = 7
+    sigma_bg: float = 0.01
+    max_interest_depth: float = 0.0
+    air_surface_thickness: float = 2000.0
+    air_mid_thickness: float = 8000.0
+    air_upper_thickness: float = 20000.0
+    transition_padding: float = 0.0
+    station_bbox: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    regions: Sequence[Sequence[float]] = field(default_factory=list)
+    flat_terrain: bool = True
+
+
+@dataclass
+class ForwardParams:
+    """ForwardParamsFV (forward_solver_fv.h:31-69) with the reference defaults."""
+    bicgstab_tol: float = 1e-7
+    bicgstab_maxiter: int = 2000
+    air_z_threshold: float = 1.0
+    div_correction: bool = True
+    div_corr_iters: int = 10
+    div_corr_tol: float = 2e-3
+    divcorr_outer_max: int = 20
+    divcorr_iter_per_dc: int = 500
+
+    def solve_params(self, precision: int = PREC_FP64, inner_tol: float = 0.0,
+                     tol: Optional[float] = None) -> "SolveParams":
+        return SolveParams(tol=self.bicgstab_tol if tol is None else tol,
+                           maxiter=self.bicgstab_maxiter, div_correction=self.div_correction,
+                           div_corr_iters=self.div_corr_iters, div_corr_tol=self.div_corr_tol,
+                           divcorr_outer_max=self.divcorr_outer_max,
+                           divcorr_iter_per_dc=self.divcorr_iter_per_dc, precision=precision,
+                           inner_tol=inner_tol)
+
+
+@dataclass
+class SolveParams:
+    tol: float = 1e-7
+    maxiter: int = 2000
+    div_correction: bool = True
+    div_corr_iters: int = 10
+    div_corr_tol: float = 2e-3
+    divcorr_outer_max: int = 20
+    divcorr_iter_per_dc: int = 500
+    precision: int = PREC_FP64
+    inner_tol: float = 0.0
+
+    def c(self) -> SolveParamsC:
+        return SolveParamsC(self.tol, self.maxiter, int(self.div_correction), self.div_corr_iters,
+                            self.div_corr_tol, self.divcorr_outer_max, self.divcorr_iter_per_dc,
+                            self.precision, self.inner_tol)
+
+
+def results_to_dicts(res) -> list:
+    return [dict(iterations=r.iterations, residual=r.residual, true_residual=r.true_residual,
+                 converged=bool(r.converged), diverged=bool(r.diverged), info=r.info,
+                 outer=r.outer) for r in res]
+
+
+# --------------------------------------------------------------------------
+# Device context and operator
+# --------------------------------------------------------------------------
+class Context:
+    """One device context (m3d_context_create)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check_dev(lib().m3d_context_create(device, C.byref(h)))
+        self.handle = h
+
+    @property
+    def stream(self) -> int:
+        return lib().m3d_context_stream(self.handle) or 0
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().m3d_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceOperator:
+    """Thin handle over an m3d_operator (owned or borrowed)."""
+
+    def __init__(self, handle, owner=None, n: int = 0, nn: int = 0):
+        self.handle = handle
+        self._owner = owner
+        self.n = n
+        self.nn = nn
+
+    def info(self) -> dict:
+        npad, nc = C.c_int(), C.c_int()
+        st, nz = C.c_longlong(), C.c_longlong()
+        _check_dev(lib().m3d_operator_info(self.handle, C.byref(npad), C.byref(nc), C.byref(st),
+                                           C.byref(nz)))
+        return dict(n_padded=npad.value, ncolours=nc.value, stored=st.value, nnz=nz.value)
+
+    def permutation(self):
+        inf = self.info()
+        perm = np.zeros(inf["n_padded"], np.int32)
+        cs = np.zeros(inf["ncolours"] + 1, np.int32)
+        _check_dev(lib().m3d_operator_permutation(self.handle, _ptr(perm), _ptr(cs)))
+        return perm, cs
+
+    def bench_kernel(self, kind: int, omegas, reps: int = 20):
+        """(avg ms per launch, algorithmic bytes per launch) of kind 0 = SpMV,
+        1 = SSOR apply, timed with HIP events on the library stream."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        ms, by = C.c_double(), C.c_double()
+        _check_dev(lib().m3d_bench_kernel(self.handle, kind, om.shape[0], _ptr(om), reps,
+                                          C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def spmv(self, omegas, x) -> np.ndarray:
+        x = _as_c128(np.atleast_2d(x))
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check_dev(lib().m3d_spmv(self.handle, x.shape[0], _ptr(om), _ptr(x), _ptr(y)))
+        return y
+
+    def ssor_apply(self, omegas, r) -> np.ndarray:
+        r = _as_c128(np.atleast_2d(r))
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        z = np.zeros_like(r)
+        _check_dev(lib().m3d_ssor_apply(self.handle, r.shape[0], _ptr(om), _ptr(r), _ptr(z)))
+        return z
+
+    def divcorr(self, omegas, E, phi0=None, params: Optional[SolveParams] = None) -> np.ndarray:
+        E = _as_c128(np.atleast_2d(E)).copy()
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        p0 = None if phi0 is None else _as_c128(np.atleast_2d(phi0))
+        prm = (params or SolveParams()).c()
+        _check_dev(lib().m3d_divcorr(self.handle, E.shape[0], _ptr(om), _ptr(E),
+                                     None if p0 is None else _ptr(p0), C.byref(prm)))
+        return E
+
+    def solve(self, mode: int, omegas, rhs, aux=None, x=None,
+              params: Optional[SolveParams] = None):
+        rhs = _as_c128(np.atleast_2d(rhs))
+        ns = rhs.shape[0]
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        assert om.shape == (ns,)
+        xx = np.zeros_like(rhs) if x is None else _as_c128(np.atleast_2d(x)).copy()
+        a = None if aux is None else _as_c128(np.atleast_2d(aux))
+        prm = (params or SolveParams()).c()
+        res = (SolveResultC * ns)()
+        _check_dev(lib().m3d_solve_batch(self.handle, ns, mode, _ptr(om), _ptr(rhs),
+                                         None if a is None else _ptr(a), _ptr(xx), C.byref(prm),
+                                         res))
+        return xx, results_to_dicts(res)
+
+    def solve_device(self, mode: int, omegas, d_rhs: int, d_aux: int, d_x: int, nsys: int,
+                     params: Optional[SolveParams] = None):
+        """Solve with device pointers (e.g. torch tensor .data_ptr())."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        prm = (params or SolveParams()).c()
+        res = (SolveResultC * nsys)()
+        _check_dev(lib().m3d_solve_batch_device(self.handle, nsys, mode, _ptr(om),
+                                                C.c_void_p(d_rhs),
+                                                C.c_void_p(d_aux) if d_aux else None,
+                                                C.c_void_p(d_x), C.byref(prm), res))
+        return results_to_dicts(res)
+
+
+# --------------------------------------------------------------------------
+# Host model (mesh + grid + operators + ForwardSolverFV-shaped driver)
+# --------------------------------------------------------------------------
+_DT = {0: np.float64, 1: np.int32, 2: np.uint8, 3: np.complex128}
+
+
+class Model:
+    """Octree mesh, staggered grid, operators and host solve data of one
+    conductivity model; with a Context it also owns the device operator."""
+
+    def __init__(self, mesh: MeshParams, stations_xyz, freqs, fwd: Optional[ForwardParams] = None,
+                 ctx: Optional[Context] = None):
+        self.mesh_params = mesh
+        self.fwd_params = fwd or ForwardParams()
+        st = np.ascontiguousarray(np.asarray(stations_xyz, dtype=np.float64).reshape(-1, 3))
+        fr = np.ascontiguousarray(np.asarray(freqs, dtype=np.float64).reshape(-1))
+        regs = np.ascontiguousarray(np.asarray(mesh.regions, dtype=np.float64).reshape(-1))
+        mp = MeshParamsC()
+        mp.domain[:] = list(mesh.domain)
+        mp.min_level, mp.max_level = mesh.min_level, mesh.max_level
+        mp.station_refine_radius = mesh.station_refine_radius
+        mp.station_refine_level = mesh.station_refine_level
+        mp.sigma_bg = mesh.sigma_bg
+        mp.max_interest_depth = mesh.max_interest_depth
+        mp.air_surface_thickness = mesh.air_surface_thickness
+        mp.air_mid_thickness = mesh.air_mid_thickness
+        mp.air_upper_thickness = mesh.air_upper_thickness
+        mp.transition_padding = mesh.transition_padding
+        mp.station_bbox[:] = list(mesh.station_bbox)
+        mp.n_regions = len(mesh.regions)
+        mp.regions = regs.ctypes.data_as(C.POINTER(C.c_double)) if len(regs) else None
+        mp.flat_terrain = int(mesh.flat_terrain)
+        f = self.fwd_params
+        fp = ForwardParamsC(f.bicgstab_tol, f.bicgstab_maxiter, f.air_z_threshold,
+                            int(f.div_correction), f.div_corr_iters, f.div_corr_tol,
+                            f.divcorr_outer_max, f.divcorr_iter_per_dc)
+        h = C.c_void_p()
+        _check_host(lib().m3dh_model_create(C.byref(mp), _ptr(st), st.shape[0], _ptr(fr),
+                                            fr.shape[0], C.byref(fp),
+                                            ctx.handle if ctx is not None else None, C.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+        self.n = int(self.count("edges"))
+        self.nc = int(self.count("cells"))
+        self.nn = int(self.count("nodes"))
+        self._stations = None
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().m3dh_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- arrays -----------------------------------------------------------
+    def array(self, name: str) -> np.ndarray:
+        dt, cnt, ptr = C.c_int(), C.c_longlong(), C.c_void_p()
+        _check_host(lib().m3dh_get_array(self.handle, name.encode(), C.byref(dt), C.byref(cnt),
+                                         C.byref(ptr)))
+        dtype = _DT[dt.value]
+        if cnt.value == 0:
+            return np.zeros(0, dtype)
+        buf = (C.c_char * (cnt.value * np.dtype(dtype).itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dtype).copy()
+
+    def count(self, name: str) -> int:
+        return int(lib().m3dh_get_count(self.handle, name.encode()))
+
+    @property
+    def device(self) -> DeviceOperator:
+        h = lib().m3dh_device_operator(self.handle)
+        if not h:
+            raise NativeError("model has no device operator (created without a Context)")
+        return DeviceOperator(C.c_void_p(h), owner=self, n=self.n, nn=self.nn)
+
+    # -- host producers ----------------------------------------------------
+    def set_sigma(self, sigma) -> None:
+        s = np.ascontiguousarray(sigma, dtype=np.float64)
+        assert s.shape == (self.nc,)
+        _check_host(lib().m3dh_set_sigma(self.handle, _ptr(s)))
+
+    def find_stations(self, xy) -> np.ndarray:
+        a = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+        _check_host(lib().m3dh_find_stations(self.handle, _ptr(a), a.shape[0]))
+        self._stations = a.shape[0]
+        return self.array("station_cell")
+
+    def primary_field(self, omega: float, direction: int) -> np.ndarray:
+        out = np.zeros(self.n, np.complex128)
+        _check_host(lib().m3dh_primary_field(self.handle, omega, direction, _ptr(out)))
+        return out
+
+    def bc_values(self, omega: float, direction: int) -> np.ndarray:
+        out = np.zeros(self.n, np.complex128)
+        _check_host(lib().m3dh_bc_values(self.handle, omega, direction, _ptr(out)))
+        return out
+
+    def rhs_total(self, ebc) -> np.ndarray:
+        e = _as_c128(ebc)
+        out = np.zeros(self.n, np.complex128)
+        _check_host(lib().m3dh_rhs_total(self.handle, _ptr(e), _ptr(out)))
+        return out
+
+    def phi0(self, omega: float, rhs) -> np.ndarray:
+        r = _as_c128(rhs)
+        out = np.zeros(self.nn, np.complex128)
+        _check_host(lib().m3dh_phi0(self.handle, omega, _ptr(r), _ptr(out)))
+        return out
+
+    def extract_impedance(self, omega: float, e1, e2) -> np.ndarray:
+        a, b = _as_c128(e1), _as_c128(e2)
+        ns = self.count("stations")
+        z = np.zeros((ns, 4), np.complex128)
+        _check_host(lib().m3dh_extract_impedance(self.handle, omega, _ptr(a), _ptr(b), _ptr(z)))
+        return z
+
+    def adjoint_rhs(self, omega: float, wr):
+        w = _as_c128(wr)
+        r1 = np.zeros(self.n, np.complex128)
+        r2 = np.zeros(self.n, np.complex128)
+        _check_host(lib().m3dh_adjoint_rhs(self.handle, omega, _ptr(w), _ptr(r1), _ptr(r2)))
+        return r1, r2
+
+    def sensitivity(self, omega: float, e_bg, lam) -> np.ndarray:
+        a, b = _as_c128(e_bg), _as_c128(lam)
+        out = np.zeros(self.nc, np.float64)
+        _check_host(lib().m3dh_sensitivity(self.handle, omega, _ptr(a), _ptr(b), _ptr(out)))
+        return out
+
+    # -- device solves -----------------------------------------------------
+    def solve_frequencies(self, freqs, params: Optional[SolveParams] = None):
+        """Forward solve (both polarisations) of every frequency in one batch.
+        Returns E [2*nf, n] (row 2f+pol), Z [nf, stations, 4] and results."""
+        fr = np.ascontiguousarray(freqs, dtype=np.float64).reshape(-1)
+        nf = fr.shape[0]
+        ns = self.count("stations")
+        e = np.zeros((2 * nf, self.n), np.complex128)
+        z = np.zeros((nf, max(ns, 0), 4), np.complex128)
+        prm = (params or self.fwd_params.solve_params()).c()
+        res = (SolveResultC * (2 * nf))()
+        _check_host(lib().m3dh_solve_frequencies(self.handle, _ptr(fr), nf, C.byref(prm), _ptr(e),
+                                                 _ptr(z) if ns > 0 else None, res))
+        return e, z, results_to_dicts(res)
+
+    def adjoint_solve(self, omegas, rhs, params: Optional[SolveParams] = None):
+        r = _as_c128(np.atleast_2d(rhs))
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        x = np.zeros_like(r)
+        prm = (params or self.fwd_params.solve_params()).c()
+        res = (SolveResultC * r.shape[0])()
+        _check_host(lib().m3dh_adjoint_solve(self.handle, _ptr(om), r.shape[0], _ptr(r),
+                                             C.byref(prm), _ptr(x), res))
+        return x, results_to_dicts(res)
