@@ -40,7 +40,34 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="profiling mode: build the workload, then only run the batched SpMV "
+                         "and SSOR kernels --reps times and a 1 GiB device copy (counter "
+                         "calibration)")
+    ap.add_argument("--reps", type=int, default=20)
     return ap.parse_args()
+
+
+def kernel_only(args):
+    """Launch only the roofline kernels (for rocprofv3 --pmc runs)."""
+    import torch
+
+    from maple3dmt_amd import Context, configs
+    wl = configs.random3d(args.seed, max_level=args.max_level, half_width=args.half_width,
+                          station_radius=args.station_radius)
+    ctx = Context(0)
+    model = configs.build_model(wl, ctx)
+    omegas = np.repeat(2 * np.pi * wl.freqs, 2)
+    ms, by = model.device.bench_kernel(0, omegas, reps=args.reps)
+    ms2, by2 = model.device.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
+    x = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(4):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"spmv_ms": ms, "spmv_alg_bytes": by, "ssor_ms": ms2, "ssor_alg_bytes": by2,
+                      "copy_bytes_each_way": x.numel() * 8, "copies": 4, "edges": model.n}),
+          flush=True)
 
 
 def log(msg):
@@ -54,10 +81,13 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.kernel_only:
+        kernel_only(args)
+        return
+
     import torch
 
     from maple3dmt_amd import MODE_FORWARD, PREC_MIXED, Context, configs
-    from maple3dmt_amd.native import SolveParams
 
     t0 = time.time()
     wl = configs.random3d(args.seed, max_level=args.max_level, half_width=args.half_width,

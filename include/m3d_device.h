@@ -113,6 +113,8 @@ typedef struct {
     int divcorr_iter_per_dc; /* divcorr_iter_per_dc */
     int precision;           /* M3D_PREC_FP64 or M3D_PREC_MIXED */
     double inner_tol;        /* MIXED: relative tolerance of each complex64 inner solve */
+    int use_initial_guess;   /* FORWARD: x holds an E_sec guess (e.g. the learned e0); it is
+                                used where ||b - A e0|| < ||b||, else the zero start */
 } m3d_solve_params;
 
 /* Per-system outcome; mirrors BiCGStabResult (bicgstab.h:23-28) plus what the
@@ -125,13 +127,15 @@ typedef struct {
     int diverged;         /* solution zeroed because the residual exceeded 1 */
     int info;             /* 0 converged, 1 early (||s||), 2 breakdown, 3 max iterations */
     int outer;            /* outer iterations (interleaved bursts / refinement steps) */
+    int guess_used;       /* FORWARD with use_initial_guess: 1 if the guess was kept */
 } m3d_solve_result;
 
 /* Solve nsys systems that share this operator, one omega each.
  *   rhs  [nsys][n] complex: b (FORWARD: rhs_total of build_rhs, :244-255)
  *   aux  [nsys][n] complex: E_pri for FORWARD, else NULL
- *   x    [nsys][n] complex: initial guess in (ignored by FORWARD and
- *                           INTERLEAVED, which start from 0), solution out
+ *   x    [nsys][n] complex: initial guess in (INNER / SYSTEM; FORWARD only
+ *                           with use_initial_guess, as E_sec; INTERLEAVED
+ *                           starts from 0), solution out
  * Host-memory variant. */
 int m3d_solve_batch(m3d_operator* op, int nsys, int mode, const double* omega,
                     const double* rhs, const double* aux, double* x,

@@ -31,6 +31,7 @@ HOST_SOURCES = [
 DEVICE_SOURCES = [
     "device/kernels.hip",
     "device/solver.hip",
+    "device/nn.hip",
 ]
 
 ARCH = os.environ.get("M3D_OFFLOAD_ARCH", "gfx950")

@@ -43,6 +43,8 @@ struct Launch64 {
     static void forward_total(hipStream_t s, int npad, const unsigned char* bc,
                               const double2* epri, const double2* b, double2* x);
     static void add_correction(hipStream_t s, size_t count, const float2* d, double2* x);
+    static void fix_constrained(hipStream_t s, int npad, const unsigned char* cons, const double2* b,
+                                double2* x);
     static void demote(hipStream_t s, size_t count, const double2* src, const double* scale,
                        float2* dst);
     static void add_scaled(hipStream_t s, size_t count, const float2* d, const double* scale,
@@ -67,6 +69,9 @@ struct Launch64 {
 };
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol);
+template <typename V>
+void launch_regroup(hipStream_t s, int rows, int na, int nb, const int* map, cvec<V>* wide,
+                    cvec<V>* narrow, bool to_narrow);
 void launch_dc_begin(hipStream_t s, SysState* st, unsigned mask);
 
 }  // namespace dev

@@ -499,6 +499,27 @@ __global__ void k_true_res(DevOp<double> op, const double* __restrict__ omega,
 // ---------------------------------------------------------------------------
 // Vector utilities.
 // ---------------------------------------------------------------------------
+
+// Move systems between batch widths: narrow[row*nb + j] = wide[row*na + map[j]]
+// (to_narrow) or the reverse, for map[j] >= 0.
+template <typename V>
+__global__ void k_regroup(int rows, int na, int nb, const int* __restrict__ map,
+                          cvec<V>* __restrict__ wide, cvec<V>* __restrict__ narrow, int to_narrow) {
+    const size_t total = size_t(rows) * nb;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int row = int(t / nb), j = int(t % nb);
+        const int src = map[j];
+        if (to_narrow) {
+            cvec<V> z;
+            z.x = 0;
+            z.y = 0;
+            narrow[t] = src >= 0 ? wide[size_t(row) * na + src] : z;
+        } else if (src >= 0) {
+            wide[size_t(row) * na + src] = narrow[t];
+        }
+    }
+}
 template <int NS, typename V>
 __global__ void k_fill_zero(size_t count, cvec<V>* __restrict__ x) {
     for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < count;
@@ -555,6 +576,17 @@ __global__ void k_forward_total(int npad, const unsigned char* __restrict__ bc,
             x[t] = mk(a.x + c.x, a.y + c.y);
         }
     }
+}
+
+// Constrained (identity) rows of a supplied initial guess take their exact
+// value x_i = b_i.
+template <int NS>
+__global__ void k_fix_constrained(int npad, const unsigned char* __restrict__ cons,
+                                  const double2* __restrict__ b, double2* __restrict__ x) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x)
+        if (cons[t / NS]) x[t] = b[t];
 }
 
 // x += d (FP64 accumulate of an FP32 correction).
@@ -941,6 +973,12 @@ void Launch64<NS>::forward_total(hipStream_t s, int npad, const unsigned char* b
                        dim3(kBlock), 0, s, npad, bc, epri, b, x);
 }
 template <int NS>
+void Launch64<NS>::fix_constrained(hipStream_t s, int npad, const unsigned char* cons, const double2* b,
+                                   double2* x) {
+    hipLaunchKernelGGL((k_fix_constrained<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
+                       dim3(kBlock), 0, s, npad, cons, b, x);
+}
+template <int NS>
 void Launch64<NS>::add_correction(hipStream_t s, size_t count, const float2* d, double2* x) {
     hipLaunchKernelGGL((k_add_correction<NS>), dim3(grid_for(count, kStreamCap)), dim3(kBlock), 0,
                        s, count, d, x);
@@ -999,6 +1037,15 @@ void Launch64<NS>::dc_correct(hipStream_t s, int npad, const int* g_col, const d
     hipLaunchKernelGGL((k_dc_correct<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
                        dim3(kBlock), 0, s, npad, g_col, g_val, bc_edge, phi, E, st);
 }
+
+template <typename V>
+void launch_regroup(hipStream_t s, int rows, int na, int nb, const int* map, cvec<V>* wide,
+                    cvec<V>* narrow, bool to_narrow) {
+    hipLaunchKernelGGL((k_regroup<V>), dim3(grid_for(size_t(rows) * nb, kStreamCap)), dim3(kBlock),
+                       0, s, rows, na, nb, map, wide, narrow, to_narrow ? 1 : 0);
+}
+template void launch_regroup<double>(hipStream_t, int, int, int, const int*, double2*, double2*, bool);
+template void launch_regroup<float>(hipStream_t, int, int, int, const int*, float2*, float2*, bool);
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol) {
     hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(64), 0, s, st, mask, maxiter, tol);

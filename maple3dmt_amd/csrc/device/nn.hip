@@ -1,0 +1,606 @@
+// Coarse-grid neural operator for the initial field e0: four 3x3x3 dilated
+// conv layers on a 32^3 grid, BF16 storage, FP32 accumulation on the CDNA
+// matrix cores (v_mfma_f32_32x32x8_bf16), trained with Adam on the device.
+//
+// Every conv pass is an implicit GEMM whose 32x32 MFMA tile is one x-row of
+// the grid (32 voxels) against 32 channels:
+//   forward      out[v, co]  = sum_{tap, ci} in[v + d*tap, ci] * W[tap][co][ci]
+//   input grad   din[v, ci]  = sum_{tap, co} dz[v - d*tap, co] * W[tap][co][ci]
+//   weight grad  dW[tap][co][ci] = sum_v dz[v, co] * in[v + d*tap, ci]
+// Activations are channels-last [batch][z][y][x][C]; zero padding outside
+// the grid.  The layer has no reference counterpart: the reference starts
+// every forward solve from zero (forward_solver_fv.cpp:353).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "m3d_net.h"
+
+namespace {
+
+constexpr int G = 32;                 // grid points per axis
+constexpr int V = G * G * G;          // voxels per sample
+constexpr int NL = 4;                 // conv layers
+constexpr int HID = 32;               // hidden channels (and padded output channels)
+constexpr int CIN0 = 8;               // input channels after padding (2 used)
+constexpr int COUT = 12;              // output channels (2 pol x 3 comp x re/im)
+constexpr int CIN_USED = 2;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f16 __attribute__((ext_vector_type(16)));
+
+__device__ inline unsigned short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return static_cast<unsigned short>(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
+
+__device__ inline void tap_offsets(int tap, int& dz, int& dy, int& dx) {
+    dz = tap / 9 - 1;
+    dy = (tap / 3) % 3 - 1;
+    dx = tap % 3 - 1;
+}
+
+// Forward conv of one layer.  One wavefront = one x-row (b, z, y) x 32 output
+// channels.  Lane l: voxel x = l % 32 for the A operand, output channel
+// l % 32 for B; k-quad q = l / 32 selects channels 4q..4q+3 of each 8-wide
+// k step.  Accumulator r of lane l is D[x = 8(r/4) + 4q + r%4][co = l % 32].
+template <int CIN, bool RELU, bool OUTF32>
+__global__ __launch_bounds__(256) void k_conv_fwd(int batch, int dil,
+                                                  const unsigned short* __restrict__ in,
+                                                  const unsigned short* __restrict__ wt,
+                                                  const float* __restrict__ bias,
+                                                  unsigned short* __restrict__ out_bf,
+                                                  float* __restrict__ out_f, int cout_store) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + wave;
+    if (row >= batch * G * G) return;
+    const int b = row / (G * G), z = (row / G) % G, y = row % G;
+    const int m = lane & 31, q = lane >> 5;
+    f16 acc;
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    for (int tap = 0; tap < 27; ++tap) {
+        int dz, dy, dx;
+        tap_offsets(tap, dz, dy, dx);
+        const int zz = z + dz * dil, yy = y + dy * dil, xx = m + dx * dil;
+        const bool ok = zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G;
+        const size_t vin = ((size_t(b) * G + (ok ? zz : 0)) * G + (ok ? yy : 0)) * G + (ok ? xx : 0);
+        const unsigned short* src = in + vin * CIN;
+        const unsigned short* wsrc = wt + (size_t(tap) * HID + m) * CIN;
+#pragma unroll
+        for (int c0 = 0; c0 < CIN; c0 += 8) {
+            s4 a = {0, 0, 0, 0};
+            if (ok) a = *reinterpret_cast<const s4*>(src + c0 + 4 * q);
+            const s4 w = *reinterpret_cast<const s4*>(wsrc + c0 + 4 * q);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w, acc, 0, 0, 0);
+        }
+    }
+    const int co = m;
+    const float bc = bias[co];
+    const size_t vrow = ((size_t(b) * G + z) * G + y) * G;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int x = 8 * (r >> 2) + 4 * q + (r & 3);
+        float v = acc[r] + bc;
+        if (RELU) v = v > 0.f ? v : 0.f;
+        if (OUTF32) {
+            if (co < cout_store) out_f[(vrow + x) * cout_store + co] = v;
+        } else {
+            out_bf[(vrow + x) * HID + co] = f2bf(v);
+        }
+    }
+}
+
+// Input gradient of a 32-channel layer: dz = dA * 1[act > 0] (when act is
+// given) gathered at v - d*tap; B operand lane l reads W[tap][co-quad][ci = l%32].
+__global__ __launch_bounds__(256) void k_conv_bwd_data(int batch, int dil,
+                                                       const unsigned short* __restrict__ dout,
+                                                       const unsigned short* __restrict__ act,
+                                                       const unsigned short* __restrict__ wb,
+                                                       unsigned short* __restrict__ din) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + wave;
+    if (row >= batch * G * G) return;
+    const int b = row / (G * G), z = (row / G) % G, y = row % G;
+    const int m = lane & 31, q = lane >> 5;
+    f16 acc;
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    for (int tap = 0; tap < 27; ++tap) {
+        int dz, dy, dx;
+        tap_offsets(tap, dz, dy, dx);
+        const int zz = z - dz * dil, yy = y - dy * dil, xx = m - dx * dil;
+        const bool ok = zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G;
+        const size_t vsrc = ((size_t(b) * G + (ok ? zz : 0)) * G + (ok ? yy : 0)) * G + (ok ? xx : 0);
+        const unsigned short* src = dout + vsrc * HID;
+        const unsigned short* msk = act ? act + vsrc * HID : nullptr;
+        const unsigned short* wsrc = wb + (size_t(tap) * HID + m) * HID;  // [tap][ci][co]
+#pragma unroll
+        for (int c0 = 0; c0 < HID; c0 += 8) {
+            s4 a = {0, 0, 0, 0};
+            if (ok) {
+                a = *reinterpret_cast<const s4*>(src + c0 + 4 * q);
+                if (msk) {
+                    const s4 g = *reinterpret_cast<const s4*>(msk + c0 + 4 * q);
+                    for (int j = 0; j < 4; ++j)
+                        if (bf2f(static_cast<unsigned short>(g[j])) <= 0.f) a[j] = 0;
+                }
+            }
+            const s4 w = *reinterpret_cast<const s4*>(wsrc + c0 + 4 * q);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w, acc, 0, 0, 0);
+        }
+    }
+    const size_t vrow = ((size_t(b) * G + z) * G + y) * G;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int x = 8 * (r >> 2) + 4 * q + (r & 3);
+        din[(vrow + x) * HID + m] = f2bf(acc[r]);
+    }
+}
+
+// Weight gradient partials: block (chunk, tap) sums dz[v, co] * in[v + d*tap, ci]
+// over the rows of its chunk; M = co (32), N = ci (CIN, zero-padded to 32),
+// K = voxels, 8 per MFMA step (x = 8s + 4q + j for lane quad q).
+template <int CIN>
+__global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows_per_chunk,
+                                                    const unsigned short* __restrict__ in,
+                                                    const unsigned short* __restrict__ dout,
+                                                    const unsigned short* __restrict__ act,
+                                                    float* __restrict__ part) {
+    __shared__ float red[4][16][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int chunk = blockIdx.x, tap = blockIdx.y;
+    const int m = lane & 31, q = lane >> 5;
+    int dz, dy, dx;
+    tap_offsets(tap, dz, dy, dx);
+    f16 acc;
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    const int rows = batch * G * G;
+    const int r0 = chunk * rows_per_chunk, r1 = min(rows, r0 + rows_per_chunk);
+    for (int row = r0 + wave; row < r1; row += 4) {
+        const int b = row / (G * G), z = (row / G) % G, y = row % G;
+        const size_t vrow = ((size_t(b) * G + z) * G + y) * G;
+        const int zz = z + dz * dil, yy = y + dy * dil;
+        const bool okzy = zz >= 0 && zz < G && yy >= 0 && yy < G;
+        const size_t vsrc_row = ((size_t(b) * G + (okzy ? zz : 0)) * G + (okzy ? yy : 0)) * G;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            s4 a, w;
+            for (int j = 0; j < 4; ++j) {
+                const int x = 8 * s + 4 * q + j;
+                const size_t v = vrow + x;
+                unsigned short g = dout[v * HID + m];
+                if (act && bf2f(act[v * HID + m]) <= 0.f) g = 0;
+                a[j] = static_cast<short>(g);
+                const int xx = x + dx * dil;
+                unsigned short iv = 0;
+                if (okzy && xx >= 0 && xx < G && m < CIN) iv = in[(vsrc_row + xx) * CIN + m];
+                w[j] = static_cast<short>(iv);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w, acc, 0, 0, 0);
+        }
+    }
+    for (int r = 0; r < 16; ++r) red[wave][r][lane] = acc[r];
+    __syncthreads();
+    if (wave == 0) {
+        for (int r = 0; r < 16; ++r) {
+            const float v = red[0][r][lane] + red[1][r][lane] + red[2][r][lane] + red[3][r][lane];
+            const int co = 8 * (r >> 2) + 4 * q + (r & 3);  // D row = M index = co
+            const int ci = m;                               // D column = N index = ci
+            if (ci < CIN) part[((size_t(chunk) * 27 + tap) * HID + co) * CIN + ci] = v;
+        }
+    }
+}
+
+__global__ void k_sum_chunks(int nchunk, int count, const float* __restrict__ part, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[size_t(c) * count + i];
+    out[i] = s;
+}
+
+// Bias gradient partials: per block sum of dz over a voxel range.
+__global__ void k_bias_grad(int nvox, const unsigned short* __restrict__ dout,
+                            const unsigned short* __restrict__ act, float* __restrict__ part) {
+    __shared__ float sm[256];
+    const int co = threadIdx.x & 31, sub = threadIdx.x >> 5;  // 8 voxel lanes per channel
+    float s = 0.f;
+    for (int v = blockIdx.x * 8 + sub; v < nvox; v += gridDim.x * 8) {
+        unsigned short g = dout[size_t(v) * HID + co];
+        if (act && bf2f(act[size_t(v) * HID + co]) <= 0.f) g = 0;
+        s += bf2f(g);
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float t = 0.f;
+        for (int k = 0; k < 8; ++k) t += sm[k * 32 + threadIdx.x];
+        part[blockIdx.x * 32 + threadIdx.x] = t;
+    }
+}
+
+// Input features [n][2] FP32 -> [n][8] BF16 (zero padded).
+__global__ void k_prep_input(int nvox, const float* __restrict__ x, unsigned short* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvox * CIN0) return;
+    const int v = i / CIN0, c = i % CIN0;
+    out[i] = c < CIN_USED ? f2bf(x[size_t(v) * CIN_USED + c]) : 0;
+}
+
+// MSE loss partials and dL/dout = 2 (out - tgt) / count as BF16 [n][32].
+__global__ void k_loss(int nvox, const float* __restrict__ out, const float* __restrict__ tgt,
+                       float scale, unsigned short* __restrict__ dout, float* __restrict__ part) {
+    __shared__ float sm[256];
+    float s = 0.f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nvox * HID; i += gridDim.x * blockDim.x) {
+        const int v = i / HID, c = i % HID;
+        float g = 0.f;
+        if (c < COUT) {
+            const float d = out[size_t(v) * COUT + c] - tgt[size_t(v) * COUT + c];
+            s += d * d;
+            g = 2.f * d * scale;
+        }
+        dout[i] = f2bf(g);
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// Adam on FP32 master parameters; refreshes the BF16 forward ([tap][co][ci])
+// and backward ([tap][ci][co]) weight copies of the layer.
+__global__ void k_adam(int count, float lr, float b1, float b2, float eps, float c1, float c2,
+                       float* __restrict__ p, const float* __restrict__ g, float* __restrict__ mo,
+                       float* __restrict__ ve) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float m = b1 * mo[i] + (1.f - b1) * gi;
+    const float v = b2 * ve[i] + (1.f - b2) * gi * gi;
+    mo[i] = m;
+    ve[i] = v;
+    p[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+__global__ void k_weights_bf16(int cin, const float* __restrict__ w, unsigned short* __restrict__ wt,
+                               unsigned short* __restrict__ wb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 27 * HID * cin) return;
+    const int tap = i / (HID * cin), co = (i / cin) % HID, ci = i % cin;
+    const unsigned short h = f2bf(w[i]);
+    wt[i] = h;
+    if (wb && cin == HID) wb[(size_t(tap) * HID + ci) * HID + co] = h;
+}
+
+// Trilinear interpolation of the voxel field (voxel centres of the box
+// [x0, x0 + 32 hx] x ...) to edge midpoints: system 2b + pol of the output
+// takes channel pol*6 + 2*dir + {re, im} of sample b; edges outside the
+// centre lattice get 0.
+__global__ void k_prolong(int batch, int n, const float* __restrict__ field,
+                          const float* __restrict__ exyz, const unsigned char* __restrict__ edir,
+                          float x0, float y0, float z0, float hx, float hy, float hz,
+                          double2* __restrict__ out) {
+    const size_t total = size_t(batch) * n;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int b = int(t / n), e = int(t % n);
+        const float fx = (exyz[3 * size_t(e)] - x0) / hx - 0.5f;
+        const float fy = (exyz[3 * size_t(e) + 1] - y0) / hy - 0.5f;
+        const float fz = (exyz[3 * size_t(e) + 2] - z0) / hz - 0.5f;
+        double2 v1 = make_double2(0.0, 0.0), v2 = make_double2(0.0, 0.0);
+        if (fx >= 0.f && fy >= 0.f && fz >= 0.f && fx <= G - 1 && fy <= G - 1 && fz <= G - 1) {
+            const int ix = min(int(fx), G - 2), iy = min(int(fy), G - 2), iz = min(int(fz), G - 2);
+            const float tx = fx - ix, ty = fy - iy, tz = fz - iz;
+            const int d = edir[e];
+            float a[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < 8; ++c) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                const float wgt = (cx ? tx : 1.f - tx) * (cy ? ty : 1.f - ty) * (cz ? tz : 1.f - tz);
+                const size_t vox = ((size_t(b) * G + iz + cz) * G + iy + cy) * G + ix + cx;
+                const float* f = field + vox * COUT;
+                a[0] += wgt * f[2 * d];
+                a[1] += wgt * f[2 * d + 1];
+                a[2] += wgt * f[6 + 2 * d];
+                a[3] += wgt * f[6 + 2 * d + 1];
+            }
+            v1 = make_double2(a[0], a[1]);
+            v2 = make_double2(a[2], a[3]);
+        }
+        out[(2 * size_t(b)) * n + e] = v1;
+        out[(2 * size_t(b) + 1) * n + e] = v2;
+    }
+}
+
+void chk(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct Buf {
+    T* p = nullptr;
+    size_t n = 0;
+    Buf() = default;
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    ~Buf() { if (p) (void)hipFree(p); }
+    void alloc(size_t c) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        chk(hipMalloc(&p, std::max<size_t>(c, 1) * sizeof(T)), "hipMalloc");
+        n = c;
+    }
+};
+
+thread_local std::string g_nn_err;
+
+}  // namespace
+
+// Layer l: input channels cin(l) = 8 for l = 0, else 32; 32 output channels
+// (the last layer's first 12 are used); dilation dil[l]; ReLU on all but the last.
+struct m3d_net {
+    hipStream_t stream = nullptr;
+    m3d_context* ctx = nullptr;
+    int max_batch = 0;
+    int dil[NL] = {1, 2, 4, 1};
+    int step = 0;
+    Buf<float> w[NL], b[NL], gw[NL], gb[NL], mw[NL], vw[NL], mb[NL], vb[NL];
+    Buf<unsigned short> wt[NL], wb[NL];
+    Buf<unsigned short> x0, act[NL - 1], grad_a, grad_b, dout;
+    Buf<float> out, part, lpart;
+    static int cin(int l) { return l == 0 ? CIN0 : HID; }
+};
+
+namespace {
+
+void upload_bf16(m3d_net& n) {
+    for (int l = 0; l < NL; ++l) {
+        const int c = m3d_net::cin(l);
+        hipLaunchKernelGGL(k_weights_bf16, dim3((27 * HID * c + 255) / 256), dim3(256), 0, n.stream, c,
+                           n.w[l].p, n.wt[l].p, l > 0 ? n.wb[l].p : nullptr);
+    }
+    chk(hipGetLastError(), "k_weights_bf16");
+}
+
+void forward(m3d_net& n, int batch, const float* d_in, float* d_out) {
+    const int nvox = batch * V;
+    hipStream_t s = n.stream;
+    hipLaunchKernelGGL(k_prep_input, dim3((nvox * CIN0 + 255) / 256), dim3(256), 0, s, nvox, d_in, n.x0.p);
+    const dim3 grid((batch * G * G + 3) / 4), blk(256);
+    hipLaunchKernelGGL((k_conv_fwd<CIN0, true, false>), grid, blk, 0, s, batch, n.dil[0], n.x0.p, n.wt[0].p,
+                       n.b[0].p, n.act[0].p, nullptr, HID);
+    hipLaunchKernelGGL((k_conv_fwd<HID, true, false>), grid, blk, 0, s, batch, n.dil[1], n.act[0].p, n.wt[1].p,
+                       n.b[1].p, n.act[1].p, nullptr, HID);
+    hipLaunchKernelGGL((k_conv_fwd<HID, true, false>), grid, blk, 0, s, batch, n.dil[2], n.act[1].p, n.wt[2].p,
+                       n.b[2].p, n.act[2].p, nullptr, HID);
+    hipLaunchKernelGGL((k_conv_fwd<HID, false, true>), grid, blk, 0, s, batch, n.dil[3], n.act[2].p, n.wt[3].p,
+                       n.b[3].p, nullptr, d_out, COUT);
+    chk(hipGetLastError(), "conv forward");
+}
+
+// Loss and parameter gradients (into gw / gb) for one batch.
+float backward(m3d_net& n, int batch, const float* d_in, const float* d_tgt) {
+    const int nvox = batch * V;
+    hipStream_t s = n.stream;
+    forward(n, batch, d_in, n.out.p);
+    const int lblocks = 512;
+    hipLaunchKernelGGL(k_loss, dim3(lblocks), dim3(256), 0, s, nvox, n.out.p, d_tgt,
+                       1.0f / float(size_t(nvox) * COUT), n.dout.p, n.lpart.p);
+    const dim3 grid((batch * G * G + 3) / 4), blk(256);
+    const int rows = batch * G * G;
+    const int nchunk = 64;
+    const int rpc = (rows + nchunk - 1) / nchunk;
+    // dz of layer l is grad (masked by act[l] for ReLU layers).
+    const unsigned short* g = n.dout.p;
+    unsigned short* bufs[2] = {n.grad_a.p, n.grad_b.p};
+    for (int l = NL - 1; l >= 0; --l) {
+        const unsigned short* a_out = (l < NL - 1) ? n.act[l].p : nullptr;  // ReLU mask
+        const unsigned short* a_in = (l == 0) ? n.x0.p : n.act[l - 1].p;
+        const int c = m3d_net::cin(l);
+        if (c == CIN0)
+            hipLaunchKernelGGL((k_conv_bwd_w<CIN0>), dim3(nchunk, 27), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
+                               a_out, n.part.p);
+        else
+            hipLaunchKernelGGL((k_conv_bwd_w<HID>), dim3(nchunk, 27), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
+                               a_out, n.part.p);
+        const int cnt = 27 * HID * c;
+        hipLaunchKernelGGL(k_sum_chunks, dim3((cnt + 255) / 256), dim3(256), 0, s, nchunk, cnt, n.part.p,
+                           n.gw[l].p);
+        hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(256), 0, s, nvox, g, a_out, n.part.p);
+        hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(32), 0, s, 256, 32, n.part.p, n.gb[l].p);
+        if (l > 0) {
+            unsigned short* dst = bufs[l & 1];
+            hipLaunchKernelGGL(k_conv_bwd_data, grid, blk, 0, s, batch, n.dil[l], g, a_out, n.wb[l].p, dst);
+            g = dst;
+        }
+    }
+    chk(hipGetLastError(), "conv backward");
+    std::vector<float> lp(lblocks);
+    chk(hipMemcpyAsync(lp.data(), n.lpart.p, lblocks * sizeof(float), hipMemcpyDeviceToHost, s), "loss copy");
+    chk(hipStreamSynchronize(s), "sync");
+    double tot = 0;
+    for (float v : lp) tot += v;
+    return float(tot / (double(nvox) * COUT));
+}
+
+template <typename F>
+int nn_guard(F&& f) {
+    try {
+        f();
+        return M3D_OK;
+    } catch (const std::exception& e) {
+        g_nn_err = e.what();
+        return M3D_ERR_HIP;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* m3d_net_last_error(void) { return g_nn_err.c_str(); }
+
+int m3d_net_create(m3d_context* ctx, int max_batch, const int* dilations, unsigned seed, m3d_net** out) {
+    return nn_guard([&] {
+        if (!ctx || !out || max_batch <= 0) throw std::invalid_argument("invalid arguments");
+        std::unique_ptr<m3d_net> n(new m3d_net);
+        n->ctx = ctx;
+        n->stream = static_cast<hipStream_t>(m3d_context_stream(ctx));
+        n->max_batch = max_batch;
+        if (dilations) for (int l = 0; l < NL; ++l) n->dil[l] = std::max(1, dilations[l]);
+        // He-uniform initialisation from a fixed LCG (deterministic per seed).
+        unsigned long long st = 0x9E3779B97F4A7C15ull ^ seed;
+        auto urand = [&]() {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            return double((st >> 11) & ((1ull << 53) - 1)) / double(1ull << 53);
+        };
+        for (int l = 0; l < NL; ++l) {
+            const int c = m3d_net::cin(l);
+            const int cnt = 27 * HID * c;
+            const int fan_in = 27 * (l == 0 ? CIN_USED : HID);
+            const double lim = std::sqrt(6.0 / fan_in) * (l == NL - 1 ? 0.1 : 1.0);
+            std::vector<float> w(cnt, 0.f), b(HID, 0.f);
+            for (int i = 0; i < cnt; ++i) {
+                const int co = (i / c) % HID, ci = i % c;
+                const bool used = (l > 0 || ci < CIN_USED) && (l < NL - 1 || co < COUT);
+                const double r = urand();
+                w[i] = used ? float((2.0 * r - 1.0) * lim) : 0.f;
+            }
+            n->w[l].alloc(cnt); n->gw[l].alloc(cnt); n->mw[l].alloc(cnt); n->vw[l].alloc(cnt);
+            n->b[l].alloc(HID); n->gb[l].alloc(HID); n->mb[l].alloc(HID); n->vb[l].alloc(HID);
+            n->wt[l].alloc(cnt);
+            if (l > 0) n->wb[l].alloc(cnt);
+            chk(hipMemcpyAsync(n->w[l].p, w.data(), cnt * sizeof(float), hipMemcpyHostToDevice, n->stream), "w");
+            chk(hipMemcpyAsync(n->b[l].p, b.data(), HID * sizeof(float), hipMemcpyHostToDevice, n->stream), "b");
+            chk(hipMemsetAsync(n->mw[l].p, 0, cnt * sizeof(float), n->stream), "m");
+            chk(hipMemsetAsync(n->vw[l].p, 0, cnt * sizeof(float), n->stream), "v");
+            chk(hipMemsetAsync(n->mb[l].p, 0, HID * sizeof(float), n->stream), "m");
+            chk(hipMemsetAsync(n->vb[l].p, 0, HID * sizeof(float), n->stream), "v");
+        }
+        const size_t nv = size_t(max_batch) * V;
+        n->x0.alloc(nv * CIN0);
+        for (int l = 0; l < NL - 1; ++l) n->act[l].alloc(nv * HID);
+        n->grad_a.alloc(nv * HID);
+        n->grad_b.alloc(nv * HID);
+        n->dout.alloc(nv * HID);
+        n->out.alloc(nv * COUT);
+        n->part.alloc(size_t(64) * 27 * HID * HID + 256 * 32);
+        n->lpart.alloc(512);
+        upload_bf16(*n);
+        chk(hipStreamSynchronize(n->stream), "sync");
+        *out = n.release();
+    });
+}
+
+int m3d_net_destroy(m3d_net* n) {
+    return nn_guard([&] {
+        if (!n) return;
+        (void)hipStreamSynchronize(n->stream);
+        delete n;
+    });
+}
+
+// Parameter count: weights 27*32*(8 + 3*32) floats in layer order, each
+// [tap][co][ci]; biases 4*32.
+int m3d_net_num_params(long long* nw, long long* nb) {
+    long long w = 0;
+    for (int l = 0; l < NL; ++l) w += 27LL * HID * m3d_net::cin(l);
+    if (nw) *nw = w;
+    if (nb) *nb = NL * HID;
+    return M3D_OK;
+}
+
+int m3d_net_get_params(m3d_net* n, float* w, float* b, float* gw, float* gb) {
+    return nn_guard([&] {
+        size_t ow = 0;
+        for (int l = 0; l < NL; ++l) {
+            const size_t cnt = size_t(27) * HID * m3d_net::cin(l);
+            if (w) chk(hipMemcpyAsync(w + ow, n->w[l].p, cnt * 4, hipMemcpyDeviceToHost, n->stream), "get w");
+            if (gw) chk(hipMemcpyAsync(gw + ow, n->gw[l].p, cnt * 4, hipMemcpyDeviceToHost, n->stream), "get gw");
+            if (b) chk(hipMemcpyAsync(b + l * HID, n->b[l].p, HID * 4, hipMemcpyDeviceToHost, n->stream), "get b");
+            if (gb) chk(hipMemcpyAsync(gb + l * HID, n->gb[l].p, HID * 4, hipMemcpyDeviceToHost, n->stream), "get gb");
+            ow += cnt;
+        }
+        chk(hipStreamSynchronize(n->stream), "sync");
+    });
+}
+
+int m3d_net_set_params(m3d_net* n, const float* w, const float* b) {
+    return nn_guard([&] {
+        size_t ow = 0;
+        for (int l = 0; l < NL; ++l) {
+            const size_t cnt = size_t(27) * HID * m3d_net::cin(l);
+            chk(hipMemcpyAsync(n->w[l].p, w + ow, cnt * 4, hipMemcpyHostToDevice, n->stream), "set w");
+            chk(hipMemcpyAsync(n->b[l].p, b + l * HID, HID * 4, hipMemcpyHostToDevice, n->stream), "set b");
+            ow += cnt;
+        }
+        upload_bf16(*n);
+        chk(hipStreamSynchronize(n->stream), "sync");
+    });
+}
+
+// d_in: device [batch][32^3][2] FP32; d_out: device [batch][32^3][12] FP32.
+int m3d_net_forward(m3d_net* n, int batch, const float* d_in, float* d_out) {
+    return nn_guard([&] {
+        if (batch <= 0 || batch > n->max_batch) throw std::invalid_argument("batch out of range");
+        forward(*n, batch, d_in, d_out);
+        chk(hipStreamSynchronize(n->stream), "sync");
+    });
+}
+
+// Loss and gradients (readable with m3d_net_get_params) without an update.
+int m3d_net_gradients(m3d_net* n, int batch, const float* d_in, const float* d_tgt, float* loss) {
+    return nn_guard([&] {
+        if (batch <= 0 || batch > n->max_batch) throw std::invalid_argument("batch out of range");
+        const float l = backward(*n, batch, d_in, d_tgt);
+        if (loss) *loss = l;
+    });
+}
+
+int m3d_net_prolong(m3d_net* n, int batch, const float* d_field, int n_edges, const float* d_edge_xyz,
+                    const unsigned char* d_edge_dir, const double* box, double* d_x0) {
+    return nn_guard([&] {
+        if (batch <= 0 || n_edges <= 0 || !box || !d_field || !d_edge_xyz || !d_edge_dir || !d_x0)
+            throw std::invalid_argument("invalid arguments");
+        const float hx = float((box[1] - box[0]) / G), hy = float((box[3] - box[2]) / G),
+                    hz = float((box[5] - box[4]) / G);
+        const size_t total = size_t(batch) * n_edges;
+        const int grid = int(std::min<size_t>((total + 255) / 256, 8192));
+        hipLaunchKernelGGL(k_prolong, dim3(grid), dim3(256), 0, n->stream, batch, n_edges, d_field, d_edge_xyz,
+                           d_edge_dir, float(box[0]), float(box[2]), float(box[4]), hx, hy, hz,
+                           reinterpret_cast<double2*>(d_x0));
+        chk(hipGetLastError(), "k_prolong");
+        chk(hipStreamSynchronize(n->stream), "sync");
+    });
+}
+
+int m3d_net_train_step(m3d_net* n, int batch, const float* d_in, const float* d_tgt, float lr, float* loss) {
+    return nn_guard([&] {
+        if (batch <= 0 || batch > n->max_batch) throw std::invalid_argument("batch out of range");
+        const float l = backward(*n, batch, d_in, d_tgt);
+        n->step++;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const float c1 = 1.f - std::pow(b1, float(n->step)), c2 = 1.f - std::pow(b2, float(n->step));
+        for (int k = 0; k < NL; ++k) {
+            const int cnt = 27 * HID * m3d_net::cin(k);
+            hipLaunchKernelGGL(k_adam, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, cnt, lr, b1, b2, eps, c1, c2,
+                               n->w[k].p, n->gw[k].p, n->mw[k].p, n->vw[k].p);
+            hipLaunchKernelGGL(k_adam, dim3(1), dim3(32), 0, n->stream, HID, lr, b1, b2, eps, c1, c2, n->b[k].p,
+                               n->gb[k].p, n->mb[k].p, n->vb[k].p);
+        }
+        upload_bf16(*n);
+        chk(hipStreamSynchronize(n->stream), "sync");
+        if (loss) *loss = l;
+    });
+}
+
+}  // extern "C"

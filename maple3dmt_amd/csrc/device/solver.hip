@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -62,17 +63,30 @@ struct DBuf {
         n = count;
     }
     void upload(const std::vector<T>& h, hipStream_t s) {
-        alloc(h.size());
+        if (!p || n != std::max<size_t>(h.size(), 1)) alloc(h.size());
         if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
     }
 };
 
 }  // namespace
 
+// A context lives until it has been destroyed by the caller and its last
+// operator is gone, whichever comes later.
 struct m3d_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    int operators = 0;
+    bool closed = false;
 };
+
+namespace {
+void context_release(m3d_context* ctx) {
+    if (!ctx->closed || ctx->operators > 0) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+}  // namespace
 
 namespace {
 
@@ -83,11 +97,74 @@ struct Work {
     DBuf<C> b, x, r, r0, p, v, s, t, y, ph, sh;
 };
 
+// Instantiated HIP graphs of fixed-length iteration chunks, keyed by the
+// solution vector they update and the chunk kind.  Kernel arguments only
+// reference buffers that live as long as the workspace.
+struct GraphCache {
+    struct Entry {
+        const void* key;
+        int kind;
+        int len;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> entries;
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache() { clear(); }
+    void clear() {
+        for (auto& e : entries) (void)hipGraphExecDestroy(e.exec);
+        entries.clear();
+    }
+    template <typename F>
+    hipGraphExec_t get(hipStream_t s, const void* key, int kind, int len, F&& body) {
+        for (auto& e : entries)
+            if (e.key == key && e.kind == kind && e.len == len) return e.exec;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) throw Fail(M3D_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        entries.push_back({key, kind, len, ex});
+        return ex;
+    }
+};
+
+bool use_graphs() {
+    static const bool on = [] {
+        const char* v = std::getenv("M3D_GRAPHS");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+bool use_compaction() {
+    static const bool on = [] {
+        const char* v = std::getenv("M3D_COMPACT");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+constexpr int kGraphIters = 8;     // BiCGStab iterations per graph launch
+constexpr int kGraphPcgIters = 16; // DivCorr PCG iterations per graph launch
+
 template <int NS>
 struct Work64 {
+    GraphCache graphs;
     DBuf<double2> b, x, aux, tmp, rres;             // FP64 system vectors
     DBuf<double2> phi0, dr, dz, dp, dphi, dap;      // DivCorr node vectors
     DBuf<double> omega, scale;
+    std::vector<double> h_omega;  // host copy of omega
     DBuf<SysState> st;
     DBuf<double> part;
     DBuf<unsigned> ctr;
@@ -256,6 +333,128 @@ struct BicgOutcome {
     int info[kMaxSys];
 };
 
+template <int NS, typename V>
+Work<NS, V>& vecws(m3d_operator& op, Work64<NS>& W);
+template <> Work<1, double>& vecws<1, double>(m3d_operator& op, Work64<1>& W) { return vec64<1>(op, W); }
+template <> Work<2, double>& vecws<2, double>(m3d_operator& op, Work64<2>& W) { return vec64<2>(op, W); }
+template <> Work<4, double>& vecws<4, double>(m3d_operator& op, Work64<4>& W) { return vec64<4>(op, W); }
+template <> Work<8, double>& vecws<8, double>(m3d_operator& op, Work64<8>& W) { return vec64<8>(op, W); }
+template <> Work<16, double>& vecws<16, double>(m3d_operator& op, Work64<16>& W) { return vec64<16>(op, W); }
+template <> Work<32, double>& vecws<32, double>(m3d_operator& op, Work64<32>& W) { return vec64<32>(op, W); }
+template <> Work<1, float>& vecws<1, float>(m3d_operator& op, Work64<1>& W) { return vec32<1>(op, W); }
+template <> Work<2, float>& vecws<2, float>(m3d_operator& op, Work64<2>& W) { return vec32<2>(op, W); }
+template <> Work<4, float>& vecws<4, float>(m3d_operator& op, Work64<4>& W) { return vec32<4>(op, W); }
+template <> Work<8, float>& vecws<8, float>(m3d_operator& op, Work64<8>& W) { return vec32<8>(op, W); }
+template <> Work<16, float>& vecws<16, float>(m3d_operator& op, Work64<16>& W) { return vec32<16>(op, W); }
+template <> Work<32, float>& vecws<32, float>(m3d_operator& op, Work64<32>& W) { return vec32<32>(op, W); }
+
+// Solution buffer of a narrow workspace for continued solves.
+template <int NB, typename V>
+cvec<V>* narrow_x(m3d_operator& op, Work64<NB>& W);
+template <int NB, typename V>
+struct NarrowX;
+template <int NB>
+struct NarrowX<NB, double> {
+    static double2* get(m3d_operator&, Work64<NB>& W) { return W.rres.p; }
+};
+template <int NB>
+struct NarrowX<NB, float> {
+    static float2* get(m3d_operator& op, Work64<NB>& W) { return vec32<NB>(op, W).x.p; }
+};
+
+template <int NS, typename V>
+void bicg_loop(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x,
+               std::vector<SysState>& h, int maxiter);
+
+// Continue the running systems listed in map (wide system index per narrow
+// slot) at batch width NB: move the iteration vectors (r, r0, p, v, x), the
+// frequencies and the device state, iterate, and move x and the state back.
+template <int NB, typename V>
+void continue_narrow(m3d_operator& op, int NA, const std::vector<int>& map, Work<kMaxSys, V>* /*unused*/,
+                     cvec<V>* r, cvec<V>* r0, cvec<V>* p, cvec<V>* v, cvec<V>* x,
+                     const std::vector<double>& omega_wide, std::vector<SysState>& h_wide, int maxiter) {
+    Work64<NB>& WB = ready_workspace<NB>(op);
+    Work<NB, V>& wb = vecws<NB, V>(op, WB);
+    cvec<V>* xb = NarrowX<NB, V>::get(op, WB);
+    hipStream_t s = op.ctx->stream;
+    std::vector<int> m(NB, -1);
+    std::vector<double> om(NB, 0.0);
+    std::vector<SysState> hs(kMaxSys);
+    for (int j = 0; j < kMaxSys; ++j) hs[j] = SysState{}, hs[j].status = kIdle;
+    for (int j = 0; j < int(map.size()); ++j) {
+        m[j] = map[j];
+        om[j] = omega_wide[map[j]];
+        hs[j] = h_wide[map[j]];
+    }
+    WB.h_omega = om;
+    HIPCHK(hipMemcpyAsync(WB.omega.p, om.data(), NB * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(WB.flags.p, m.data(), NB * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(WB.st.p, hs.data(), kMaxSys * sizeof(SysState), hipMemcpyHostToDevice, s));
+    const int rows = op.npad;
+    launch_regroup<V>(s, rows, NA, NB, WB.flags.p, r, wb.r.p, true);
+    launch_regroup<V>(s, rows, NA, NB, WB.flags.p, r0, wb.r0.p, true);
+    launch_regroup<V>(s, rows, NA, NB, WB.flags.p, p, wb.p.p, true);
+    launch_regroup<V>(s, rows, NA, NB, WB.flags.p, v, wb.v.p, true);
+    launch_regroup<V>(s, rows, NA, NB, WB.flags.p, x, xb, true);
+    HIPCHK(hipGetLastError());
+    bicg_loop<NB, V>(op, WB, wb, xb, hs, maxiter);
+    launch_regroup<V>(s, rows, NA, NB, WB.flags.p, x, xb, false);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    for (int j = 0; j < int(map.size()); ++j) h_wide[map[j]] = hs[j];
+}
+
+// Iterate until every system in h has stopped; when few systems are still
+// running, continue them at a narrower batch width.
+template <int NS, typename V>
+void bicg_loop(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x,
+               std::vector<SysState>& h, int maxiter) {
+    hipStream_t s = op.ctx->stream;
+    int launched = 0;
+    int chunk = 4;
+    auto running = [&]() {
+        int k = 0;
+        for (int i = 0; i < NS; ++i)
+            if (h[i].status == kRunning) ++k;
+        return k;
+    };
+    hipGraphExec_t ex = nullptr;
+    if (use_graphs())
+        ex = W.graphs.get(s, x, sizeof(V), kGraphIters, [&] {
+            for (int k = 0; k < kGraphIters; ++k) bicg_iteration<NS, V>(op, W, w, x);
+        });
+    // Iterations past a system's stop are no-ops for it (kernels check its
+    // status on the device), so whole chunks can be replayed.
+    while (launched < maxiter) {
+        const int k = running();
+        if (k == 0) break;
+        if (NS > 1 && use_compaction() && k <= NS / 4) {
+            std::vector<int> map;
+            for (int i = 0; i < NS; ++i)
+                if (h[i].status == kRunning) map.push_back(i);
+            const int left = maxiter;  // per-system maxiter is enforced on the device
+            if (k == 1) continue_narrow<1, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
+            else if (k == 2) continue_narrow<2, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
+            else if (k <= 4) continue_narrow<4, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
+            else continue_narrow<8, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
+            HIPCHK(hipMemcpyAsync(W.st.p, h.data(), kMaxSys * sizeof(SysState), hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));
+            return;
+        }
+        if (ex) {
+            HIPCHK(hipGraphLaunch(ex, s));
+            launched += kGraphIters;
+        } else {
+            const int todo = std::min(chunk, maxiter - launched);
+            for (int i = 0; i < todo; ++i) bicg_iteration<NS, V>(op, W, w, x);
+            HIPCHK(hipGetLastError());
+            launched += todo;
+            chunk = std::min(chunk * 2, 32);
+        }
+        h = read_states(op, W.st);
+    }
+}
+
 // Runs BiCGStab on the masked systems until every one has stopped.
 template <int NS, typename V>
 BicgOutcome bicgstab(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, const cvec<V>* b,
@@ -265,22 +464,8 @@ BicgOutcome bicgstab(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, const cvec
     Launch<NS, V>::bicg_init(s, op.view<V>(), W.omega.p, b, x, w.r.p, w.r0.p, W.st.p, W.part.p,
                              W.ctr.p, x_zero ? 1 : 0);
     HIPCHK(hipGetLastError());
-    int launched = 0;
-    int chunk = 4;
     std::vector<SysState> h = read_states(op, W.st);
-    auto running = [&](const std::vector<SysState>& hs) {
-        for (int i = 0; i < NS; ++i)
-            if ((mask & (1u << i)) && hs[i].status == kRunning) return true;
-        return false;
-    };
-    while (running(h) && launched < maxiter) {
-        const int todo = std::min(chunk, maxiter - launched);
-        for (int k = 0; k < todo; ++k) bicg_iteration<NS, V>(op, W, w, x);
-        HIPCHK(hipGetLastError());
-        launched += todo;
-        chunk = std::min(chunk * 2, 32);
-        h = read_states(op, W.st);
-    }
+    bicg_loop<NS, V>(op, W, w, x, h, maxiter);
     BicgOutcome o{};
     for (int i = 0; i < NS; ++i) {
         o.iterations[i] = h[i].iter;
@@ -314,16 +499,24 @@ void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, u
         }
         if (!any_dc) break;
         int chunk = 8;
-        while (any_pcg) {
-            for (int k = 0; k < chunk; ++k) {
-                L::dc_lp(s, op.nn, op.l_ptr.p, op.l_colidx.p, op.l_vals.p, W.dp.p, W.dap.p, W.st.p,
+        auto pcg_iter = [&] {
+            L::dc_lp(s, op.nn, op.l_ptr.p, op.l_colidx.p, op.l_vals.p, W.dp.p, W.dap.p, W.st.p,
+                     W.part.p, W.ctr.p);
+            L::dc_update(s, op.nn, op.l_diag.p, W.dp.p, W.dap.p, W.dphi.p, W.dr.p, W.dz.p, W.st.p,
                          W.part.p, W.ctr.p);
-                L::dc_update(s, op.nn, op.l_diag.p, W.dp.p, W.dap.p, W.dphi.p, W.dr.p, W.dz.p, W.st.p,
-                             W.part.p, W.ctr.p);
-                L::dc_p(s, op.nn, W.dz.p, W.dp.p, W.st.p);
+            L::dc_p(s, op.nn, W.dz.p, W.dp.p, W.st.p);
+        };
+        while (any_pcg) {
+            if (use_graphs()) {
+                hipGraphExec_t ex = W.graphs.get(s, W.dp.p, 100, kGraphPcgIters, [&] {
+                    for (int k = 0; k < kGraphPcgIters; ++k) pcg_iter();
+                });
+                HIPCHK(hipGraphLaunch(ex, s));
+            } else {
+                for (int k = 0; k < chunk; ++k) pcg_iter();
+                HIPCHK(hipGetLastError());
+                chunk = std::min(chunk * 2, 64);
             }
-            HIPCHK(hipGetLastError());
-            chunk = std::min(chunk * 2, 64);
             h = read_states(op, W.st);
             any_pcg = false;
             for (int i = 0; i < NS; ++i)
@@ -373,17 +566,24 @@ BicgOutcome solve_linear(m3d_operator& op, Work64<NS>& W, const double2* b, doub
         return bicgstab<NS, double>(op, W, w, b, x, mask, maxiter, prm.tol, x_zero);
     }
     // Iterative refinement: r = b - A x in FP64, A d = r in complex64 to a
-    // loose relative tolerance, x += d; stop when ||b - A x|| / ||b|| < tol.
+    // loose relative tolerance with a capped iteration count, x += d; a
+    // system converges when the FP64 true residual ||b - A x|| / ||b|| < tol.
+    // Refinement contracts only while kappa(A) * eps32 < 1; a system whose
+    // true residual falls by less than 2x in a step (the curl-curl
+    // near-null space at low frequency) is finished by FP64 BiCGStab from the
+    // refined x, so the result always meets the FP64 criterion.
     auto& w = vec32<NS>(op, W);
     hipStream_t s = op.ctx->stream;
     const size_t nv = size_t(op.npad) * NS;
     if (x_zero) Launch<NS, double>::fill_zero(s, nv, x);
     BicgOutcome total{};
-    unsigned active = mask;
-    const double inner_tol = prm.inner_tol > 0 ? prm.inner_tol : 1e-4;
+    unsigned active = mask, fallback = 0;
+    const double inner_tol = prm.inner_tol > 0 ? prm.inner_tol : 1e-3;
+    const int inner_cap = std::max(50, std::min(maxiter, 400));
     int steps = 0;
-    std::vector<double> scale(NS, 1.0);
-    for (int step = 0; step < 30 && active; ++step) {
+    std::vector<double> scale(NS, 1.0), prev(NS, 1e300);
+    std::vector<int> used(NS, 0);
+    for (int step = 0; step < 40 && active; ++step) {
         std::vector<double> tr = true_residual<NS>(op, W, x, b, W.rres.p);
         std::vector<SysState> h = read_states(op, W.st);
         for (int i = 0; i < NS; ++i) {
@@ -393,10 +593,14 @@ BicgOutcome solve_linear(m3d_operator& op, Work64<NS>& W, const double2* b, doub
                 active &= ~(1u << i);
                 total.status[i] = kConverged;
                 total.info[i] = 0;
+            } else if (tr[i] > 0.5 * prev[i] || used[i] >= maxiter) {
+                active &= ~(1u << i);
+                fallback |= 1u << i;
             }
+            prev[i] = tr[i];
         }
         if (!active) break;
-        // Scale each correction system to unit right-hand side before FP32.
+        // Scale each correction system to a unit right-hand side before FP32.
         for (int i = 0; i < NS; ++i) {
             const double rn = tr[i] * h[i].true_bnorm;
             scale[i] = rn > 0 ? rn : 1.0;
@@ -406,33 +610,32 @@ BicgOutcome solve_linear(m3d_operator& op, Work64<NS>& W, const double2* b, doub
         HIPCHK(hipMemcpyAsync(W.scale.p, inv.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
         Launch64<NS>::demote(s, nv, W.rres.p, W.scale.p, w.b.p);
         Launch<NS, float>::fill_zero(s, nv, w.x.p);
-        const int left = maxiter - 0;
-        BicgOutcome o = bicgstab<NS, float>(op, W, w, w.b.p, w.x.p, active, left, inner_tol, true);
+        BicgOutcome o = bicgstab<NS, float>(op, W, w, w.b.p, w.x.p, active, inner_cap, inner_tol, true);
         HIPCHK(hipMemcpyAsync(W.scale.p, scale.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
         Launch64<NS>::add_scaled(s, nv, w.x.p, W.scale.p, x);
         for (int i = 0; i < NS; ++i)
-            if (active & (1u << i)) total.iterations[i] += o.iterations[i];
-        ++steps;
-        // A correction solve that made no progress ends refinement for it.
-        for (int i = 0; i < NS; ++i) {
-            if (!(active & (1u << i))) continue;
-            if (o.status[i] == kBreakdown && o.iterations[i] <= 1) {
-                active &= ~(1u << i);
-                total.status[i] = kBreakdown;
-                total.info[i] = 2;
+            if (active & (1u << i)) {
+                total.iterations[i] += o.iterations[i];
+                used[i] += o.iterations[i];
             }
+        ++steps;
+    }
+    fallback |= active;  // out of refinement steps
+    if (fallback) {
+        auto& w64 = vec64<NS>(op, W);
+        int left = 0;
+        for (int i = 0; i < NS; ++i)
+            if (fallback & (1u << i)) left = std::max(left, maxiter - used[i]);
+        BicgOutcome o = bicgstab<NS, double>(op, W, w64, b, x, fallback, std::max(left, 1), prm.tol, false);
+        for (int i = 0; i < NS; ++i) {
+            if (!(fallback & (1u << i))) continue;
+            total.iterations[i] += o.iterations[i];
+            total.status[i] = o.status[i];
+            total.info[i] = o.info[i];
+            total.residual[i] = o.residual[i];
         }
     }
-    std::vector<double> tr = true_residual<NS>(op, W, x, b, W.rres.p);
-    for (int i = 0; i < NS; ++i) {
-        if (!(mask & (1u << i))) continue;
-        total.residual[i] = tr[i];
-        if (active & (1u << i)) {
-            total.status[i] = tr[i] < prm.tol ? kConverged : kMaxIter;
-            total.info[i] = tr[i] < prm.tol ? 0 : 3;
-        }
-    }
-    *outer_steps = steps;
+    *outer_steps = steps + (fallback ? 1 : 0);
     return total;
 }
 
@@ -444,6 +647,7 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
     hipStream_t s = op.ctx->stream;
     std::vector<double> om(NS, 0.0);
     for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    W.h_omega = om;
     HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
     const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
     using L = Launch<NS, double>;
@@ -456,15 +660,35 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
     if (mode == M3D_MODE_FORWARD) {
         L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_aux, W.aux.p);
         L::spmv(s, op.view<double>(), W.omega.p, W.aux.p, W.b.p, W.tmp.p);  // rhs_sec
-        L::fill_zero(s, nv, W.x.p);
+        bool x_zero = true;
+        std::vector<int> used(NS, 0);
+        if (prm.use_initial_guess) {
+            // Caller's E_sec guess, kept only where ||b - A e0|| < ||b||;
+            // elsewhere the reference's zero start.
+            L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
+            L64::fix_constrained(s, op.npad, op.constrained.p, W.tmp.p, W.x.p);
+            std::vector<double> tr = true_residual<NS>(op, W, W.x.p, W.tmp.p, nullptr);
+            std::vector<int> zero(NS, 0);
+            for (int i = 0; i < NS; ++i) {
+                used[i] = (i < nsys && tr[i] < 1.0) ? 1 : 0;
+                zero[i] = used[i] ? 0 : 1;
+            }
+            HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int), hipMemcpyHostToDevice, s));
+            L64::zero_systems(s, op.npad, W.flags.p, W.x.p);
+            x_zero = false;
+        } else {
+            L::fill_zero(s, nv, W.x.p);
+        }
         int steps = 1;
-        BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, mask, prm.maxiter, prm, true, &steps);
+        BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, mask, prm.maxiter, prm, x_zero, &steps);
+        for (int i = 0; i < nsys; ++i) res[i].guess_used = used[i];
         interp_all<NS>(op, W.x.p);
         L64::forward_total(s, op.npad, op.bc.p, W.aux.p, W.b.p, W.x.p);
         if (dc) divcorr<NS>(op, W, W.x.p, nullptr, mask, prm);
         for (int i = 0; i < nsys; ++i) {
             fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
             res[i].outer = steps;
+            res[i].guess_used = used[i];
         }
     } else if (mode == M3D_MODE_INNER || mode == M3D_MODE_SYSTEM) {
         L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
@@ -757,7 +981,17 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
     HIPCHK(hipStreamSynchronize(s));
 }
 
+void clear_graphs(m3d_operator& op) {
+    if (op.w1) op.w1->graphs.clear();
+    if (op.w2) op.w2->graphs.clear();
+    if (op.w4) op.w4->graphs.clear();
+    if (op.w8) op.w8->graphs.clear();
+    if (op.w16) op.w16->graphs.clear();
+    if (op.w32) op.w32->graphs.clear();
+}
+
 void set_mass(m3d_operator& op, const double* me) {
+    clear_graphs(op);
     std::vector<double> m(op.npad, 0.0);
     for (int r = 0; r < op.npad; ++r) {
         const int e = op.perm[r];
@@ -771,6 +1005,7 @@ void set_mass(m3d_operator& op, const double* me) {
 }
 
 void set_divcorr(m3d_operator& op, const m3d_divcorr_desc& d) {
+    clear_graphs(op);
     const int n = op.n, nn = d.nn;
     hipStream_t s = op.ctx->stream;
     if (nn <= 0) throw Fail(M3D_ERR_ARG, "DivCorr needs at least one node");
@@ -849,10 +1084,9 @@ int m3d_context_create(int device, m3d_context** out) {
 
 int m3d_context_destroy(m3d_context* ctx) {
     return guarded([&] {
-        if (!ctx) return;
-        (void)hipSetDevice(ctx->device);
-        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-        delete ctx;
+        if (!ctx || ctx->closed) return;
+        ctx->closed = true;
+        context_release(ctx);
     });
 }
 
@@ -870,18 +1104,23 @@ int m3d_operator_create(m3d_context* ctx, const m3d_operator_desc* desc, m3d_ope
             if (desc->colidx[k] < 0 || desc->colidx[k] >= desc->n) throw Fail(M3D_ERR_ARG, "column index out of range");
         HIPCHK(hipSetDevice(ctx->device));
         std::unique_ptr<m3d_operator> op(new m3d_operator);
+        if (ctx->closed) throw Fail(M3D_ERR_STATE, "context already destroyed");
         op->ctx = ctx;
         build_operator(*op, *desc);
         *out = op.release();
+        ctx->operators++;
     });
 }
 
 int m3d_operator_destroy(m3d_operator* op) {
     return guarded([&] {
         if (!op) return;
-        (void)hipSetDevice(op->ctx->device);
-        (void)hipStreamSynchronize(op->ctx->stream);
+        m3d_context* ctx = op->ctx;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
         delete op;
+        ctx->operators--;
+        context_release(ctx);
     });
 }
 
@@ -977,6 +1216,7 @@ void spmv_impl(m3d_operator& op, int nsys, const double* omega, const double2* d
     hipStream_t s = op.ctx->stream;
     std::vector<double> om(NS, 0.0);
     for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    W.h_omega = om;
     HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
     Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
     Launch<NS, double>::spmv(s, op.view<double>(), W.omega.p, W.x.p, nullptr, W.tmp.p);
@@ -991,6 +1231,7 @@ void ssor_impl(m3d_operator& op, int nsys, const double* omega, const double2* d
     hipStream_t s = op.ctx->stream;
     std::vector<double> om(NS, 0.0);
     for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    W.h_omega = om;
     HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
     const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
     launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
@@ -1015,6 +1256,7 @@ void divcorr_host_impl(m3d_operator& op, int nsys, const double* omega, double2*
     hipStream_t s = op.ctx->stream;
     std::vector<double> om(NS, 0.0);
     for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    W.h_omega = om;
     HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
     Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_E, W.x.p);
     const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
@@ -1036,6 +1278,7 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     hipStream_t s = op.ctx->stream;
     std::vector<double> om(NS, 0.0);
     for (int i = 0; i < nsys; ++i) om[i] = omega[i];
+    W.h_omega = om;
     HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
     const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
     launch_bicg_begin(s, W.st.p, mask, 1, 1.0);

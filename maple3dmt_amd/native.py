@@ -67,6 +67,7 @@ class SolveParamsC(C.Structure):
         ("divcorr_iter_per_dc", C.c_int),
         ("precision", C.c_int),
         ("inner_tol", C.c_double),
+        ("use_initial_guess", C.c_int),
     ]
 
 
@@ -79,6 +80,7 @@ class SolveResultC(C.Structure):
         ("diverged", C.c_int),
         ("info", C.c_int),
         ("outer", C.c_int),
+        ("guess_used", C.c_int),
     ]
 
 
@@ -249,17 +251,18 @@ class SolveParams:
     divcorr_iter_per_dc: int = 500
     precision: int = PREC_FP64
     inner_tol: float = 0.0
+    use_initial_guess: bool = False
 
     def c(self) -> SolveParamsC:
         return SolveParamsC(self.tol, self.maxiter, int(self.div_correction), self.div_corr_iters,
                             self.div_corr_tol, self.divcorr_outer_max, self.divcorr_iter_per_dc,
-                            self.precision, self.inner_tol)
+                            self.precision, self.inner_tol, int(self.use_initial_guess))
 
 
 def results_to_dicts(res) -> list:
     return [dict(iterations=r.iterations, residual=r.residual, true_residual=r.true_residual,
                  converged=bool(r.converged), diverged=bool(r.diverged), info=r.info,
-                 outer=r.outer) for r in res]
+                 outer=r.outer, guess_used=bool(r.guess_used)) for r in res]
 
 
 # --------------------------------------------------------------------------

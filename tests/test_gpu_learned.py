@@ -1,0 +1,151 @@
+"""Device network (BF16 MFMA conv3d) against the NumPy oracle, device e0
+prolongation against NumPy, and the initial-guess path of the FORWARD solve.
+
+Tolerances: the device and the oracle apply the same BF16 roundings to
+weights, inputs, activations and inter-layer gradients, but accumulate in FP32
+(device) vs float64 (oracle); a BF16 rounding that flips near a tie moves a
+value by 2^-8 relative, so outputs are compared to 1e-2 and gradients to 3e-2
+in relative norm.
+"""
+import numpy as np
+import pytest
+
+from maple3dmt_amd import configs
+from maple3dmt_amd.learned import Box, CoarseNet, deposit, initial_guess
+from maple3dmt_amd.native import MODE_FORWARD, SolveParams
+from oracle import nn_oracle as no
+
+pytestmark = pytest.mark.gpu
+
+G = 32
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    return torch, torch.device("cuda:0")
+
+
+def _random_params(net, rng):
+    w, b = net.params()
+    ws, _ = no.unpack(w, b)
+    for l, x in enumerate(ws):
+        x[...] = rng.standard_normal(x.shape).astype(np.float32) * (0.6 / np.sqrt(27 * x.shape[2]))
+    ws[0][:, :, 2:] = 0
+    ws[3][:, 12:, :] = 0
+    w = np.concatenate([x.ravel() for x in ws])
+    b = (rng.standard_normal(b.shape) * 0.05).astype(np.float32)
+    net.set_params(w, b)
+    return w, b
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_and_gradients_match_oracle(ctx, torch_dev):
+    torch, dev = torch_dev
+    rng = np.random.default_rng(0)
+    net = CoarseNet(ctx, max_batch=2)
+    w, b = _random_params(net, rng)
+    x = rng.standard_normal((2, G, G, G, 2)).astype(np.float32)
+    t = (rng.standard_normal((2, G, G, G, 12)) * 0.5).astype(np.float32)
+    dx = torch.from_numpy(x.reshape(2, -1, 2)).to(dev)
+    dt = torch.from_numpy(t.reshape(2, -1, 12)).to(dev)
+    out = torch.empty((2, G ** 3, 12), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    net.forward(dx.data_ptr(), out.data_ptr(), 2)
+    ref, _ = no.forward(w, b, x, dil=net.dil)
+    assert _rel(out.cpu().numpy().reshape(ref.shape), ref) <= 1e-2
+    loss = net.gradients(dx.data_ptr(), dt.data_ptr(), 2)
+    gw, gb = net.params(grads=True)
+    rloss, rgw, rgb = no.gradients(w, b, x, t, dil=net.dil)
+    assert abs(loss - rloss) <= 1e-3 * rloss
+    offs = np.cumsum([0] + [27 * 32 * c for c in no.CINS])
+    for l in range(4):
+        seg = slice(offs[l], offs[l + 1])
+        assert _rel(gw[seg], rgw[seg]) <= 3e-2, (l, _rel(gw[seg], rgw[seg]))
+        assert _rel(gb[l * 32:(l + 1) * 32], rgb[l * 32:(l + 1) * 32]) <= 3e-2
+    net.close()
+
+
+def test_training_reduces_loss(ctx, torch_dev):
+    torch, dev = torch_dev
+    rng = np.random.default_rng(1)
+    net = CoarseNet(ctx, max_batch=2, seed=3)
+    x = rng.standard_normal((2, G ** 3, 2)).astype(np.float32)
+    # A smooth target the network can represent: a linear map of the inputs.
+    t = np.zeros((2, G ** 3, 12), np.float32)
+    t[..., 0] = 0.5 * x[..., 0]
+    t[..., 7] = -0.25 * x[..., 1]
+    dx = torch.from_numpy(x).to(dev)
+    dt = torch.from_numpy(t).to(dev)
+    torch.cuda.synchronize()
+    losses = [net.train_step(dx.data_ptr(), dt.data_ptr(), 2, 2e-3) for _ in range(60)]
+    assert losses[-1] < 0.5 * losses[0]
+    net.close()
+
+
+def test_prolongation_matches_numpy(ctx, torch_dev):
+    torch, dev = torch_dev
+    wl = configs.halfspace()
+    m = configs.build_model(wl, ctx)
+    box = Box()
+    rng = np.random.default_rng(2)
+    field = rng.standard_normal((1, G ** 3, 12)).astype(np.float32)
+    net = CoarseNet(ctx, max_batch=1)
+    exyz = np.stack([m.array("edge_x"), m.array("edge_y"), m.array("edge_z")], 1).astype(np.float32)
+    edir = m.array("edge_dir")
+    d_f = torch.from_numpy(field).to(dev)
+    d_xyz = torch.from_numpy(exyz).to(dev)
+    d_dir = torch.from_numpy(edir).to(dev)
+    x0 = torch.zeros((2, m.n), dtype=torch.complex128, device=dev)
+    torch.cuda.synchronize()
+    net.prolong(d_f.data_ptr(), 1, m.n, d_xyz.data_ptr(), d_dir.data_ptr(), box, x0.data_ptr())
+    got = x0.cpu().numpy()
+    h = (box.x1 - box.x0) / G, (box.y1 - box.y0) / G, (box.z1 - box.z0) / G
+    f = field[0].reshape(G, G, G, 12)
+    ref = np.zeros((2, m.n), np.complex128)
+    for e in range(m.n):
+        fx = (exyz[e, 0] - box.x0) / h[0] - 0.5
+        fy = (exyz[e, 1] - box.y0) / h[1] - 0.5
+        fz = (exyz[e, 2] - box.z0) / h[2] - 0.5
+        if min(fx, fy, fz) < 0 or max(fx, fy, fz) > G - 1:
+            continue
+        ix, iy, iz = min(int(fx), G - 2), min(int(fy), G - 2), min(int(fz), G - 2)
+        tx, ty, tz = fx - ix, fy - iy, fz - iz
+        v = np.zeros(12)
+        for c in range(8):
+            cx, cy, cz = c & 1, (c >> 1) & 1, c >> 2
+            wgt = (tx if cx else 1 - tx) * (ty if cy else 1 - ty) * (tz if cz else 1 - tz)
+            v += wgt * f[iz + cz, iy + cy, ix + cx]
+        d = edir[e]
+        ref[0, e] = v[2 * d] + 1j * v[2 * d + 1]
+        ref[1, e] = v[6 + 2 * d] + 1j * v[6 + 2 * d + 1]
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+    net.close()
+
+
+def test_forward_initial_guess_and_fallback(ctx, torch_dev):
+    torch, dev = torch_dev
+    wl = configs.random3d(7, min_level=3, max_level=5, n_freq=2, half_width=50e3,
+                          station_radius=5e3)
+    m = configs.build_model(wl, ctx)
+    omegas = np.repeat(2 * np.pi * wl.freqs, 2)
+    rhs = np.stack([m.rhs_total(m.bc_values(w, p % 2)) for p, w in enumerate(omegas)])
+    aux = np.stack([m.primary_field(w, p % 2) for p, w in enumerate(omegas)])
+    prm = m.fwd_params.solve_params()
+    x_ref, r_ref = m.device.solve(MODE_FORWARD, omegas, rhs, aux, params=prm)
+    esec = x_ref - aux
+    # The converged E_sec as the guess: kept, and far fewer iterations.
+    prm_g = m.fwd_params.solve_params()
+    prm_g.use_initial_guess = True
+    x1, r1 = m.device.solve(MODE_FORWARD, omegas, rhs, aux, x=esec, params=prm_g)
+    assert all(r["guess_used"] and r["converged"] for r in r1)
+    assert sum(r["iterations"] for r in r1) < 0.2 * sum(r["iterations"] for r in r_ref)
+    assert np.abs(x1 - x_ref).max() <= 1e-4 * np.abs(x_ref).max()
+    # A useless guess falls back to the zero start and gives the same answer.
+    bad = esec * 1e6
+    x2, r2 = m.device.solve(MODE_FORWARD, omegas, rhs, aux, x=bad, params=prm_g)
+    assert not any(r["guess_used"] for r in r2)
+    assert [r["iterations"] for r in r2] == [r["iterations"] for r in r_ref]

@@ -196,8 +196,11 @@ def test_adjoint_interleaved_matches_oracle(block):
 
 
 def test_batched_matches_single(block):
+    # Batch width changes the block decomposition of the reductions, so the
+    # iterates differ by rounding; both stop at the 1e-7 residual.
     wl, m, o = block
     E2, Z2, _ = m.solve_frequencies(wl.freqs)
     for fi, f in enumerate(wl.freqs):
         E1, Z1, _ = m.solve_frequencies([f])
-        assert np.abs(Z1[0] - Z2[fi]).max() <= 1e-9 * np.abs(Z2[fi]).max()
+        rel = float(np.abs(Z1[0] - Z2[fi]).max() / np.abs(Z2[fi]).max())
+        assert rel <= 1e-6, rel
