@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--half-width", type=float, default=100e3)
     ap.add_argument("--station-radius", type=float, default=80e3)
     ap.add_argument("--max-level", type=int, default=7)
+    ap.add_argument("--maxiter", type=int, default=40000,
+                    help="BiCGStab iteration cap per system (the 1e-3 Hz systems of the plain "
+                         "FP64 method need 12-25 thousand)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
@@ -60,12 +63,19 @@ def kernel_only(args):
     omegas = np.repeat(2 * np.pi * wl.freqs, 2)
     ms, by = model.device.bench_kernel(0, omegas, reps=args.reps)
     ms2, by2 = model.device.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
+    iters = {}
+    for kind, tag in ((2, "fp64"), (3, "c64")):
+        for k in (1, 2, 4, 16):
+            t, b = model.device.bench_kernel(kind, omegas[:k], reps=max(1, args.reps // 4))
+            iters[f"{tag}_ns{k}"] = {"ms": t, "alg_GBs": b / t / 1e6}
+            log(f"[kernel-only] BiCGStab iteration {tag} x{k}: {t:.3f} ms, {b / t / 1e6:.0f} GB/s")
     x = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
     y = torch.empty_like(x)
     for _ in range(4):
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"spmv_ms": ms, "spmv_alg_bytes": by, "ssor_ms": ms2, "ssor_alg_bytes": by2,
+                      "iteration": iters,
                       "copy_bytes_each_way": x.numel() * 8, "copies": 4, "edges": model.n}),
           flush=True)
 
@@ -91,7 +101,7 @@ def main():
 
     t0 = time.time()
     wl = configs.random3d(args.seed, max_level=args.max_level, half_width=args.half_width,
-                          station_radius=args.station_radius)
+                          station_radius=args.station_radius, maxiter=args.maxiter)
     ctx = Context(0)
     model = configs.build_model(wl, ctx)
     op = model.device

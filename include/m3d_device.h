@@ -113,8 +113,11 @@ typedef struct {
     int divcorr_iter_per_dc; /* divcorr_iter_per_dc */
     int precision;           /* M3D_PREC_FP64 or M3D_PREC_MIXED */
     double inner_tol;        /* MIXED: relative tolerance of each complex64 inner solve */
-    int use_initial_guess;   /* FORWARD: x holds an E_sec guess (e.g. the learned e0); it is
-                                used where ||b - A e0|| < ||b||, else the zero start */
+    int use_initial_guess;   /* FORWARD: x holds an E_sec guess (e.g. the learned e0).
+                                1: used as given where ||b - A e0|| < ||b||;
+                                2: scaled per system by gamma = (A e0, b) / (A e0, A e0), the
+                                least-squares factor, and used where ||b - gamma A e0|| <
+                                0.999 ||b||.  Elsewhere the zero start. */
 } m3d_solve_params;
 
 /* Per-system outcome; mirrors BiCGStabResult (bicgstab.h:23-28) plus what the
@@ -170,7 +173,9 @@ int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long*
 /* Measurement (no reference counterpart): average duration in ms of one
  * launch of a kernel class over `reps` launches, timed with HIP events on the
  * context stream, and its algorithmic HBM bytes per launch.
- * kind 0: batched SpMV y = A x; kind 1: one multicolour SSOR apply. */
+ * kind 0: batched SpMV y = A x; kind 1: one multicolour SSOR apply;
+ * kind 2: one FP64 BiCGStab iteration (b = 1, tolerance 0, so every system
+ * keeps running); kind 3: the same iteration in complex64. */
 int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
                      double* avg_ms, double* alg_bytes);
 

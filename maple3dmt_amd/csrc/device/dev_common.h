@@ -12,8 +12,9 @@ namespace dev {
 
 constexpr int kBlock = 256;        // threads per block (4 wavefronts of 64)
 constexpr int kSlice = 64;         // SELL slice height = one wavefront of rows
-constexpr int kMaxBlocks = 1024;   // grid cap for reduction kernels (4 blocks per CU)
+constexpr int kMaxBlocks = 2048;   // grid cap for reduction kernels (8 blocks per CU)
 constexpr int kMaxSys = 32;        // systems per batched launch
+constexpr int kXcds = 8;           // accelerator complex dies (one L2 each)
 
 // Complex value types per precision.
 template <typename V> struct CT;
@@ -77,6 +78,8 @@ struct SysState {
     int dc_status, pcg_status, pcg_iter, pcg_max;
     // True residual / refinement
     double true_res, true_bnorm;
+    // Least-squares scale of an initial guess
+    double2 guess_scale;
     int pad0, pad1;
 };
 
@@ -96,63 +99,103 @@ struct DevOp {
     const V* me;           // per row; 0 on constrained / padding rows
 };
 
-// Sum NQ per-thread values over the threads of each system in the block and
-// then over all blocks.  Returns true in exactly one block (the last to
-// finish), whose threads tid < NS then hold the grid totals in out[q].
-// Partials are summed in block order, so results are run-to-run identical.
+// Row-block schedule.  The dispatcher deals workgroups to the XCDs round
+// robin (workgroup b runs on XCD b % 8), so a plain grid-stride loop spreads
+// neighbouring rows over all eight L2s and every x gather misses.  Here XCD j
+// walks the contiguous j-th eighth of the row blocks: rows that gather the
+// same x entries run under one L2.  gridDim.x must be a multiple of kXcds.
+// c_row_sched selects it (1) or the plain grid stride (0), under which all
+// XCDs share one moving window of rows; set per device by set_row_schedule.
+static __constant__ int c_row_sched = 0;
+struct XcdRange {
+    int first, end, step;
+};
+__device__ inline XcdRange xcd_range(int nrb) {
+    if (c_row_sched == 0) {
+        XcdRange r;
+        r.first = blockIdx.x;
+        r.end = nrb;
+        r.step = gridDim.x;
+        return r;
+    }
+    const int xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const int per = gridDim.x / kXcds;
+    const int chunk = (nrb + kXcds - 1) / kXcds;
+    XcdRange r;
+    r.first = xcd * chunk + local;
+    r.end = min(nrb, (xcd + 1) * chunk);
+    r.step = per;
+    return r;
+}
+
+// Reductions run in two kernels.  The row kernel leaves one partial sum per
+// (block, system, quantity); a one-block finishing kernel, ordered after it on
+// the stream, adds them up and updates the solver scalars.  The kernel
+// boundary makes the partials visible, so no block issues a device-scope
+// fence (on CDNA that writes back the XCD's L2) or a same-address atomic.
+
+constexpr int kFinBlock = 1024;  // threads of a finishing kernel
+
+// Block stage: sums NQ per-thread values over the threads of each system in
+// the block and stores them block-major, part[blockIdx * P + p] with
+// P = NQ * NS pairs, p = q * NS + sys.  All threads must call it.
 template <int NS, int NQ>
-__device__ inline bool grid_reduce(double (&v)[NQ], double* part, unsigned* counter,
-                                   double (&out)[NQ]) {
-    __shared__ double sm[kBlock / 64][NQ][NS];
-    __shared__ double fin[kBlock];
-    __shared__ bool last;
+__device__ inline void block_partials(const double (&v)[NQ], double* part) {
+    constexpr int P = NQ * NS;
+    __shared__ double sm[kBlock / 64][P];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         double x = v[q];
         for (int off = 32; off >= NS; off >>= 1) x += __shfl_xor(x, off);
-        if (lane < NS) sm[wave][q][lane] = x;
+        if (lane < NS) sm[wave][q * NS + lane] = x;
     }
     __syncthreads();
-    const int nb = gridDim.x;
-    if (tid < NS * NQ) {
-        const int q = tid / NS, s = tid % NS;
+    if (tid < P) {
         double acc = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) acc += sm[w][q][s];
-        part[size_t(q * NS + s) * kMaxBlocks + blockIdx.x] = acc;
+        for (int w = 0; w < kBlock / 64; ++w) acc += sm[w][tid];
+        part[size_t(blockIdx.x) * P + tid] = acc;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned t = atomicAdd(counter, 1u);
-        last = (t == unsigned(nb - 1));
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    // P = NQ*NS pairs; G groups of threads share one pair.
+}
+
+// Finish stage, in a kernel of one block of kFinBlock threads: the totals of
+// the nb block partials.  Returns true for threads tid < NS, which hold the
+// totals of system tid in out[q].  Each pair is summed in a fixed order, so
+// results are run-to-run identical.  All threads must call it.
+template <int NS, int NQ>
+__device__ inline bool finish_partials(const double* __restrict__ part, int nb, double (&out)[NQ]) {
     constexpr int P = NQ * NS;
-    constexpr int G = (kBlock / P) > 0 ? (kBlock / P) : 1;
+    constexpr int G = kFinBlock / P;  // thread groups per pair
+    constexpr int S = G * P;          // threads that load
+    __shared__ double fin[kFinBlock];
+    const int tid = threadIdx.x;
     double acc = 0.0;
-    if (tid < P * G) {
-        const int p = tid % P, g = tid / P;
-        const double* src = part + size_t(p) * kMaxBlocks;
-        for (int b = g; b < nb; b += G)
-            acc += __hip_atomic_load(src + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < S) {
+        // Thread tid owns pair tid % P and blocks tid / P, tid / P + G, ...
+        const size_t total = size_t(nb) * P;
+        size_t i = tid;
+        for (; i + 3 * size_t(S) < total; i += 4 * size_t(S)) {
+            const double a0 = part[i], a1 = part[i + S], a2 = part[i + 2 * S], a3 = part[i + 3 * S];
+            acc += a0;
+            acc += a1;
+            acc += a2;
+            acc += a3;
+        }
+        for (; i < total; i += S) acc += part[i];
     }
     fin[tid] = acc;
     __syncthreads();
-    if (tid < NS) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int p = q * NS + tid;
-            double s = 0.0;
-            for (int g = 0; g < G; ++g) s += fin[g * P + p];
-            out[q] = s;
-        }
+    if (tid >= NS) return false;
+    // Kept rolled: unrolling G LDS loads per pair spills registers.
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) {
+        const int p = q * NS + tid;
+        double s = 0.0;
+#pragma unroll 1
+        for (int g = 0; g < G; ++g) s += fin[g * P + p];
+        out[q] = s;
     }
-    if (tid == 0) *counter = 0u;
     return true;
 }
 

@@ -6,6 +6,10 @@
 namespace m3d {
 namespace dev {
 
+// Row schedule of the row kernels on the current device (0 = grid stride,
+// 1 = one contiguous eighth of the rows per XCD).
+hipError_t set_row_schedule(int mode);
+
 template <int NS, typename V>
 struct Launch {
     using C = cvec<V>;
@@ -14,20 +18,22 @@ struct Launch {
     static void unpack(hipStream_t s, int npad, int n, int nsys, const int* perm, const C* src,
                        double2* dst);
     static void bicg_init(hipStream_t s, const DevOp<V>& op, const double* omega, const C* b,
-                          const C* x, C* r, C* r0, SysState* st, double* part, unsigned* ctr,
+                          const C* x, C* r, C* r0, SysState* st, double* part,
                           int x_zero);
     static void sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
                           bool pupd, const C* in, C* p, const C* v, C* y, const SysState* st);
     static void sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
                           const C* y, C* z, const SysState* st);
+    // ys != nullptr: z is SSOR-preconditioned and ys its forward-sweep output
+    // (Eisenstat form, lower couplings only); nullptr: plain A z.
     static void spmv_v(hipStream_t s, const DevOp<V>& op, const double* omega, const C* z,
-                       const C* r0, C* v, SysState* st, double* part, unsigned* ctr);
+                       const C* ys, const C* r0, C* v, SysState* st, double* part);
     static void bicg_s(hipStream_t s, int n, const C* r, const C* v, C* sv, SysState* st,
-                       double* part, unsigned* ctr);
+                       double* part);
     static void spmv_t(hipStream_t s, const DevOp<V>& op, const double* omega, const C* z,
-                       const C* sv, C* t, SysState* st, double* part, unsigned* ctr);
+                       const C* ys, const C* sv, C* t, SysState* st, double* part);
     static void bicg_xr(hipStream_t s, int n, C* x, const C* ph, const C* sh, const C* sv,
-                        const C* t, const C* r0, C* r, SysState* st, double* part, unsigned* ctr);
+                        const C* t, const C* r0, C* r, SysState* st, double* part);
     static void spmv(hipStream_t s, const DevOp<V>& op, const double* omega, const C* x, const C* b,
                      C* y);
     static void fill_zero(hipStream_t s, size_t count, C* x);
@@ -38,7 +44,10 @@ template <int NS>
 struct Launch64 {
     static void true_res(hipStream_t s, const DevOp<double>& op, const double* omega,
                          const double2* x, const double2* b, double2* r, SysState* st,
-                         double* part, unsigned* ctr);
+                         double* part);
+    // x *= gamma per system, gamma = (A x, b) / (A x, A x).
+    static void scale_guess(hipStream_t s, const DevOp<double>& op, const double* omega,
+                            const double2* b, double2* x, SysState* st, double* part);
     static void zero_systems(hipStream_t s, int npad, const int* flags, double2* x);
     static void forward_total(hipStream_t s, int npad, const unsigned char* bc,
                               const double2* epri, const double2* b, double2* x);
@@ -56,12 +65,12 @@ struct Launch64 {
                         const int* nt_edge, const double* nt_val, const double* ms,
                         const unsigned char* bc_node, const double* ldiag, const double2* E,
                         const double2* phi0, double2* r, double2* z, double2* p, double2* phi,
-                        SysState* st, double* part, unsigned* ctr);
+                        SysState* st, double* part);
     static void dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
-                      const double2* p, double2* ap, SysState* st, double* part, unsigned* ctr);
+                      const double2* p, double2* ap, SysState* st, double* part);
     static void dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
                           const double2* ap, double2* phi, double2* r, double2* z, SysState* st,
-                          double* part, unsigned* ctr);
+                          double* part);
     static void dc_p(hipStream_t s, int nn, const double2* z, double2* p, const SysState* st);
     static void dc_correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
                            const unsigned char* bc_edge, const double2* phi, double2* E,

@@ -48,6 +48,23 @@ __device__ inline cvec<V> apply_row(const DevOp<V>& op, int row, int NS, int sys
     return y;
 }
 
+// (A z)_row for the SSOR-preconditioned direction z = (D+U)^-1 D y with
+// y = (D+L)^-1 p, by Eisenstat's identity A z = L z + (D+U) z = L z + D y:
+// only the lower (earlier-colour) couplings are gathered.
+template <typename V>
+__device__ inline cvec<V> apply_row_eis(const DevOp<V>& op, int row, int NS, int sys, V w,
+                                        const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ y) {
+    const int sl = row >> 6, lane = row & 63;
+    V ar = 0, ai = 0;
+    row_gather(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, z, NS, sys, ar, ai);
+    const V dr = op.diag_s[row], di = w * op.me[row];
+    const cvec<V> yi = y[size_t(row) * NS + sys];
+    cvec<V> out;
+    out.x = ar + (dr * yi.x - di * yi.y);
+    out.y = ai + (dr * yi.y + di * yi.x);
+    return out;
+}
+
 template <typename V>
 __device__ inline cvec<V> to_v(double2 a) {
     cvec<V> r;
@@ -122,14 +139,15 @@ template <int NS, typename V>
 __global__ void k_bicg_init(DevOp<V> op, const double* __restrict__ omega,
                             const cvec<V>* __restrict__ b, const cvec<V>* __restrict__ x,
                             cvec<V>* __restrict__ r, cvec<V>* __restrict__ r0, SysState* st,
-                            double* part, unsigned* counter, int x_zero) {
+                            double* part, int x_zero) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = st[sys].status == kRunning;
     const V w = V(omega[sys]);
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n || !run) continue;
         const size_t i = size_t(row) * NS + sys;
@@ -148,9 +166,14 @@ __global__ void k_bicg_init(DevOp<V> op, const double* __restrict__ omega,
         const double2 rr = cconjmul(rd, rd);
         acc[2] += rr.x;
     }
+    block_partials<NS, 3>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_bicg_init_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[3];
-    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 3>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.status != kRunning) return;
     double bn = sqrt(out[0]);
@@ -191,7 +214,8 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
     const int nrb = (nrows + rpb - 1) / rpb;
     const bool first = S.iter == 0;
     const V br = V(S.beta.x), bi = V(S.beta.y), wr = V(S.omega.x), wi = V(S.omega.y);
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = r0 + rb * rpb + tid / NS;
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
@@ -232,7 +256,8 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
     const V w = V(omega[sys]);
     const int nrows = r1 - r0;
     const int nrb = (nrows + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = r0 + rb * rpb + tid / NS;
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
@@ -250,29 +275,37 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
 }
 
 // v = A p_hat and (r0, v); alpha = rho / (r0, v) (bicgstab.cpp:76-87).
+// With ys (the forward-sweep output of p_hat) v = L p_hat + D ys.
 template <int NS, typename V>
 __global__ void k_spmv_v(DevOp<V> op, const double* __restrict__ omega,
-                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ r0,
-                         cvec<V>* __restrict__ v, SysState* st, double* part, unsigned* counter) {
+                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys,
+                         const cvec<V>* __restrict__ r0, cvec<V>* __restrict__ v, SysState* st,
+                         double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = st[sys].status == kRunning;
     const V w = V(omega[sys]);
     double acc[2] = {0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n || !run) continue;
         const size_t i = size_t(row) * NS + sys;
-        const cvec<V> y = apply_row(op, row, NS, sys, w, z);
+        const cvec<V> y = ys ? apply_row_eis(op, row, NS, sys, w, z, ys) : apply_row(op, row, NS, sys, w, z);
         v[i] = y;
         const double2 d = cconjmul(to_d<V>(r0[i]), to_d<V>(y));
         acc[0] += d.x;
         acc[1] += d.y;
     }
+    block_partials<NS, 2>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_spmv_v_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[2];
-    if (!grid_reduce<NS, 2>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 2>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.status != kRunning) return;
     const double2 r0v = mk(out[0], out[1]);
@@ -290,14 +323,15 @@ __global__ void k_spmv_v(DevOp<V> op, const double* __restrict__ omega,
 // (bicgstab.cpp:89-103); x += alpha p_hat is applied by k_bicg_xr.
 template <int NS, typename V>
 __global__ void k_bicg_s(int n, const cvec<V>* __restrict__ r, const cvec<V>* __restrict__ v,
-                         cvec<V>* __restrict__ s, SysState* st, double* part, unsigned* counter) {
+                         cvec<V>* __restrict__ s, SysState* st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = st[sys].status == kRunning;
     const V ar = V(st[sys].alpha.x), ai = V(st[sys].alpha.y);
     double acc[1] = {0.0};
     const int nrb = (n + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= n || !run) continue;
         const size_t i = size_t(row) * NS + sys;
@@ -308,9 +342,14 @@ __global__ void k_bicg_s(int n, const cvec<V>* __restrict__ r, const cvec<V>* __
         s[i] = si;
         acc[0] += cabs2(to_d<V>(si));
     }
+    block_partials<NS, 1>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_bicg_s_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[1];
-    if (!grid_reduce<NS, 1>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 1>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.status != kRunning) return;
     S.snorm = sqrt(out[0]);
@@ -323,21 +362,24 @@ __global__ void k_bicg_s(int n, const cvec<V>* __restrict__ r, const cvec<V>* __
 }
 
 // t = A s_hat, (t,t), (t,s); omega = (t,s)/(t,t) (bicgstab.cpp:112-125).
+// With ys (the forward-sweep output of s_hat) t = L s_hat + D ys.
 template <int NS, typename V>
 __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
-                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ s,
-                         cvec<V>* __restrict__ t, SysState* st, double* part, unsigned* counter) {
+                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys,
+                         const cvec<V>* __restrict__ s, cvec<V>* __restrict__ t, SysState* st,
+                         double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = st[sys].status == kRunning;
     const V w = V(omega[sys]);
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n || !run) continue;
         const size_t i = size_t(row) * NS + sys;
-        const cvec<V> y = apply_row(op, row, NS, sys, w, z);
+        const cvec<V> y = ys ? apply_row_eis(op, row, NS, sys, w, z, ys) : apply_row(op, row, NS, sys, w, z);
         t[i] = y;
         const double2 td = to_d<V>(y);
         acc[0] += cabs2(td);
@@ -345,9 +387,14 @@ __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
         acc[1] += ts.x;
         acc[2] += ts.y;
     }
+    block_partials<NS, 3>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_spmv_t_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[3];
-    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 3>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.status != kRunning) return;
     const double tt = out[0];
@@ -367,7 +414,7 @@ template <int NS, typename V>
 __global__ void k_bicg_xr(int n, cvec<V>* __restrict__ x, const cvec<V>* __restrict__ ph,
                           const cvec<V>* __restrict__ sh, const cvec<V>* __restrict__ s,
                           const cvec<V>* __restrict__ t, const cvec<V>* __restrict__ r0,
-                          cvec<V>* __restrict__ r, SysState* st, double* part, unsigned* counter) {
+                          cvec<V>* __restrict__ r, SysState* st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const int status = st[sys].status;
@@ -377,7 +424,8 @@ __global__ void k_bicg_xr(int n, cvec<V>* __restrict__ x, const cvec<V>* __restr
     const V wr = V(st[sys].omega.x), wi = V(st[sys].omega.y);
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (n + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= n || !(run || pend)) continue;
         const size_t i = size_t(row) * NS + sys;
@@ -404,9 +452,14 @@ __global__ void k_bicg_xr(int n, cvec<V>* __restrict__ x, const cvec<V>* __restr
         acc[1] += d.x;
         acc[2] += d.y;
     }
+    block_partials<NS, 3>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_bicg_xr_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[3];
-    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 3>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.status == kPendConvEarly) { S.status = kConvergedEarly; return; }
     if (S.status == kPendBreakTT) { S.status = kBreakdown; return; }
@@ -450,7 +503,8 @@ __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
     const int rpb = rows_per_block<NS>();
     const V w = V(omega[sys]);
     const int nrb = (op.n + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
         const size_t i = size_t(row) * NS + sys;
@@ -469,14 +523,14 @@ __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
 template <int NS>
 __global__ void k_true_res(DevOp<double> op, const double* __restrict__ omega,
                            const double2* __restrict__ x, const double2* __restrict__ b,
-                           double2* __restrict__ r, SysState* st, double* part,
-                           unsigned* counter) {
+                           double2* __restrict__ r, SysState* st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const double w = omega[sys];
     double acc[2] = {0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
         const size_t i = size_t(row) * NS + sys;
@@ -487,13 +541,60 @@ __global__ void k_true_res(DevOp<double> op, const double* __restrict__ omega,
         acc[0] += cabs2(d);
         acc[1] += cabs2(bi);
     }
+    block_partials<NS, 2>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_true_res_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[2];
-    if (!grid_reduce<NS, 2>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 2>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     double bn = sqrt(out[1]);
     if (bn < 1e-30) bn = 1.0;
     st[tid].true_bnorm = bn;
     st[tid].true_res = sqrt(out[0]) / bn;
+}
+
+// Least-squares scale of an initial guess: y = A x0, gamma = (y, b) / (y, y)
+// per system, so that ||b - gamma A x0|| is minimal.
+template <int NS>
+__global__ void k_guess_dots(DevOp<double> op, const double* __restrict__ omega,
+                             const double2* __restrict__ x, const double2* __restrict__ b, double* part) {
+    const int tid = threadIdx.x, sys = tid % NS;
+    const int rpb = rows_per_block<NS>();
+    const double w = omega[sys];
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int nrb = (op.n + rpb - 1) / rpb;
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
+        const int row = rb * rpb + tid / NS;
+        if (row >= op.n) continue;
+        const size_t i = size_t(row) * NS + sys;
+        const double2 y = apply_row(op, row, NS, sys, w, x);
+        const double2 yb = cconjmul(y, b[i]);
+        acc[0] += yb.x;
+        acc[1] += yb.y;
+        acc[2] += cabs2(y);
+    }
+    block_partials<NS, 3>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_guess_dots_fin(SysState* st, const double* __restrict__ part, int nb) {
+    double out[3];
+    if (!finish_partials<NS, 3>(part, nb, out)) return;
+    const int tid = threadIdx.x;
+    st[tid].guess_scale = out[2] > 0.0 ? cdiv(mk(out[0], out[1]), mk(out[2], 0.0)) : mk(0.0, 0.0);
+}
+
+// x *= guess_scale of its system.
+template <int NS>
+__global__ void k_scale_guess(int npad, const SysState* __restrict__ st, double2* __restrict__ x) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        x[t] = cmul(st[t % NS].guess_scale, x[t]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -514,7 +615,8 @@ __global__ void k_regroup(int rows, int na, int nb, const int* __restrict__ map,
             cvec<V> z;
             z.x = 0;
             z.y = 0;
-            narrow[t] = src >= 0 ? wide[size_t(row) * na + src] : z;
+            if (src >= 0) z = wide[size_t(row) * na + src];
+            narrow[t] = z;
         } else if (src >= 0) {
             wide[size_t(row) * na + src] = narrow[t];
         }
@@ -664,19 +766,19 @@ __global__ void k_dc_begin(SysState* st, unsigned mask) {
 // divJ = G^T (Me_sigma E) - phi0, zero on Dirichlet nodes, ||divJ||; PCG start
 // r = divJ, z = r / diag, p = z, phi = 0, rz = (r, z) (:636-677).
 template <int NS>
-__global__ void k_dc_divj(int nn, int dc_iter, double dc_tol, int cg_max,
-                          const int* __restrict__ nt_ptr, const int* __restrict__ nt_edge,
+__global__ void k_dc_divj(int nn, const int* __restrict__ nt_ptr, const int* __restrict__ nt_edge,
                           const double* __restrict__ nt_val, const double* __restrict__ ms,
                           const unsigned char* __restrict__ bc_node, const double* __restrict__ ldiag,
                           const double2* __restrict__ E, const double2* __restrict__ phi0,
                           double2* __restrict__ r, double2* __restrict__ z, double2* __restrict__ p,
-                          double2* __restrict__ phi, SysState* st, double* part, unsigned* counter) {
+                          double2* __restrict__ phi, SysState* st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = st[sys].dc_status == kRunning;
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (nn + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
         const size_t i = size_t(v) * NS + sys;
@@ -706,9 +808,14 @@ __global__ void k_dc_divj(int nn, int dc_iter, double dc_tol, int cg_max,
         acc[1] += rz.x;
         acc[2] += rz.y;
     }
+    block_partials<NS, 3>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_dc_divj_fin(int dc_iter, double dc_tol, int cg_max, SysState* st, const double* __restrict__ part, int nb) {
     double out[3];
-    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 3>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.dc_status != kRunning) return;
     const double dn = sqrt(out[0]);
@@ -730,13 +837,14 @@ __global__ void k_dc_divj(int nn, int dc_iter, double dc_tol, int cg_max,
 template <int NS>
 __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restrict__ lcol,
                         const double* __restrict__ lval, const double2* __restrict__ p,
-                        double2* __restrict__ ap, SysState* st, double* part, unsigned* counter) {
+                        double2* __restrict__ ap, SysState* st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = st[sys].pcg_status == kRunning;
     double acc[2] = {0.0, 0.0};
     const int nrb = (nn + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
         const size_t i = size_t(v) * NS + sys;
@@ -752,9 +860,14 @@ __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restr
         acc[0] += d.x;
         acc[1] += d.y;
     }
+    block_partials<NS, 2>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_dc_lp_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[2];
-    if (!grid_reduce<NS, 2>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 2>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.pcg_status != kRunning) return;
     const double2 pap = mk(out[0], out[1]);
@@ -771,14 +884,15 @@ template <int NS>
 __global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const double2* __restrict__ p,
                             const double2* __restrict__ ap, double2* __restrict__ phi,
                             double2* __restrict__ r, double2* __restrict__ z, SysState* st,
-                            double* part, unsigned* counter) {
+                            double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = st[sys].pcg_status == kRunning;
     const double2 al = st[sys].pcg_alpha;
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (nn + rpb - 1) / rpb;
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    const XcdRange R = xcd_range(nrb);
+    for (int rb = R.first; rb < R.end; rb += R.step) {
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
         const size_t i = size_t(v) * NS + sys;
@@ -800,9 +914,14 @@ __global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const doub
         acc[1] += rz.x;
         acc[2] += rz.y;
     }
+    block_partials<NS, 3>(acc, part);
+}
+
+template <int NS>
+__global__ __launch_bounds__(kFinBlock) void k_dc_update_fin(SysState* st, const double* __restrict__ part, int nb) {
     double out[3];
-    if (!grid_reduce<NS, 3>(acc, part, counter, out)) return;
-    if (tid >= NS) return;
+    if (!finish_partials<NS, 3>(part, nb, out)) return;
+    const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.pcg_status != kRunning) return;
     S.pcg_iter += 1;
@@ -863,13 +982,18 @@ inline int grid_for(size_t work_items, int cap) {
     const size_t b = (work_items + kBlock - 1) / kBlock;
     return int(b < size_t(cap) ? (b > 0 ? b : 1) : cap);
 }
+// Grid of a row kernel: one block per row block up to cap, rounded up to a
+// multiple of the XCD count (xcd_range needs it; surplus blocks find no rows).
 template <int NS>
 inline int row_grid(int rows, int cap = kMaxBlocks) {
     const int rpb = kBlock / NS;
-    const int b = (rows + rpb - 1) / rpb;
-    return b < cap ? (b > 0 ? b : 1) : cap;
+    int b = (rows + rpb - 1) / rpb;
+    b = b < cap ? b : cap;
+    b = (b + kXcds - 1) / kXcds * kXcds;
+    return b > 0 ? b : kXcds;
 }
 constexpr int kStreamCap = 4096;
+static_assert(kMaxBlocks % kXcds == 0 && kStreamCap % kXcds == 0, "grid caps must split over the XCDs");
 }  // namespace
 
 template <int NS, typename V>
@@ -887,9 +1011,11 @@ void Launch<NS, V>::unpack(hipStream_t s, int npad, int n, int nsys, const int* 
 template <int NS, typename V>
 void Launch<NS, V>::bicg_init(hipStream_t s, const DevOp<V>& op, const double* omega,
                               const cvec<V>* b, const cvec<V>* x, cvec<V>* r, cvec<V>* r0,
-                              SysState* st, double* part, unsigned* ctr, int x_zero) {
-    hipLaunchKernelGGL((k_bicg_init<NS, V>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op,
-                       omega, b, x, r, r0, st, part, ctr, x_zero);
+                              SysState* st, double* part, int x_zero) {
+    const int g = row_grid<NS>(op.n);
+    hipLaunchKernelGGL((k_bicg_init<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, b, x, r, r0, st,
+                       part, x_zero);
+    hipLaunchKernelGGL((k_bicg_init_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
 void Launch<NS, V>::sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
@@ -911,30 +1037,35 @@ void Launch<NS, V>::sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* o
 }
 template <int NS, typename V>
 void Launch<NS, V>::spmv_v(hipStream_t s, const DevOp<V>& op, const double* omega,
-                           const cvec<V>* z, const cvec<V>* r0, cvec<V>* v, SysState* st,
-                           double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_spmv_v<NS, V>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op, omega,
-                       z, r0, v, st, part, ctr);
+                           const cvec<V>* z, const cvec<V>* ys, const cvec<V>* r0, cvec<V>* v,
+                           SysState* st, double* part) {
+    const int g = row_grid<NS>(op.n);
+    hipLaunchKernelGGL((k_spmv_v<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, r0, v, st, part);
+    hipLaunchKernelGGL((k_spmv_v_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
 void Launch<NS, V>::bicg_s(hipStream_t s, int n, const cvec<V>* r, const cvec<V>* v, cvec<V>* sv,
-                           SysState* st, double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_bicg_s<NS, V>), dim3(row_grid<NS>(n)), dim3(kBlock), 0, s, n, r, v, sv,
-                       st, part, ctr);
+                           SysState* st, double* part) {
+    const int g = row_grid<NS>(n);
+    hipLaunchKernelGGL((k_bicg_s<NS, V>), dim3(g), dim3(kBlock), 0, s, n, r, v, sv, st, part);
+    hipLaunchKernelGGL((k_bicg_s_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
 void Launch<NS, V>::spmv_t(hipStream_t s, const DevOp<V>& op, const double* omega,
-                           const cvec<V>* z, const cvec<V>* sv, cvec<V>* t, SysState* st,
-                           double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_spmv_t<NS, V>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op, omega,
-                       z, sv, t, st, part, ctr);
+                           const cvec<V>* z, const cvec<V>* ys, const cvec<V>* sv, cvec<V>* t,
+                           SysState* st, double* part) {
+    const int g = row_grid<NS>(op.n);
+    hipLaunchKernelGGL((k_spmv_t<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, sv, t, st, part);
+    hipLaunchKernelGGL((k_spmv_t_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
 void Launch<NS, V>::bicg_xr(hipStream_t s, int n, cvec<V>* x, const cvec<V>* ph, const cvec<V>* sh,
                             const cvec<V>* sv, const cvec<V>* t, const cvec<V>* r0, cvec<V>* r,
-                            SysState* st, double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_bicg_xr<NS, V>), dim3(row_grid<NS>(n)), dim3(kBlock), 0, s, n, x, ph, sh,
-                       sv, t, r0, r, st, part, ctr);
+                            SysState* st, double* part) {
+    const int g = row_grid<NS>(n);
+    hipLaunchKernelGGL((k_bicg_xr<NS, V>), dim3(g), dim3(kBlock), 0, s, n, x, ph, sh, sv, t, r0, r, st,
+                       part);
+    hipLaunchKernelGGL((k_bicg_xr_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
 void Launch<NS, V>::spmv(hipStream_t s, const DevOp<V>& op, const double* omega, const cvec<V>* x,
@@ -957,9 +1088,19 @@ void Launch<NS, V>::interp_hanging(hipStream_t s, int npairs, const int* tri, cv
 template <int NS>
 void Launch64<NS>::true_res(hipStream_t s, const DevOp<double>& op, const double* omega,
                             const double2* x, const double2* b, double2* r, SysState* st,
-                            double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_true_res<NS>), dim3(row_grid<NS>(op.n)), dim3(kBlock), 0, s, op, omega, x,
-                       b, r, st, part, ctr);
+                            double* part) {
+    const int g = row_grid<NS>(op.n);
+    hipLaunchKernelGGL((k_true_res<NS>), dim3(g), dim3(kBlock), 0, s, op, omega, x, b, r, st, part);
+    hipLaunchKernelGGL((k_true_res_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
+}
+template <int NS>
+void Launch64<NS>::scale_guess(hipStream_t s, const DevOp<double>& op, const double* omega,
+                               const double2* b, double2* x, SysState* st, double* part) {
+    const int g = row_grid<NS>(op.n);
+    hipLaunchKernelGGL((k_guess_dots<NS>), dim3(g), dim3(kBlock), 0, s, op, omega, x, b, part);
+    hipLaunchKernelGGL((k_guess_dots_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
+    hipLaunchKernelGGL((k_scale_guess<NS>), dim3(grid_for(size_t(op.n) * NS, kStreamCap)), dim3(kBlock),
+                       0, s, op.n, st, x);
 }
 template <int NS>
 void Launch64<NS>::zero_systems(hipStream_t s, int npad, const int* flags, double2* x) {
@@ -1007,23 +1148,27 @@ void Launch64<NS>::dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max
                            const int* nt_edge, const double* nt_val, const double* ms,
                            const unsigned char* bc_node, const double* ldiag, const double2* E,
                            const double2* phi0, double2* r, double2* z, double2* p, double2* phi,
-                           SysState* st, double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_dc_divj<NS>), dim3(row_grid<NS>(nn)), dim3(kBlock), 0, s, nn, it, tol,
-                       cg_max, nt_ptr, nt_edge, nt_val, ms, bc_node, ldiag, E, phi0, r, z, p, phi,
-                       st, part, ctr);
+                           SysState* st, double* part) {
+    const int g = row_grid<NS>(nn);
+    hipLaunchKernelGGL((k_dc_divj<NS>), dim3(g), dim3(kBlock), 0, s, nn, nt_ptr, nt_edge, nt_val, ms,
+                       bc_node, ldiag, E, phi0, r, z, p, phi, st, part);
+    hipLaunchKernelGGL((k_dc_divj_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, it, tol, cg_max, st, part, g);
 }
 template <int NS>
 void Launch64<NS>::dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
-                         const double2* p, double2* ap, SysState* st, double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_dc_lp<NS>), dim3(row_grid<NS>(nn)), dim3(kBlock), 0, s, nn, lptr, lcol, lval,
-                       p, ap, st, part, ctr);
+                         const double2* p, double2* ap, SysState* st, double* part) {
+    const int g = row_grid<NS>(nn);
+    hipLaunchKernelGGL((k_dc_lp<NS>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st, part);
+    hipLaunchKernelGGL((k_dc_lp_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS>
 void Launch64<NS>::dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
                              const double2* ap, double2* phi, double2* r, double2* z, SysState* st,
-                             double* part, unsigned* ctr) {
-    hipLaunchKernelGGL((k_dc_update<NS>), dim3(row_grid<NS>(nn)), dim3(kBlock), 0, s, nn, ldiag, p,
-                       ap, phi, r, z, st, part, ctr);
+                             double* part) {
+    const int g = row_grid<NS>(nn);
+    hipLaunchKernelGGL((k_dc_update<NS>), dim3(g), dim3(kBlock), 0, s, nn, ldiag, p, ap, phi, r, z, st,
+                       part);
+    hipLaunchKernelGGL((k_dc_update_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS>
 void Launch64<NS>::dc_p(hipStream_t s, int nn, const double2* z, double2* p, const SysState* st) {
@@ -1046,6 +1191,10 @@ void launch_regroup(hipStream_t s, int rows, int na, int nb, const int* map, cve
 }
 template void launch_regroup<double>(hipStream_t, int, int, int, const int*, double2*, double2*, bool);
 template void launch_regroup<float>(hipStream_t, int, int, int, const int*, float2*, float2*, bool);
+
+hipError_t set_row_schedule(int mode) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_row_sched), &mode, sizeof(int));
+}
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol) {
     hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(64), 0, s, st, mask, maxiter, tol);

@@ -147,6 +147,24 @@ bool use_graphs() {
     return on;
 }
 
+// A p_hat and A s_hat by Eisenstat's identity (lower couplings + D y) unless
+// M3D_EISENSTAT=0 asks for the full product.
+bool use_eisenstat() {
+    static const bool on = [] {
+        const char* v = std::getenv("M3D_EISENSTAT");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+int row_schedule() {
+    static const int mode = [] {
+        const char* v = std::getenv("M3D_ROW_SCHED");
+        return (v && v[0] == '1') ? 1 : 0;
+    }();
+    return mode;
+}
+
 bool use_compaction() {
     static const bool on = [] {
         const char* v = std::getenv("M3D_COMPACT");
@@ -167,7 +185,6 @@ struct Work64 {
     std::vector<double> h_omega;  // host copy of omega
     DBuf<SysState> st;
     DBuf<double> part;
-    DBuf<unsigned> ctr;
     DBuf<int> flags;
     std::unique_ptr<Work<NS, double>> w64;
     std::unique_ptr<Work<NS, float>> w32;
@@ -248,9 +265,7 @@ Work64<NS>& ready_workspace(m3d_operator& op) {
     W.omega.alloc(NS); W.scale.alloc(NS);
     W.st.alloc(kMaxSys);
     W.part.alloc(size_t(3) * kMaxSys * kMaxBlocks);
-    W.ctr.alloc(16);
     W.flags.alloc(NS);
-    HIPCHK(hipMemsetAsync(W.ctr.p, 0, 16 * sizeof(unsigned), op.ctx->stream));
     HIPCHK(hipMemsetAsync(W.st.p, 0, kMaxSys * sizeof(SysState), op.ctx->stream));
     W.ready = true;
     return W;
@@ -315,15 +330,17 @@ void bicg_iteration(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x)
                      w.v.p, w.y.p, W.st.p);
     for (int c = K - 1; c >= 0; --c)
         L::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p, w.ph.p, W.st.p);
-    L::spmv_v(s, A, W.omega.p, w.ph.p, w.r0.p, w.v.p, W.st.p, W.part.p, W.ctr.p);
-    L::bicg_s(s, op.npad, w.r.p, w.v.p, w.s.p, W.st.p, W.part.p, W.ctr.p);
+    // v = A p_hat; in Eisenstat form from y, the forward-sweep output of p.
+    const bool eis = use_eisenstat();
+    L::spmv_v(s, A, W.omega.p, w.ph.p, eis ? w.y.p : nullptr, w.r0.p, w.v.p, W.st.p, W.part.p);
+    L::bicg_s(s, op.npad, w.r.p, w.v.p, w.s.p, W.st.p, W.part.p);
     for (int c = 0; c < K; ++c)
         L::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], false, w.s.p,
                      nullptr, nullptr, w.y.p, W.st.p);
     for (int c = K - 1; c >= 0; --c)
         L::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p, w.sh.p, W.st.p);
-    L::spmv_t(s, A, W.omega.p, w.sh.p, w.s.p, w.t.p, W.st.p, W.part.p, W.ctr.p);
-    L::bicg_xr(s, op.npad, x, w.ph.p, w.sh.p, w.s.p, w.t.p, w.r0.p, w.r.p, W.st.p, W.part.p, W.ctr.p);
+    L::spmv_t(s, A, W.omega.p, w.sh.p, eis ? w.y.p : nullptr, w.s.p, w.t.p, W.st.p, W.part.p);
+    L::bicg_xr(s, op.npad, x, w.ph.p, w.sh.p, w.s.p, w.t.p, w.r0.p, w.r.p, W.st.p, W.part.p);
 }
 
 struct BicgOutcome {
@@ -428,7 +445,7 @@ void bicg_loop(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x,
     while (launched < maxiter) {
         const int k = running();
         if (k == 0) break;
-        if (NS > 1 && use_compaction() && k <= NS / 4) {
+        if (NS > 1 && use_compaction() && k <= NS / 2) {
             std::vector<int> map;
             for (int i = 0; i < NS; ++i)
                 if (h[i].status == kRunning) map.push_back(i);
@@ -436,7 +453,8 @@ void bicg_loop(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x,
             if (k == 1) continue_narrow<1, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
             else if (k == 2) continue_narrow<2, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
             else if (k <= 4) continue_narrow<4, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
-            else continue_narrow<8, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
+            else if (k <= 8) continue_narrow<8, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
+            else continue_narrow<16, V>(op, NS, map, nullptr, w.r.p, w.r0.p, w.p.p, w.v.p, x, W.h_omega, h, left);
             HIPCHK(hipMemcpyAsync(W.st.p, h.data(), kMaxSys * sizeof(SysState), hipMemcpyHostToDevice, s));
             HIPCHK(hipStreamSynchronize(s));
             return;
@@ -462,7 +480,7 @@ BicgOutcome bicgstab(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, const cvec
     hipStream_t s = op.ctx->stream;
     launch_bicg_begin(s, W.st.p, mask, maxiter, tol);
     Launch<NS, V>::bicg_init(s, op.view<V>(), W.omega.p, b, x, w.r.p, w.r0.p, W.st.p, W.part.p,
-                             W.ctr.p, x_zero ? 1 : 0);
+                             x_zero ? 1 : 0);
     HIPCHK(hipGetLastError());
     std::vector<SysState> h = read_states(op, W.st);
     bicg_loop<NS, V>(op, W, w, x, h, maxiter);
@@ -489,7 +507,7 @@ void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, u
     for (int it = 0; it < prm.div_corr_iters; ++it) {
         L::dc_divj(s, op.nn, it, prm.div_corr_tol, cg_max, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p,
                    op.ms.p, op.dc_bc_node.p, op.l_diag.p, E, phi0, W.dr.p, W.dz.p, W.dp.p, W.dphi.p,
-                   W.st.p, W.part.p, W.ctr.p);
+                   W.st.p, W.part.p);
         HIPCHK(hipGetLastError());
         std::vector<SysState> h = read_states(op, W.st);
         bool any_dc = false, any_pcg = false;
@@ -501,9 +519,9 @@ void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, u
         int chunk = 8;
         auto pcg_iter = [&] {
             L::dc_lp(s, op.nn, op.l_ptr.p, op.l_colidx.p, op.l_vals.p, W.dp.p, W.dap.p, W.st.p,
-                     W.part.p, W.ctr.p);
+                     W.part.p);
             L::dc_update(s, op.nn, op.l_diag.p, W.dp.p, W.dap.p, W.dphi.p, W.dr.p, W.dz.p, W.st.p,
-                         W.part.p, W.ctr.p);
+                         W.part.p);
             L::dc_p(s, op.nn, W.dz.p, W.dp.p, W.st.p);
         };
         while (any_pcg) {
@@ -530,8 +548,7 @@ void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, u
 template <int NS>
 std::vector<double> true_residual(m3d_operator& op, Work64<NS>& W, const double2* x,
                                   const double2* b, double2* r) {
-    Launch64<NS>::true_res(op.ctx->stream, op.view<double>(), W.omega.p, x, b, r, W.st.p, W.part.p,
-                           W.ctr.p);
+    Launch64<NS>::true_res(op.ctx->stream, op.view<double>(), W.omega.p, x, b, r, W.st.p, W.part.p);
     HIPCHK(hipGetLastError());
     std::vector<SysState> h = read_states(op, W.st);
     std::vector<double> out(NS);
@@ -663,14 +680,21 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
         bool x_zero = true;
         std::vector<int> used(NS, 0);
         if (prm.use_initial_guess) {
-            // Caller's E_sec guess, kept only where ||b - A e0|| < ||b||;
-            // elsewhere the reference's zero start.
+            // Caller's E_sec guess (mode 2: scaled by the least-squares factor
+            // first), kept only where it lowers the residual below ||b||
+            // (0.999 ||b|| when scaled); elsewhere the reference's zero start.
             L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
             L64::fix_constrained(s, op.npad, op.constrained.p, W.tmp.p, W.x.p);
+            const bool scaled = prm.use_initial_guess == 2;
+            if (scaled) {
+                L64::scale_guess(s, op.view<double>(), W.omega.p, W.tmp.p, W.x.p, W.st.p, W.part.p);
+                L64::fix_constrained(s, op.npad, op.constrained.p, W.tmp.p, W.x.p);
+            }
             std::vector<double> tr = true_residual<NS>(op, W, W.x.p, W.tmp.p, nullptr);
             std::vector<int> zero(NS, 0);
+            const double keep = scaled ? 0.999 : 1.0;
             for (int i = 0; i < NS; ++i) {
-                used[i] = (i < nsys && tr[i] < 1.0) ? 1 : 0;
+                used[i] = (i < nsys && tr[i] < keep) ? 1 : 0;
                 zero[i] = used[i] ? 0 : 1;
             }
             HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int), hipMemcpyHostToDevice, s));
@@ -803,6 +827,10 @@ void check_params(const m3d_solve_params* p) {
     if (!(p->tol > 0) || p->maxiter <= 0) throw Fail(M3D_ERR_ARG, "tol and maxiter must be positive");
     if (p->div_correction && (p->div_corr_iters < 0 || p->divcorr_outer_max < 0 || p->divcorr_iter_per_dc <= 0))
         throw Fail(M3D_ERR_ARG, "invalid DivCorr parameters");
+    if (p->precision != M3D_PREC_FP64 && p->precision != M3D_PREC_MIXED)
+        throw Fail(M3D_ERR_ARG, "precision must be M3D_PREC_FP64 or M3D_PREC_MIXED");
+    if (p->use_initial_guess < 0 || p->use_initial_guess > 2)
+        throw Fail(M3D_ERR_ARG, "use_initial_guess must be 0, 1 or 2");
 }
 
 template <typename F>
@@ -1072,6 +1100,7 @@ int m3d_context_create(int device, m3d_context** out) {
         HIPCHK(hipGetDeviceCount(&count));
         if (device < 0 || device >= count) throw Fail(M3D_ERR_ARG, "device index out of range");
         HIPCHK(hipSetDevice(device));
+        HIPCHK(set_row_schedule(row_schedule()));
         auto* c = new m3d_context;
         c->device = device;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1281,13 +1310,39 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     W.h_omega = om;
     HIPCHK(hipMemcpyAsync(W.omega.p, om.data(), NS * sizeof(double), hipMemcpyHostToDevice, s));
     const unsigned mask = (nsys >= 32) ? 0xFFFFFFFFu : ((1u << nsys) - 1u);
-    launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
     const size_t nv = size_t(op.npad) * NS;
-    Launch<NS, double>::fill_zero(s, nv, W.x.p);
     const DevOp<double> A = op.view<double>();
+    if (kind >= 2) {
+        // BiCGStab iterations on b = 1, x = 0 with tol 0, so no system stops.
+        std::vector<double2> ones(nv, make_double2(1.0, 0.0));
+        HIPCHK(hipMemcpyAsync(W.tmp.p, ones.data(), nv * sizeof(double2), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        launch_bicg_begin(s, W.st.p, mask, 1 << 30, 0.0);
+        if (kind == 2) {
+            Launch<NS, double>::fill_zero(s, nv, W.x.p);
+            Launch<NS, double>::bicg_init(s, A, W.omega.p, W.tmp.p, W.x.p, w.r.p, w.r0.p, W.st.p, W.part.p,
+                                          1);
+        } else {
+            auto& w32 = vec32<NS>(op, W);
+            std::vector<float2> ones32(nv, make_float2(1.f, 0.f));
+            HIPCHK(hipMemcpyAsync(w32.b.p, ones32.data(), nv * sizeof(float2), hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));
+            Launch<NS, float>::fill_zero(s, nv, w32.x.p);
+            Launch<NS, float>::bicg_init(s, op.view<float>(), W.omega.p, w32.b.p, w32.x.p, w32.r.p, w32.r0.p,
+                                         W.st.p, W.part.p, 1);
+        }
+    } else {
+        launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
+    }
+    Launch<NS, double>::fill_zero(s, nv, W.x.p);
     auto one = [&]() {
         if (kind == 0) {
             Launch<NS, double>::spmv(s, A, W.omega.p, W.x.p, nullptr, W.tmp.p);
+        } else if (kind == 2) {
+            bicg_iteration<NS, double>(op, W, w, W.x.p);
+        } else if (kind == 3) {
+            auto& w32 = vec32<NS>(op, W);
+            bicg_iteration<NS, float>(op, W, w32, w32.x.p);
         } else {
             for (int c = 0; c < op.ncolours; ++c)
                 Launch<NS, double>::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
@@ -1315,8 +1370,16 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     long long nfree = 0;
     for (int e = 0; e < op.n; ++e) nfree += op.h_constrained[e] ? 0 : 1;
     const double m = double(op.nnz), n = double(nfree), k = double(nsys);
-    *bytes = kind == 0 ? m * 12.0 + n * 16.0 + 2.0 * k * n * 16.0
-                       : m * 12.0 + n * 32.0 + 4.0 * k * n * 16.0;
+    // Algorithmic bytes: values + columns of the m off-diagonal nonzeros, the
+    // (S_ii, Me_i) diagonal pair, and each vector touched once.
+    auto spmv_b = [&](double wv, double wc) { return m * (wv + 4.0) + 2.0 * n * wv + 2.0 * k * n * wc; };
+    auto ssor_b = [&](double wv, double wc) { return m * (wv + 4.0) + 4.0 * n * wv + 4.0 * k * n * wc; };
+    // One right-preconditioned BiCGStab iteration: 2 SpMV, 2 SSOR and 17
+    // vector passes of the AXPY / dot work.
+    auto iter_b = [&](double wv, double wc) {
+        return 2.0 * spmv_b(wv, wc) + 2.0 * ssor_b(wv, wc) + 17.0 * k * n * wc;
+    };
+    *bytes = kind == 0 ? spmv_b(8, 16) : kind == 1 ? ssor_b(8, 16) : kind == 2 ? iter_b(8, 16) : iter_b(4, 8);
 }
 
 template <typename F>
@@ -1381,7 +1444,7 @@ int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, 
                      double* avg_ms, double* alg_bytes) {
     return guarded([&] {
         check_ready(op);
-        if (!omega || !avg_ms || !alg_bytes || reps <= 0 || (kind != 0 && kind != 1))
+        if (!omega || !avg_ms || !alg_bytes || reps <= 0 || kind < 0 || kind > 3)
             throw Fail(M3D_ERR_ARG, "invalid benchmark arguments");
         HIPCHK(hipSetDevice(op->ctx->device));
         for_width(nsys, [&](int k) {

@@ -136,6 +136,15 @@ def lib() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise NativeError(f"{LIB_PATH} is missing: run maple3dmt_amd.build.build() first")
+    # The ROCm wheel of torch ships its own HIP and HSA runtimes and asks for
+    # them by unversioned names.  Loaded after this library, it would bring a
+    # second HSA runtime into the process, which then finds no GPU.  Loaded
+    # first, its libamdhip64.so.7 also satisfies this library, so the process
+    # has one runtime and device pointers pass freely between the two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i, d, p = C.c_void_p, C.c_int, C.c_double, C.POINTER
     L.m3d_last_error.restype = C.c_char_p
@@ -251,7 +260,7 @@ class SolveParams:
     divcorr_iter_per_dc: int = 500
     precision: int = PREC_FP64
     inner_tol: float = 0.0
-    use_initial_guess: bool = False
+    use_initial_guess: int = 0   # 0 zero start, 1 guess as given, 2 least-squares scaled guess
 
     def c(self) -> SolveParamsC:
         return SolveParamsC(self.tol, self.maxiter, int(self.div_correction), self.div_corr_iters,
@@ -317,7 +326,8 @@ class DeviceOperator:
 
     def bench_kernel(self, kind: int, omegas, reps: int = 20):
         """(avg ms per launch, algorithmic bytes per launch) of kind 0 = SpMV,
-        1 = SSOR apply, timed with HIP events on the library stream."""
+        1 = SSOR apply, 2 = one FP64 BiCGStab iteration, 3 = one complex64
+        BiCGStab iteration, timed with HIP events on the library stream."""
         om = np.ascontiguousarray(omegas, dtype=np.float64)
         ms, by = C.c_double(), C.c_double()
         _check_dev(lib().m3d_bench_kernel(self.handle, kind, om.shape[0], _ptr(om), reps,

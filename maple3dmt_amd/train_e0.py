@@ -47,7 +47,7 @@ def evaluate(ctx, net, box, seeds, max_level, repeats=1):
         d_x = torch.zeros_like(d_rhs)
         prm0 = wl.fwd.solve_params()
         prm1 = wl.fwd.solve_params()
-        prm1.use_initial_guess = True
+        prm1.use_initial_guess = 2
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(repeats):
