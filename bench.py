@@ -7,10 +7,16 @@ relative residual 1e-7 with the reference's forward path (scattered field,
 SSOR-BiCGStab from zero, dead-edge interpolation, post-hoc DivCorr) as one
 batched device call whose inputs (rhs_total, E_pri) already sit in HBM.
 
-value = solved systems per second over the K timed steps.  The JSON line also
-carries the roofline of the batched SpMV kernel (HIP events on the library's
-stream), the mixed-precision variant, and the CPU oracle timed on a bounded
-sample of the same workload.
+value = solved systems per second of the plain FP64 method over the K timed
+steps.  The JSON line also carries:
+  mixed          the same batch with complex64 inner solves in FP64 refinement
+  mixed_learned  mixed, started from the trained network's e0 (inference and
+                 prolongation inside the timed region), when its weights exist
+  roofline       the kernel that dominates the timed region (the backward SSOR
+                 sweep at the tail batch width), timed with HIP events on the
+                 library's stream, against HBM peak
+  spmv           the batched SpMV of all 16 systems (the metric's SpMV TB/s)
+  cpu_baseline   the CPU oracle timed on a bounded sample of the same workload
 """
 from __future__ import annotations
 
@@ -26,12 +32,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+WEIGHTS = os.path.join(ROOT, "maple3dmt_amd", "data", "e0_net.npz")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--half-width", type=float, default=100e3)
@@ -42,17 +49,25 @@ def parse():
                          "FP64 method need 12-25 thousand)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--tail-width", type=int, default=2,
+                    help="batch width of the roofline kernel (the width the low-frequency tail runs at)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
     ap.add_argument("--kernel-only", action="store_true",
-                    help="profiling mode: build the workload, then only run the batched SpMV "
-                         "and SSOR kernels --reps times and a 1 GiB device copy (counter "
-                         "calibration)")
+                    help="profiling mode: build the workload, then only run the benchmark "
+                         "kernels (SpMV, SSOR, BiCGStab iterations, backward sweep) and four "
+                         "1 GiB device copies (counter calibration)")
     ap.add_argument("--reps", type=int, default=20)
     return ap.parse_args()
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def kernel_only(args):
-    """Launch only the roofline kernels (for rocprofv3 --pmc runs)."""
+    """Launch only the benchmark kernels (for rocprofv3 --pmc runs)."""
     import torch
 
     from maple3dmt_amd import Context, configs
@@ -63,6 +78,7 @@ def kernel_only(args):
     omegas = np.repeat(2 * np.pi * wl.freqs, 2)
     ms, by = model.device.bench_kernel(0, omegas, reps=args.reps)
     ms2, by2 = model.device.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
+    ms4, by4 = model.device.bench_kernel(4, omegas[:args.tail_width], reps=args.reps)
     iters = {}
     for kind, tag in ((2, "fp64"), (3, "c64")):
         for k in (1, 2, 4, 16):
@@ -75,13 +91,18 @@ def kernel_only(args):
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"spmv_ms": ms, "spmv_alg_bytes": by, "ssor_ms": ms2, "ssor_alg_bytes": by2,
-                      "iteration": iters,
+                      "sweep_bwd_ms": ms4, "sweep_bwd_alg_bytes": by4, "iteration": iters,
                       "copy_bytes_each_way": x.numel() * 8, "copies": 4, "edges": model.n}),
           flush=True)
 
 
-def log(msg):
-    print(msg, file=sys.stderr, flush=True)
+def impedances(model, omegas, x, nf):
+    return np.stack([model.extract_impedance(omegas[2 * f], x[2 * f], x[2 * f + 1]) for f in range(nf)])
+
+
+def rel_dz(z, zref):
+    off = (slice(None), slice(None), slice(1, 3))
+    return float(np.max(np.abs(z[off] - zref[off]) / np.abs(zref[off])))
 
 
 def main():
@@ -125,60 +146,87 @@ def main():
     d_x = torch.zeros_like(d_rhs)
     torch.cuda.synchronize()
 
+    def solve(prm, x):
+        return op.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(), x.data_ptr(),
+                               ns, prm)
+
+    def timed(prm, make_x, label):
+        """Warm-up, then K timed steps; make_x() runs inside the timed region."""
+        for _ in range(args.warmup):
+            res = solve(prm, make_x())
+            log(f"[bench] {label} warmup: iterations {[r['iterations'] for r in res]}")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            x = make_x()
+            res = solve(prm, x)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        return res, x, dt
+
+    # Plain FP64 (the reference's method, batched).
     prm64 = wl.fwd.solve_params()
 
-    def step(prm):
-        return op.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(),
-                               d_x.data_ptr(), ns, prm)
+    def zero_x():
+        d_x.zero_()
+        torch.cuda.synchronize()  # the library runs on its own stream
+        return d_x
 
-    for _ in range(args.warmup):
-        res = step(prm64)
-        log(f"[bench] warmup: iterations {[r['iterations'] for r in res]}")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        res = step(prm64)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t1
+    res, x64, dt = timed(prm64, zero_x, "fp64")
     ms_per_step = 1e3 * dt / args.steps
     value = ns * args.steps / dt
     iters = [r["iterations"] for r in res]
     conv = all(r["converged"] for r in res)
     log(f"[bench] fp64: {ms_per_step:.1f} ms/step, {value:.2f} solves/s, converged={conv}, "
-        f"true residual max {max(r['true_residual'] for r in res):.2e}")
-    x64 = d_x.cpu().numpy()
-    z64 = np.stack([model.extract_impedance(omegas[2 * f], x64[2 * f], x64[2 * f + 1])
-                    for f in range(nf)])
+        f"iterations {iters}, true residual max {max(r['true_residual'] for r in res):.2e}")
+    z64 = impedances(model, omegas, x64.cpu().numpy(), nf)
 
     # Mixed precision: complex64 inner BiCGStab inside FP64 refinement.
     mixed = None
     if not args.no_mixed:
         prm32 = wl.fwd.solve_params(precision=PREC_MIXED, inner_tol=1e-3)
-        step(prm32)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        for _ in range(args.steps):
-            r32 = step(prm32)
-        torch.cuda.synchronize()
-        d32 = time.perf_counter() - t2
-        x32 = d_x.cpu().numpy()
-        z32 = np.stack([model.extract_impedance(omegas[2 * f], x32[2 * f], x32[2 * f + 1])
-                        for f in range(nf)])
-        off = (slice(None), slice(None), slice(1, 3))
-        dz = float(np.max(np.abs(z32[off] - z64[off]) / np.abs(z64[off])))
+        r32, x32, d32 = timed(prm32, zero_x, "mixed")
         mixed = {"solves_per_s": ns * args.steps / d32, "ms_per_step": 1e3 * d32 / args.steps,
+                 "speedup_vs_fp64": (ns * args.steps / d32) / value,
                  "converged": all(r["converged"] for r in r32),
-                 "max_rel_dZ_vs_fp64": dz,
+                 "max_rel_dZ_vs_fp64": rel_dz(impedances(model, omegas, x32.cpu().numpy(), nf), z64),
                  "iterations": [r["iterations"] for r in r32]}
         log(f"[bench] mixed: {mixed}")
 
-    # Roofline of the batched SpMV (the operator apply of every iteration).
-    ms_spmv, bytes_spmv = op.bench_kernel(0, omegas, reps=20)
-    ms_ssor, bytes_ssor = op.bench_kernel(1, omegas, reps=5)
-    ach = bytes_spmv / (ms_spmv * 1e-3) / 1e9
+    # Mixed precision started from the network's e0.
+    learned = None
+    if not args.no_mixed and not args.no_learned and os.path.exists(args.weights):
+        from maple3dmt_amd.learned import Box, CoarseNet, initial_guess
+        wts = np.load(args.weights)
+        box = Box(*wts["box"].tolist())
+        net = CoarseNet(ctx, max_batch=nf, dilations=tuple(int(v) for v in wts["dilations"]))
+        net.set_params(wts["w"], wts["b"])
+        prml = wl.fwd.solve_params(precision=PREC_MIXED, inner_tol=1e-3)
+        prml.use_initial_guess = 2
+        rl, xl, dl = timed(prml, lambda: initial_guess(net, model, wl, box), "mixed+learned")
+        learned = {"solves_per_s": ns * args.steps / dl, "ms_per_step": 1e3 * dl / args.steps,
+                   "speedup_vs_fp64": (ns * args.steps / dl) / value,
+                   "converged": all(r["converged"] for r in rl),
+                   "max_rel_dZ_vs_fp64": rel_dz(impedances(model, omegas, xl.cpu().numpy(), nf), z64),
+                   "guess_used": [int(r["guess_used"]) for r in rl],
+                   "guess_residual": [round(r["guess_residual"], 4) for r in rl],
+                   "iterations": [r["iterations"] for r in rl],
+                   "weights": os.path.relpath(args.weights, ROOT)}
+        log(f"[bench] mixed+learned: {learned}")
+        net.close()
+
+    # Roofline of the kernel that dominates the timed region: the backward
+    # SSOR sweep at the tail width (the low-frequency systems run their
+    # thousands of iterations there); and the 16-system batched SpMV.
+    tw = max(1, min(args.tail_width, ns))
+    ms_sw, bytes_sw = op.bench_kernel(4, omegas[:tw], reps=args.reps)
+    ach = bytes_sw / (ms_sw * 1e-3) / 1e9
+    ms_spmv, bytes_spmv = op.bench_kernel(0, omegas, reps=args.reps)
+    ach_spmv = bytes_spmv / (ms_spmv * 1e-3) / 1e9
+    ms_ssor, bytes_ssor = op.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
     info = op.info()
-    log(f"[bench] spmv {ms_spmv:.3f} ms, {ach:.0f} GB/s; ssor {ms_ssor:.3f} ms, "
-        f"{bytes_ssor / (ms_ssor * 1e-3) / 1e9:.0f} GB/s; {info}")
+    log(f"[bench] sweep_bwd x{tw} {ms_sw * 1e3:.1f} us/launch, {ach:.0f} GB/s; spmv x{ns} {ms_spmv:.3f} ms, "
+        f"{ach_spmv:.0f} GB/s; ssor x{ns} {ms_ssor:.3f} ms; {info}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -199,17 +247,20 @@ def main():
         "data": "synthetic (seeded random 3-D model)",
         "config": {"workload": wl.name + "_11x11st_8f_2pol_fwd", "systems_per_step": ns,
                    "edges": n, "cells": model.nc, "nodes": model.nn, "tol": wl.fwd.bicgstab_tol,
-                   "precision": "fp64", "batched": True},
+                   "maxiter": args.maxiter, "precision": "fp64", "batched": True},
         "converged": conv,
         "iterations": iters,
-        "roofline": {"bound": "hbm", "kernel": "batched SpMV (k_spmv, 16 systems)",
-                     "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": bytes_spmv, "avg_launch_ms": ms_spmv},
-        "ssor_apply": {"avg_launch_ms": ms_ssor, "alg_bytes": bytes_ssor,
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_sweep_bwd<{tw},double> (backward SSOR sweep, one colour per launch)",
+                     "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_launch": bytes_sw, "avg_launch_ms": ms_sw},
+        "spmv": {"kernel": f"k_spmv<{ns},double>", "avg_launch_ms": ms_spmv, "alg_bytes": bytes_spmv,
+                 "achieved_GBs": ach_spmv, "TBs": ach_spmv / 1e3},
+        "ssor_apply": {"systems": ns, "avg_ms": ms_ssor, "alg_bytes": bytes_ssor,
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
-        "spmv_TBs": ach / 1e3,
+        "operator": info,
         "mixed": mixed,
+        "mixed_learned": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -219,21 +270,18 @@ def main():
 
 def cpu_baseline(model, wl, budget, gpu_iters):
     """The CPU oracle (natural-order SSOR-BiCGStab, complex128, one core) on a
-    bounded sample: both polarisations of the highest frequency solved to
-    completion if that fits the budget, otherwise the measured time per
-    iteration scaled by the GPU's iteration counts."""
-    from oracle.oracle import Oracle
+    bounded sample: the measured time per iteration of the 100 Hz system,
+    scaled by the GPU's mean iteration count per system."""
+    from oracle.oracle import Oracle, bicgstab_csr
     t = time.perf_counter()
     o = Oracle(model, sigma_bg=wl.sigma_bg)
     o.set_sigma(model.array("sigma"))
     setup = time.perf_counter() - t
     f = float(wl.freqs[-1])
     o.assemble(2 * np.pi * f)
-    # Time per BiCGStab iteration from a capped solve.
     ptr, col, val = o.csr("A")
     rhs, epri = o.rhs(0)
     b = rhs - o.spmv(epri)
-    from oracle.oracle import bicgstab_csr
     t = time.perf_counter()
     _, r = bicgstab_csr(ptr, col, val, b, tol=1e-7, maxiter=20)
     per_iter = (time.perf_counter() - t) / max(r["iterations"], 1)

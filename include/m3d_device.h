@@ -117,7 +117,7 @@ typedef struct {
                                 1: used as given where ||b - A e0|| < ||b||;
                                 2: scaled per system by gamma = (A e0, b) / (A e0, A e0), the
                                 least-squares factor, and used where ||b - gamma A e0|| <
-                                0.999 ||b||.  Elsewhere the zero start. */
+                                0.5 ||b||.  Elsewhere the zero start. */
 } m3d_solve_params;
 
 /* Per-system outcome; mirrors BiCGStabResult (bicgstab.h:23-28) plus what the
@@ -131,6 +131,8 @@ typedef struct {
     int info;             /* 0 converged, 1 early (||s||), 2 breakdown, 3 max iterations */
     int outer;            /* outer iterations (interleaved bursts / refinement steps) */
     int guess_used;       /* FORWARD with use_initial_guess: 1 if the guess was kept */
+    double guess_residual; /* FORWARD with use_initial_guess: ||b - A x0|| / ||b|| of the
+                              (scaled) guess, whether kept or not; else 0 */
 } m3d_solve_result;
 
 /* Solve nsys systems that share this operator, one omega each.
@@ -175,7 +177,8 @@ int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long*
  * context stream, and its algorithmic HBM bytes per launch.
  * kind 0: batched SpMV y = A x; kind 1: one multicolour SSOR apply;
  * kind 2: one FP64 BiCGStab iteration (b = 1, tolerance 0, so every system
- * keeps running); kind 3: the same iteration in complex64. */
+ * keeps running); kind 3: the same iteration in complex64; kind 4: one
+ * backward-sweep launch (the mean over the colours of a backward sweep). */
 int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
                      double* avg_ms, double* alg_bytes);
 

@@ -34,6 +34,9 @@ int m3d_net_set_params(m3d_net* net, const float* w, const float* b);
 int m3d_net_forward(m3d_net* net, int batch, const float* d_in, float* d_out);
 /* Mean-squared-error loss and parameter gradients, without an update. */
 int m3d_net_gradients(m3d_net* net, int batch, const float* d_in, const float* d_target, float* loss);
+/* Global gradient-norm clipping for m3d_net_train_step: gradients are scaled
+ * by min(1, max_norm / ||g||) before the Adam update (0 disables; default). */
+int m3d_net_set_clip(m3d_net* net, float max_norm);
 /* One Adam step (beta 0.9 / 0.999) at learning rate lr; returns the loss. */
 int m3d_net_train_step(m3d_net* net, int batch, const float* d_in, const float* d_target, float lr,
                        float* loss);

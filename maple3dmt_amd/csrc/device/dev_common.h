@@ -186,16 +186,21 @@ __device__ inline bool finish_partials(const double* __restrict__ part, int nb, 
     }
     fin[tid] = acc;
     __syncthreads();
-    if (tid >= NS) return false;
-    // Kept rolled: unrolling G LDS loads per pair spills registers.
+    // Pairwise tree over the G groups of each pair (fin[g * P + p]): group g
+    // takes group g + span, for span = G2/2, G2/4, ..., 1 (G2 = G rounded up
+    // to a power of two).
+    constexpr int G2 = G <= 1 ? 1 : 1 << (32 - __builtin_clz(G - 1));
 #pragma unroll 1
-    for (int q = 0; q < NQ; ++q) {
-        const int p = q * NS + tid;
-        double s = 0.0;
-#pragma unroll 1
-        for (int g = 0; g < G; ++g) s += fin[g * P + p];
-        out[q] = s;
+    for (int span = G2 / 2; span > 0; span >>= 1) {
+        for (int idx = tid; idx < span * P; idx += kFinBlock) {
+            const int g = idx / P;
+            if (g + span < G) fin[idx] += fin[idx + span * P];
+        }
+        __syncthreads();
     }
+    if (tid >= NS) return false;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = fin[q * NS + tid];
     return true;
 }
 

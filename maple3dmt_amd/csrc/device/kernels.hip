@@ -31,6 +31,35 @@ __device__ inline void row_gather(const V* __restrict__ val, const int* __restri
     }
 }
 
+// Sweeps of narrow batches split each row over SUB lanes (entries k = sub,
+// sub + SUB, ...; lanes of one row are NS apart), so that one colour of ~1e5
+// rows still puts enough wavefronts on the GPU to hide gather latency.
+template <int NS>
+__host__ __device__ constexpr int sweep_sub() { return NS >= 4 ? 1 : 4 / NS; }
+template <int NS>
+__host__ __device__ constexpr int sweep_rpb() { return kBlock / (NS * sweep_sub<NS>()); }
+
+template <int SUB, typename V>
+__device__ inline void row_gather_split(const V* __restrict__ val, const int* __restrict__ col, int off,
+                                        int w, int lane, const cvec<V>* __restrict__ x, int NS, int sys,
+                                        int sub, V& ar, V& ai) {
+    for (int k = sub; k < w; k += SUB) {
+        const int idx = off + k * kSlice + lane;
+        const V a = val[idx];
+        const cvec<V> xv = x[size_t(col[idx]) * NS + sys];
+        ar += a * xv.x;
+        ai += a * xv.y;
+    }
+    if (SUB >= 2) {
+        ar += __shfl_xor(ar, NS);
+        ai += __shfl_xor(ai, NS);
+    }
+    if (SUB >= 4) {
+        ar += __shfl_xor(ar, 2 * NS);
+        ai += __shfl_xor(ai, 2 * NS);
+    }
+}
+
 // (A x)_row for one system: lower + upper couplings plus the diagonal
 // (diag_s + i*omega*me) * x_row.
 template <typename V>
@@ -205,8 +234,9 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
                             const cvec<V>* __restrict__ in, cvec<V>* __restrict__ p,
                             const cvec<V>* __restrict__ v, cvec<V>* __restrict__ y,
                             const SysState* __restrict__ st) {
-    const int tid = threadIdx.x, sys = tid % NS;
-    const int rpb = rows_per_block<NS>();
+    constexpr int SUB = sweep_sub<NS>();
+    const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
+    constexpr int rpb = sweep_rpb<NS>();
     const SysState& S = st[sys];
     if (S.status != kRunning) return;
     const V w = V(omega[sys]);
@@ -216,7 +246,7 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
     const V br = V(S.beta.x), bi = V(S.beta.y), wr = V(S.omega.x), wi = V(S.omega.y);
     const XcdRange R = xcd_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
-        const int row = r0 + rb * rpb + tid / NS;
+        const int row = r0 + rb * rpb + tid / (NS * SUB);
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
         cvec<V> src = in[i];
@@ -229,18 +259,18 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
                 src.x = src.x + (br * tr - bi * ti);
                 src.y = src.y + (br * ti + bi * tr);
             }
-            p[i] = src;
+            if (sub == 0) p[i] = src;
         }
         const int sl = row >> 6, lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, y, NS, sys, ar, ai);
+        row_gather_split<SUB>(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, y, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(op.diag_s[row], w * op.me[row], dr, di);
         const V er = src.x - ar, ei = src.y - ai;
         cvec<V> out;
         out.x = er * dr - ei * di;
         out.y = er * di + ei * dr;
-        y[i] = out;
+        if (sub == 0) y[i] = out;
     }
 }
 
@@ -250,27 +280,28 @@ template <int NS, typename V>
 __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
                             const cvec<V>* __restrict__ y, cvec<V>* __restrict__ z,
                             const SysState* __restrict__ st) {
-    const int tid = threadIdx.x, sys = tid % NS;
-    const int rpb = rows_per_block<NS>();
+    constexpr int SUB = sweep_sub<NS>();
+    const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
+    constexpr int rpb = sweep_rpb<NS>();
     if (st[sys].status != kRunning) return;
     const V w = V(omega[sys]);
     const int nrows = r1 - r0;
     const int nrb = (nrows + rpb - 1) / rpb;
     const XcdRange R = xcd_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
-        const int row = r0 + rb * rpb + tid / NS;
+        const int row = r0 + rb * rpb + tid / (NS * SUB);
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
         const int sl = row >> 6, lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather(op.u_val, op.u_col, op.u_off[sl], op.u_w[sl], lane, z, NS, sys, ar, ai);
+        row_gather_split<SUB>(op.u_val, op.u_col, op.u_off[sl], op.u_w[sl], lane, z, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(op.diag_s[row], w * op.me[row], dr, di);
         const cvec<V> yi = y[i];
         cvec<V> out;
         out.x = yi.x - (ar * dr - ai * di);
         out.y = yi.y - (ar * di + ai * dr);
-        z[i] = out;
+        if (sub == 0) z[i] = out;
     }
 }
 
@@ -984,13 +1015,15 @@ inline int grid_for(size_t work_items, int cap) {
 }
 // Grid of a row kernel: one block per row block up to cap, rounded up to a
 // multiple of the XCD count (xcd_range needs it; surplus blocks find no rows).
-template <int NS>
-inline int row_grid(int rows, int cap = kMaxBlocks) {
-    const int rpb = kBlock / NS;
+inline int grid_rows(int rows, int rpb, int cap) {
     int b = (rows + rpb - 1) / rpb;
     b = b < cap ? b : cap;
     b = (b + kXcds - 1) / kXcds * kXcds;
     return b > 0 ? b : kXcds;
+}
+template <int NS>
+inline int row_grid(int rows, int cap = kMaxBlocks) {
+    return grid_rows(rows, kBlock / NS, cap);
 }
 constexpr int kStreamCap = 4096;
 static_assert(kMaxBlocks % kXcds == 0 && kStreamCap % kXcds == 0, "grid caps must split over the XCDs");
@@ -1021,7 +1054,7 @@ template <int NS, typename V>
 void Launch<NS, V>::sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
                               int r1, bool pupd, const cvec<V>* in, cvec<V>* p, const cvec<V>* v,
                               cvec<V>* y, const SysState* st) {
-    const dim3 g(row_grid<NS>(r1 - r0, kStreamCap));
+    const dim3 g(grid_rows(r1 - r0, sweep_rpb<NS>(), kStreamCap));
     if (pupd)
         hipLaunchKernelGGL((k_sweep_fwd<NS, V, true>), g, dim3(kBlock), 0, s, op, omega, r0, r1, in,
                            p, v, y, st);
@@ -1032,8 +1065,8 @@ void Launch<NS, V>::sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* o
 template <int NS, typename V>
 void Launch<NS, V>::sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
                               int r1, const cvec<V>* y, cvec<V>* z, const SysState* st) {
-    hipLaunchKernelGGL((k_sweep_bwd<NS, V>), dim3(row_grid<NS>(r1 - r0, kStreamCap)), dim3(kBlock),
-                       0, s, op, omega, r0, r1, y, z, st);
+    hipLaunchKernelGGL((k_sweep_bwd<NS, V>), dim3(grid_rows(r1 - r0, sweep_rpb<NS>(), kStreamCap)),
+                       dim3(kBlock), 0, s, op, omega, r0, r1, y, z, st);
 }
 template <int NS, typename V>
 void Launch<NS, V>::spmv_v(hipStream_t s, const DevOp<V>& op, const double* omega,

@@ -257,14 +257,39 @@ __global__ void k_loss(int nvox, const float* __restrict__ out, const float* __r
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-// Adam on FP32 master parameters; refreshes the BF16 forward ([tap][co][ci])
-// and backward ([tap][ci][co]) weight copies of the layer.
+// Global gradient norm of all layers, one block: gscale[0] = min(1, clip / ||g||)
+// (1 when clip <= 0), summed in a fixed order.
+struct GradPtrs {
+    const float* w[4];
+    const float* b[4];
+    int nw[4];
+};
+__global__ __launch_bounds__(1024) void k_grad_scale(GradPtrs g, float clip, float* __restrict__ gscale) {
+    __shared__ float sm[1024];
+    float s = 0.f;
+    for (int l = 0; l < 4; ++l) {
+        for (int i = threadIdx.x; i < g.nw[l]; i += blockDim.x) s += g.w[l][i] * g.w[l][i];
+        for (int i = threadIdx.x; i < 32; i += blockDim.x) s += g.b[l][i] * g.b[l][i];
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float nrm = sqrtf(sm[0]);
+        gscale[0] = (clip > 0.f && nrm > clip) ? clip / nrm : 1.f;
+    }
+}
+
+// Adam on FP32 master parameters (gradient scaled by gscale[0]).
 __global__ void k_adam(int count, float lr, float b1, float b2, float eps, float c1, float c2,
-                       float* __restrict__ p, const float* __restrict__ g, float* __restrict__ mo,
-                       float* __restrict__ ve) {
+                       float* __restrict__ p, const float* __restrict__ g, const float* __restrict__ gscale,
+                       float* __restrict__ mo, float* __restrict__ ve) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const float gi = g[i];
+    const float gi = g[i] * gscale[0];
     const float m = b1 * mo[i] + (1.f - b1) * gi;
     const float v = b2 * ve[i] + (1.f - b2) * gi * gi;
     mo[i] = m;
@@ -356,7 +381,8 @@ struct m3d_net {
     Buf<float> w[NL], b[NL], gw[NL], gb[NL], mw[NL], vw[NL], mb[NL], vb[NL];
     Buf<unsigned short> wt[NL], wb[NL];
     Buf<unsigned short> x0, act[NL - 1], grad_a, grad_b, dout;
-    Buf<float> out, part, lpart;
+    Buf<float> out, part, lpart, gscale;
+    float clip = 0.f;  // global gradient-norm clip (0: none)
     static int cin(int l) { return l == 0 ? CIN0 : HID; }
 };
 
@@ -495,6 +521,7 @@ int m3d_net_create(m3d_context* ctx, int max_batch, const int* dilations, unsign
         n->out.alloc(nv * COUT);
         n->part.alloc(size_t(64) * 27 * HID * HID + 256 * 32);
         n->lpart.alloc(512);
+        n->gscale.alloc(1);
         upload_bf16(*n);
         chk(hipStreamSynchronize(n->stream), "sync");
         *out = n.release();
@@ -583,6 +610,13 @@ int m3d_net_prolong(m3d_net* n, int batch, const float* d_field, int n_edges, co
     });
 }
 
+int m3d_net_set_clip(m3d_net* n, float max_norm) {
+    return nn_guard([&] {
+        if (!n || !(max_norm >= 0.f)) throw std::invalid_argument("invalid arguments");
+        n->clip = max_norm;
+    });
+}
+
 int m3d_net_train_step(m3d_net* n, int batch, const float* d_in, const float* d_tgt, float lr, float* loss) {
     return nn_guard([&] {
         if (batch <= 0 || batch > n->max_batch) throw std::invalid_argument("batch out of range");
@@ -590,12 +624,19 @@ int m3d_net_train_step(m3d_net* n, int batch, const float* d_in, const float* d_
         n->step++;
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float c1 = 1.f - std::pow(b1, float(n->step)), c2 = 1.f - std::pow(b2, float(n->step));
+        GradPtrs gp;
+        for (int k = 0; k < NL; ++k) {
+            gp.w[k] = n->gw[k].p;
+            gp.b[k] = n->gb[k].p;
+            gp.nw[k] = 27 * HID * m3d_net::cin(k);
+        }
+        hipLaunchKernelGGL(k_grad_scale, dim3(1), dim3(1024), 0, n->stream, gp, n->clip, n->gscale.p);
         for (int k = 0; k < NL; ++k) {
             const int cnt = 27 * HID * m3d_net::cin(k);
             hipLaunchKernelGGL(k_adam, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, cnt, lr, b1, b2, eps, c1, c2,
-                               n->w[k].p, n->gw[k].p, n->mw[k].p, n->vw[k].p);
+                               n->w[k].p, n->gw[k].p, n->gscale.p, n->mw[k].p, n->vw[k].p);
             hipLaunchKernelGGL(k_adam, dim3(1), dim3(32), 0, n->stream, HID, lr, b1, b2, eps, c1, c2, n->b[k].p,
-                               n->gb[k].p, n->mb[k].p, n->vb[k].p);
+                               n->gb[k].p, n->gscale.p, n->mb[k].p, n->vb[k].p);
         }
         upload_bf16(*n);
         chk(hipStreamSynchronize(n->stream), "sync");

@@ -679,10 +679,13 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
         L::spmv(s, op.view<double>(), W.omega.p, W.aux.p, W.b.p, W.tmp.p);  // rhs_sec
         bool x_zero = true;
         std::vector<int> used(NS, 0);
+        std::vector<double> guess_res(NS, 0.0);
         if (prm.use_initial_guess) {
             // Caller's E_sec guess (mode 2: scaled by the least-squares factor
             // first), kept only where it lowers the residual below ||b||
-            // (0.999 ||b|| when scaled); elsewhere the reference's zero start.
+            // (0.5 ||b|| when scaled: a guess that removes less than half the
+            // residual mostly adds rough error that BiCGStab must undo);
+            // elsewhere the reference's zero start.
             L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
             L64::fix_constrained(s, op.npad, op.constrained.p, W.tmp.p, W.x.p);
             const bool scaled = prm.use_initial_guess == 2;
@@ -692,10 +695,11 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
             }
             std::vector<double> tr = true_residual<NS>(op, W, W.x.p, W.tmp.p, nullptr);
             std::vector<int> zero(NS, 0);
-            const double keep = scaled ? 0.999 : 1.0;
+            const double keep = scaled ? 0.5 : 1.0;
             for (int i = 0; i < NS; ++i) {
                 used[i] = (i < nsys && tr[i] < keep) ? 1 : 0;
                 zero[i] = used[i] ? 0 : 1;
+                if (i < nsys) guess_res[i] = tr[i];
             }
             HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int), hipMemcpyHostToDevice, s));
             L64::zero_systems(s, op.npad, W.flags.p, W.x.p);
@@ -713,6 +717,7 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
             fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
             res[i].outer = steps;
             res[i].guess_used = used[i];
+            res[i].guess_residual = guess_res[i];
         }
     } else if (mode == M3D_MODE_INNER || mode == M3D_MODE_SYSTEM) {
         L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
@@ -867,11 +872,14 @@ void solve_dispatch(m3d_operator* op, int nsys, int mode, const double* omega, c
 }
 
 // Greedy first-fit colouring of the coupling graph in caller order.
-std::vector<int> colour_rows(int n, const int* ptr, const int* col, const unsigned char* constrained,
-                             int* ncolours) {
-    std::vector<int> c(n, -1), stamp(64, -1);
+// Greedy colouring of the rows visited in `order` (smallest colour not used
+// by an already coloured neighbour); constrained and empty rows get colour 0.
+int greedy_colours(int n, const int* ptr, const int* col, const unsigned char* constrained,
+                   const std::vector<int>& order, std::vector<int>& c) {
+    c.assign(n, -1);
+    std::vector<int> stamp(64, -1);
     int nc = 1;
-    for (int i = 0; i < n; ++i) {
+    for (int i : order) {
         if (constrained[i] || ptr[i + 1] == ptr[i]) { c[i] = 0; continue; }
         for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
             const int cj = c[col[k]];
@@ -885,6 +893,27 @@ std::vector<int> colour_rows(int n, const int* ptr, const int* col, const unsign
         if (pick >= int(stamp.size())) stamp.resize(pick + 1, -1);
         c[i] = pick;
         nc = std::max(nc, pick + 1);
+    }
+    return nc;
+}
+
+// Multicolouring for the SSOR sweeps: greedy in row order, then iterated
+// greedy passes (rows re-visited class by class, highest colour first), which
+// never add a colour and usually remove the sparse last ones: every colour is
+// one kernel launch per sweep.
+std::vector<int> colour_rows(int n, const int* ptr, const int* col, const unsigned char* constrained,
+                             int* ncolours) {
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::vector<int> c, trial;
+    int nc = greedy_colours(n, ptr, col, constrained, order, c);
+    for (int pass = 0; pass < 4; ++pass) {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return c[a] > c[b]; });
+        const int nt = greedy_colours(n, ptr, col, constrained, order, trial);
+        if (nt > nc) break;
+        c.swap(trial);
+        nc = nt;
+        for (int i = 0; i < n; ++i) order[i] = i;
     }
     *ncolours = nc;
     return c;
@@ -1318,7 +1347,7 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
         HIPCHK(hipMemcpyAsync(W.tmp.p, ones.data(), nv * sizeof(double2), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
         launch_bicg_begin(s, W.st.p, mask, 1 << 30, 0.0);
-        if (kind == 2) {
+        if (kind != 3) {
             Launch<NS, double>::fill_zero(s, nv, W.x.p);
             Launch<NS, double>::bicg_init(s, A, W.omega.p, W.tmp.p, W.x.p, w.r.p, w.r0.p, W.st.p, W.part.p,
                                           1);
@@ -1343,6 +1372,10 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
         } else if (kind == 3) {
             auto& w32 = vec32<NS>(op, W);
             bicg_iteration<NS, float>(op, W, w32, w32.x.p);
+        } else if (kind == 4) {
+            for (int c = op.ncolours - 1; c >= 0; --c)
+                Launch<NS, double>::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
+                                              W.x.p, W.tmp.p, W.st.p);
         } else {
             for (int c = 0; c < op.ncolours; ++c)
                 Launch<NS, double>::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
@@ -1366,7 +1399,8 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     (void)hipEventDestroy(e1);
     launch_bicg_begin(s, W.st.p, 0u, 1, 1.0);
     HIPCHK(hipStreamSynchronize(s));
-    *avg_ms = double(ms) / reps;
+    // Kind 4 reports one launch: the mean over the colours of a sweep.
+    *avg_ms = double(ms) / reps / (kind == 4 ? op.ncolours : 1);
     long long nfree = 0;
     for (int e = 0; e < op.n; ++e) nfree += op.h_constrained[e] ? 0 : 1;
     const double m = double(op.nnz), n = double(nfree), k = double(nsys);
@@ -1379,7 +1413,16 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     auto iter_b = [&](double wv, double wc) {
         return 2.0 * spmv_b(wv, wc) + 2.0 * ssor_b(wv, wc) + 17.0 * k * n * wc;
     };
-    *bytes = kind == 0 ? spmv_b(8, 16) : kind == 1 ? ssor_b(8, 16) : kind == 2 ? iter_b(8, 16) : iter_b(4, 8);
+    // Backward sweep: the upper half of the couplings, the diagonal pair, y
+    // in and z out, per launch (one colour on average).
+    auto bwd_b = [&](double wv, double wc) {
+        return (0.5 * m * (wv + 4.0) + 2.0 * n * wv + 2.0 * k * n * wc) / op.ncolours;
+    };
+    *bytes = kind == 0   ? spmv_b(8, 16)
+             : kind == 1 ? ssor_b(8, 16)
+             : kind == 2 ? iter_b(8, 16)
+             : kind == 3 ? iter_b(4, 8)
+                         : bwd_b(8, 16);
 }
 
 template <typename F>
@@ -1444,7 +1487,7 @@ int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, 
                      double* avg_ms, double* alg_bytes) {
     return guarded([&] {
         check_ready(op);
-        if (!omega || !avg_ms || !alg_bytes || reps <= 0 || kind < 0 || kind > 3)
+        if (!omega || !avg_ms || !alg_bytes || reps <= 0 || kind < 0 || kind > 4)
             throw Fail(M3D_ERR_ARG, "invalid benchmark arguments");
         HIPCHK(hipSetDevice(op->ctx->device));
         for_width(nsys, [&](int k) {

@@ -19,7 +19,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import configs
-from .native import Context, Model, NativeError, lib, _ptr
+from .native import PREC_MIXED, Context, Model, NativeError, lib, _ptr
 
 G = 32
 NCH = 12
@@ -144,6 +144,7 @@ class CoarseNet:
         L.m3d_net_forward.argtypes = [vp, i, vp, vp]
         L.m3d_net_gradients.argtypes = [vp, i, vp, vp, C.POINTER(f)]
         L.m3d_net_train_step.argtypes = [vp, i, vp, vp, f, C.POINTER(f)]
+        L.m3d_net_set_clip.argtypes = [vp, f]
         L.m3d_net_prolong.argtypes = [vp, i, vp, i, vp, vp, vp, vp]
         self.L = L
         self.ctx = ctx
@@ -195,6 +196,10 @@ class CoarseNet:
                                            C.byref(loss)))
         return loss.value
 
+    def set_clip(self, max_norm: float) -> None:
+        """Global gradient-norm clip applied by train_step (0 disables)."""
+        self._chk(self.L.m3d_net_set_clip(self.handle, max_norm))
+
     def train_step(self, d_in, d_tgt, batch: int, lr: float) -> float:
         loss = C.c_float()
         self._chk(self.L.m3d_net_train_step(self.handle, batch, C.c_void_p(d_in), C.c_void_p(d_tgt),
@@ -219,9 +224,13 @@ class Sample:
     iterations: list = field(default_factory=list)
 
 
-def solve_model(model: Model, wl: configs.Workload):
-    """Forward solve of every frequency; returns E_sec per system and results."""
-    E, Z, res = model.solve_frequencies(wl.freqs)
+def solve_model(model: Model, wl: configs.Workload, params=None):
+    """Forward solve of every frequency; returns E_sec per system and results.
+    Training targets need far less than the solve tolerance, so by default the
+    mixed-precision solve runs to a relative residual of 1e-6."""
+    if params is None:
+        params = wl.fwd.solve_params(precision=PREC_MIXED, inner_tol=1e-3, tol=1e-6)
+    E, Z, res = model.solve_frequencies(wl.freqs, params)
     esec = np.empty_like(E)
     for f, fr in enumerate(wl.freqs):
         w = 2 * np.pi * fr
@@ -230,20 +239,40 @@ def solve_model(model: Model, wl: configs.Workload):
     return esec, res
 
 
-def make_sample(model: Model, wl: configs.Workload, box: Box) -> Sample:
+def normalise(y: np.ndarray) -> np.ndarray:
+    """Scale each polarisation's six channels of each sample ([nf, G^3, 12])
+    to unit RMS.  The network learns the shape of E_sec; the forward solve
+    restores the amplitude per system with its least-squares scaling of the
+    guess (use_initial_guess = 2)."""
+    out = y.copy()
+    for pol in range(2):
+        ch = slice(6 * pol, 6 * pol + 6)
+        rms = np.sqrt(np.mean(out[:, :, ch].astype(np.float64) ** 2, axis=(1, 2)))
+        rms[rms == 0] = 1.0
+        out[:, :, ch] /= rms[:, None, None].astype(np.float32)
+    return out
+
+
+def make_sample(model: Model, wl: configs.Workload, box: Box) -> Optional[Sample]:
+    """Training sample of one model (None if a solve did not converge)."""
     esec, res = solve_model(model, wl)
+    if not all(r["converged"] for r in res):
+        return None
     logsig = sample_sigma(model, model.array("sigma"), box)
     x = features(logsig, wl.freqs)
     y = np.stack([deposit(model, box, esec[2 * f], esec[2 * f + 1]) for f in range(len(wl.freqs))])
-    return Sample(x, y, [r["iterations"] for r in res])
+    return Sample(x, normalise(y), [r["iterations"] for r in res])
 
 
 def training_workload(seed: int, max_level: int = 5) -> configs.Workload:
     return configs.random3d(seed, max_level=max_level, half_width=100e3, station_radius=20e3)
 
 
-def train(net: CoarseNet, samples: List[Sample], steps: int, lr: float = 1e-3, batch: int = 8,
-          seed: int = 0, log=None) -> List[float]:
+def train(net: CoarseNet, samples: List[Sample], steps: int, lr: float = 5e-4, batch: int = 8,
+          seed: int = 0, log=None, warmup: int = 100) -> List[float]:
+    """Adam on random mini-batches: linear warm-up over `warmup` steps, then
+    cosine decay to lr/100; gradients clipped to twice the median global norm
+    of the first 20 batches."""
     import torch
     dev = torch.device("cuda:0")
     X = torch.from_numpy(np.concatenate([s.x for s in samples])).to(dev)
@@ -252,16 +281,35 @@ def train(net: CoarseNet, samples: List[Sample], steps: int, lr: float = 1e-3, b
     rng = np.random.default_rng(seed)
     xb = torch.empty((batch,) + tuple(X.shape[1:]), dtype=torch.float32, device=dev)
     yb = torch.empty((batch,) + tuple(Y.shape[1:]), dtype=torch.float32, device=dev)
-    losses = []
-    for it in range(steps):
+
+    def load():
         idx = torch.from_numpy(rng.integers(0, n, batch)).to(dev)
         xb.copy_(X[idx])
         yb.copy_(Y[idx])
         torch.cuda.synchronize()
-        step_lr = lr * (0.1 ** (it / max(steps, 1)))
+
+    norms = []
+    for _ in range(20):
+        load()
+        net.gradients(xb.data_ptr(), yb.data_ptr(), batch)
+        gw, gb = net.params(grads=True)
+        norms.append(float(np.sqrt(np.sum(gw.astype(np.float64) ** 2) + np.sum(gb.astype(np.float64) ** 2))))
+    clip = 2.0 * float(np.median(norms))
+    net.set_clip(clip)
+    if log:
+        log(f"[learned] gradient clip {clip:.3e} (median initial norm {np.median(norms):.3e})")
+    losses = []
+    for it in range(steps):
+        load()
+        if it < warmup:
+            step_lr = lr * (it + 1) / warmup
+        else:
+            t = (it - warmup) / max(steps - warmup, 1)
+            step_lr = lr * (0.01 + 0.99 * 0.5 * (1.0 + np.cos(np.pi * t)))
         losses.append(net.train_step(xb.data_ptr(), yb.data_ptr(), batch, step_lr))
-        if log and (it % 100 == 0 or it == steps - 1):
-            log(f"[learned] step {it} loss {losses[-1]:.4e}")
+        if log and (it % 500 == 0 or it == steps - 1):
+            log(f"[learned] step {it} lr {step_lr:.2e} loss {losses[-1]:.4e} "
+                f"(mean of last 100: {np.mean(losses[-100:]):.4e})")
     return losses
 
 

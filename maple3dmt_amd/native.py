@@ -81,6 +81,7 @@ class SolveResultC(C.Structure):
         ("info", C.c_int),
         ("outer", C.c_int),
         ("guess_used", C.c_int),
+        ("guess_residual", C.c_double),
     ]
 
 
@@ -271,7 +272,8 @@ class SolveParams:
 def results_to_dicts(res) -> list:
     return [dict(iterations=r.iterations, residual=r.residual, true_residual=r.true_residual,
                  converged=bool(r.converged), diverged=bool(r.diverged), info=r.info,
-                 outer=r.outer, guess_used=bool(r.guess_used)) for r in res]
+                 outer=r.outer, guess_used=bool(r.guess_used),
+                 guess_residual=r.guess_residual) for r in res]
 
 
 # --------------------------------------------------------------------------
@@ -327,7 +329,8 @@ class DeviceOperator:
     def bench_kernel(self, kind: int, omegas, reps: int = 20):
         """(avg ms per launch, algorithmic bytes per launch) of kind 0 = SpMV,
         1 = SSOR apply, 2 = one FP64 BiCGStab iteration, 3 = one complex64
-        BiCGStab iteration, timed with HIP events on the library stream."""
+        BiCGStab iteration, 4 = one backward-sweep launch (mean over the
+        colours), timed with HIP events on the library stream."""
         om = np.ascontiguousarray(omegas, dtype=np.float64)
         ms, by = C.c_double(), C.c_double()
         _check_dev(lib().m3d_bench_kernel(self.handle, kind, om.shape[0], _ptr(om), reps,

@@ -1,14 +1,27 @@
-"""Train the coarse-grid e0 network on solver-generated solutions and
-evaluate it against the zero initial guess.
+"""Train the coarse-grid e0 network on solver-generated solutions, and
+evaluate what it gains over the zero initial guess.
 
-    python -m maple3dmt_amd.train_e0 --train-models 200 --eval-models 20 \
-        --steps 1500 --out maple3dmt_amd/data/e0_net.npz
+    python -m maple3dmt_amd.train_e0 train --models 250 --level 6 --steps 1500 \
+        --out maple3dmt_amd/data/e0_net.npz
+    python -m maple3dmt_amd.train_e0 eval --weights maple3dmt_amd/data/e0_net.npz \
+        --models 12 --level 6 --report e0_eval.json
 
-Training models are seeded random 3-D models (three layers plus blocks) on
-their own octree meshes; each contributes its 8 frequencies x 2
-polarisations.  Evaluation solves held-out models twice, from zero and from
-the predicted e0 (zero-guess fallback where ||b - A e0|| >= ||b||), at the
-same tolerance, and reports iterations and solves/s.
+Training models are seeded random 3-D models (three layers plus one to four
+blocks, configs.random3d) on their own octree meshes; each contributes its 8
+frequencies.  Targets are the solver's own E_sec = E - E_pri (mixed-precision
+solve to a relative residual of 1e-6; models with an unconverged system are
+skipped) deposited on the network's voxel grid and scaled to unit RMS per
+polarisation: the network predicts the shape, the solve's least-squares
+scaling of the guess restores the amplitude.
+
+Evaluation solves held-out models (seeds disjoint from training) at the same
+tolerance four ways: plain FP64 from zero, FP64 from the network's e0, mixed
+precision from zero and mixed from e0.  The e0 start is the least-squares
+scaled guess (use_initial_guess = 2): kept per system only where it lowers the
+residual by half, else the zero start.  Reported per model: BiCGStab
+iterations per system, solve wall time (the e0 variants include the network
+inference and prolongation) and the largest |dZ|/|Z| against the FP64
+zero-start solve.
 """
 from __future__ import annotations
 
@@ -22,129 +35,158 @@ import numpy as np
 
 from . import configs
 from .learned import Box, CoarseNet, initial_guess, make_sample, train, training_workload
-from .native import MODE_FORWARD, Context
+from .native import MODE_FORWARD, PREC_MIXED, Context
 
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+TRAIN_SEED0 = 10_000
+EVAL_SEED0 = 50_000
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def evaluate(ctx, net, box, seeds, max_level, repeats=1):
+def cmd_train(a):
+    import torch  # noqa: F401  (device memory for the training batches)
+    ctx = Context(0)
+    box = Box()
+    net = CoarseNet(ctx, max_batch=max(a.batch, 8), seed=a.seed)
+    t0 = time.time()
+    samples, its, edges, skipped = [], 0, [], 0
+    for k in range(a.models):
+        wl = training_workload(TRAIN_SEED0 + k, max_level=a.level)
+        m = configs.build_model(wl, ctx)
+        s = make_sample(m, wl, box)
+        m.close()
+        if s is None:
+            skipped += 1
+            continue
+        its += sum(s.iterations)
+        edges.append(m.n)
+        samples.append(s)
+        if k % 10 == 0:
+            log(f"[data] model {k}: {m.n} edges, {time.time() - t0:.0f}s")
+    t_data = time.time() - t0
+    t1 = time.time()
+    losses = train(net, samples, a.steps, a.lr, a.batch, seed=a.seed, log=log)
+    t_train = time.time() - t1
+    w, b = net.params()
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    np.savez(a.out, w=w, b=b, box=box.arr(), dilations=np.array(net.dil))
+    rep = {"models": a.models, "skipped_unconverged": skipped, "level": a.level,
+           "edges_mean": float(np.mean(edges)), "samples": len(samples) * 8, "data_seconds": t_data, "data_solver_iterations": its,
+           "steps": a.steps, "batch": a.batch, "lr": a.lr, "train_seconds": t_train,
+           "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:]))}
+    log(f"[train] {json.dumps(rep)}; saved {a.out}")
+    print(json.dumps(rep), flush=True)
+    net.close()
+    ctx.close()
+
+
+def _solve(m, omegas, d_rhs, d_aux, d_x, prm):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r = m.device.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(), d_x.data_ptr(),
+                              len(omegas), prm)
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t
+
+
+def cmd_eval(a):
     import torch
     dev = torch.device("cuda:0")
+    ctx = Context(0)
+    d = np.load(a.weights)
+    box = Box(*d["box"].tolist())
+    net = CoarseNet(ctx, max_batch=8, dilations=tuple(int(v) for v in d["dilations"]))
+    net.set_params(d["w"], d["b"])
     rows = []
-    for seed in seeds:
-        wl = configs.random3d(seed, max_level=max_level, half_width=100e3, station_radius=20e3)
+    for k in range(a.models):
+        seed = EVAL_SEED0 + k
+        wl = configs.random3d(seed, max_level=a.level, half_width=100e3, station_radius=20e3,
+                              maxiter=a.maxiter)
         m = configs.build_model(wl, ctx)
+        nf = len(wl.freqs)
         omegas = np.repeat(2 * np.pi * wl.freqs, 2)
-        ns = len(omegas)
         rhs = np.stack([m.rhs_total(m.bc_values(w, p % 2)) for p, w in enumerate(omegas)])
         aux = np.stack([m.primary_field(w, p % 2) for p, w in enumerate(omegas)])
         d_rhs = torch.from_numpy(rhs).to(dev)
         d_aux = torch.from_numpy(aux).to(dev)
-        d_x = torch.zeros_like(d_rhs)
-        prm0 = wl.fwd.solve_params()
-        prm1 = wl.fwd.solve_params()
-        prm1.use_initial_guess = 2
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(repeats):
-            r0 = m.device.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(),
-                                       d_x.data_ptr(), ns, prm0)
-        torch.cuda.synchronize()
-        dt0 = (time.perf_counter() - t0) / repeats
-        x_zero = d_x.cpu().numpy()
-        t1 = time.perf_counter()
-        for _ in range(repeats):
-            x0 = initial_guess(net, m, wl, box)       # network + prolongation on the device
-            r1 = m.device.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(),
-                                       x0.data_ptr(), ns, prm1)
-        torch.cuda.synchronize()
-        dt1 = (time.perf_counter() - t1) / repeats
-        x_learn = x0.cpu().numpy()
-        dz = []
-        for f in range(len(wl.freqs)):
-            za = m.extract_impedance(omegas[2 * f], x_zero[2 * f], x_zero[2 * f + 1])
-            zb = m.extract_impedance(omegas[2 * f], x_learn[2 * f], x_learn[2 * f + 1])
-            dz.append(float(np.max(np.abs(zb[:, 1:3] - za[:, 1:3]) / np.abs(za[:, 1:3]))))
-        row = dict(seed=seed, edges=m.n,
-                   iters_zero=[r["iterations"] for r in r0],
-                   iters_e0=[r["iterations"] for r in r1],
-                   guess_used=[r["guess_used"] for r in r1],
-                   converged=all(r["converged"] for r in r0 + r1),
-                   s_zero=dt0, s_e0=dt1, max_rel_dZ=max(dz))
-        log(f"[eval] seed {seed}: {m.n} edges, iterations {sum(row['iters_zero'])} -> "
-            f"{sum(row['iters_e0'])}, {dt0:.2f}s -> {dt1:.2f}s, dZ {row['max_rel_dZ']:.1e}")
+        row = {"seed": seed, "edges": m.n}
+        zref = None
+        for prec_name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
+            for start in ("zero", "e0"):
+                prm = wl.fwd.solve_params(precision=prec, inner_tol=1e-3 if prec else 0.0)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                if start == "zero":
+                    d_x = torch.zeros_like(d_rhs)
+                    r, dt = _solve(m, omegas, d_rhs, d_aux, d_x, prm)
+                    dt_net = 0.0
+                else:
+                    d_x = initial_guess(net, m, wl, box)
+                    torch.cuda.synchronize()
+                    dt_net = time.perf_counter() - t
+                    prm.use_initial_guess = 2
+                    r, dt = _solve(m, omegas, d_rhs, d_aux, d_x, prm)
+                x = d_x.cpu().numpy()
+                z = np.stack([m.extract_impedance(omegas[2 * f], x[2 * f], x[2 * f + 1]) for f in range(nf)])
+                if zref is None:
+                    zref = z
+                off = (slice(None), slice(None), slice(1, 3))
+                key = f"{prec_name}_{start}"
+                row[key] = {"iterations": [q["iterations"] for q in r],
+                            "guess_used": [q["guess_used"] for q in r],
+                            "guess_residual": [q["guess_residual"] for q in r],
+                            "converged": all(q["converged"] for q in r),
+                            "seconds": dt + dt_net, "net_seconds": dt_net,
+                            "max_rel_dZ": float(np.max(np.abs(z[off] - zref[off]) / np.abs(zref[off])))}
+        log(f"[eval] seed {seed}: {m.n} edges; iterations fp64 {sum(row['fp64_zero']['iterations'])} -> "
+            f"{sum(row['fp64_e0']['iterations'])}, mixed {sum(row['mixed_zero']['iterations'])} -> "
+            f"{sum(row['mixed_e0']['iterations'])}; seconds fp64 {row['fp64_zero']['seconds']:.2f} -> "
+            f"{row['fp64_e0']['seconds']:.2f}, mixed {row['mixed_zero']['seconds']:.2f} -> "
+            f"{row['mixed_e0']['seconds']:.2f}")
         rows.append(row)
         m.close()
-    return rows
+    summ = {}
+    for key in ("fp64_zero", "fp64_e0", "mixed_zero", "mixed_e0"):
+        summ[key] = {"iterations": int(sum(sum(r[key]["iterations"]) for r in rows)),
+                     "seconds": float(sum(r[key]["seconds"] for r in rows)),
+                     "solves_per_s": float(16 * len(rows) / sum(r[key]["seconds"] for r in rows)),
+                     "all_converged": all(r[key]["converged"] for r in rows),
+                     "guess_used_fraction": float(np.mean([np.mean(r[key]["guess_used"]) for r in rows])),
+                     "max_rel_dZ": float(max(r[key]["max_rel_dZ"] for r in rows))}
+    rep = {"weights": os.path.basename(a.weights), "models": a.models, "level": a.level,
+           "summary": summ, "per_model": rows}
+    log(f"[eval] summary {json.dumps(summ)}")
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f, indent=1)
+    print(json.dumps(summ), flush=True)
+    net.close()
+    ctx.close()
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train-models", type=int, default=200)
-    ap.add_argument("--eval-models", type=int, default=20)
-    ap.add_argument("--train-level", type=int, default=5)
-    ap.add_argument("--eval-level", type=int, default=6)
-    ap.add_argument("--steps", type=int, default=1500)
-    ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--out", default=os.path.join(DATA, "e0_net.npz"))
-    ap.add_argument("--weights", default=None, help="skip training, load these weights")
-    ap.add_argument("--report", default=None)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--models", type=int, default=250)
+    t.add_argument("--level", type=int, default=6)
+    t.add_argument("--steps", type=int, default=6000)
+    t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--lr", type=float, default=5e-4)
+    t.add_argument("--seed", type=int, default=1)
+    t.add_argument("--out", default=os.path.join(DATA, "e0_net.npz"))
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", default=os.path.join(DATA, "e0_net.npz"))
+    e.add_argument("--models", type=int, default=12)
+    e.add_argument("--level", type=int, default=6)
+    e.add_argument("--maxiter", type=int, default=40000)
+    e.add_argument("--report", default=None)
     a = ap.parse_args(argv)
-
-    import torch  # noqa: F401  (device memory for training batches)
-    ctx = Context(0)
-    box = Box()
-    net = CoarseNet(ctx, max_batch=max(a.batch, 8), seed=1)
-    report = {"box": box.arr().tolist(), "train_models": a.train_models, "train_level": a.train_level,
-              "eval_level": a.eval_level}
-    if a.weights:
-        d = np.load(a.weights)
-        net.set_params(d["w"], d["b"])
-    else:
-        t0 = time.time()
-        samples = []
-        gen_solve_iters = 0
-        for k in range(a.train_models):
-            wl = training_workload(10_000 + k, max_level=a.train_level)
-            m = configs.build_model(wl, ctx)
-            s = make_sample(m, wl, box)
-            gen_solve_iters += sum(s.iterations)
-            samples.append(s)
-            m.close()
-            if k % 20 == 0:
-                log(f"[data] model {k}: {m.n} edges, {time.time() - t0:.0f}s")
-        report["data_seconds"] = time.time() - t0
-        report["data_solver_iterations"] = gen_solve_iters
-        t1 = time.time()
-        losses = train(net, samples, a.steps, a.lr, a.batch, log=log)
-        report["train_seconds"] = time.time() - t1
-        report["loss_first"] = float(np.mean(losses[:20]))
-        report["loss_last"] = float(np.mean(losses[-20:]))
-        w, b = net.params()
-        os.makedirs(os.path.dirname(a.out), exist_ok=True)
-        np.savez(a.out, w=w, b=b, box=box.arr(), dilations=np.array(net.dil))
-        log(f"[train] loss {report['loss_first']:.3e} -> {report['loss_last']:.3e}; saved {a.out}")
-    rows = evaluate(ctx, net, box, [50_000 + k for k in range(a.eval_models)], a.eval_level)
-    it0 = sum(sum(r["iters_zero"]) for r in rows)
-    it1 = sum(sum(r["iters_e0"]) for r in rows)
-    s0 = sum(r["s_zero"] for r in rows)
-    s1 = sum(r["s_e0"] for r in rows)
-    report["eval"] = rows
-    report["summary"] = {"iterations_zero": it0, "iterations_e0": it1,
-                         "iteration_ratio": it1 / max(it0, 1), "seconds_zero": s0, "seconds_e0": s1,
-                         "speedup": s0 / max(s1, 1e-9),
-                         "max_rel_dZ": max(r["max_rel_dZ"] for r in rows),
-                         "all_converged": all(r["converged"] for r in rows)}
-    log(f"[eval] {json.dumps(report['summary'])}")
-    if a.report:
-        with open(a.report, "w") as f:
-            json.dump(report, f, indent=1)
-    print(json.dumps(report["summary"]), flush=True)
+    cmd_train(a) if a.cmd == "train" else cmd_eval(a)
 
 
 if __name__ == "__main__":
