@@ -118,6 +118,12 @@ typedef struct {
                                 2: scaled per system by gamma = (A e0, b) / (A e0, A e0), the
                                 least-squares factor, and used where ||b - gamma A e0|| <
                                 0.5 ||b||.  Elsewhere the zero start. */
+    int interleaved_forward; /* FORWARD: 0 = the reference's full BiCGStab on E_sec
+                                (forward_solver_fv.cpp:347-354); 1 = the interleaved
+                                BiCGStab + DivCorr bursts of solve_with_divcorr_
+                                (:778-888) on E_sec, with a full BiCGStab from the
+                                current E_sec for any system that stagnates or
+                                diverges.  The post-processing is the same. */
 } m3d_solve_params;
 
 /* Per-system outcome; mirrors BiCGStabResult (bicgstab.h:23-28) plus what the

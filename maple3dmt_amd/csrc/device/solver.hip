@@ -656,6 +656,69 @@ BicgOutcome solve_linear(m3d_operator& op, Work64<NS>& W, const double2* b, doub
     return total;
 }
 
+// ModEM-style interleaved BiCGStab + DivCorr on A x = b from the current x
+// (solve_with_divcorr_, forward_solver_fv.cpp:778-888): phi0 = G^T b / (i w),
+// an initial DivCorr, then bursts of divcorr_iter_per_dc BiCGStab iterations,
+// each followed by DivCorr and an FP64 true residual.  A system stops when
+// the burst or the true residual reaches tol, when a burst ends above
+// residual 1 (x zeroed, diverged), or when a burst gains less than 5 %
+// (accepted as is).  Iterations accumulate into res.  With `unfinished`
+// non-null, systems that diverged, stagnated or used every burst are not
+// accepted but reported there (their x is left as it is; zero if diverged).
+template <int NS>
+void interleaved(m3d_operator& op, Work64<NS>& W, const double2* b, double2* x, unsigned mask, int nsys,
+                 const m3d_solve_params& prm, m3d_solve_result* res, unsigned* unfinished) {
+    using L64 = Launch64<NS>;
+    hipStream_t s = op.ctx->stream;
+    ensure_dc_vectors<NS>(op, W);
+    L64::phi0(s, op.nn, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p, op.dc_dead.p, W.omega.p, b, W.phi0.p);
+    divcorr<NS>(op, W, x, W.phi0.p, mask, prm);
+    unsigned active = mask, left = 0;
+    std::vector<double> prev(NS, 1e30);
+    std::vector<int> zero(NS, 0);
+    for (int outer = 0; outer < prm.divcorr_outer_max && active; ++outer) {
+        int steps = 1;
+        BicgOutcome o = solve_linear<NS>(op, W, b, x, active, prm.divcorr_iter_per_dc, prm, false, &steps);
+        for (int i = 0; i < nsys; ++i) {
+            if (!(active & (1u << i))) continue;
+            res[i].iterations += o.iterations[i];
+            res[i].residual = o.residual[i];
+            res[i].info = o.info[i];
+            res[i].outer = outer + 1;
+            if (o.residual[i] > 1.0) {
+                std::fill(zero.begin(), zero.end(), 0);
+                zero[i] = 1;
+                res[i].diverged = unfinished ? 0 : 1;
+                active &= ~(1u << i);
+                left |= 1u << i;
+                HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int), hipMemcpyHostToDevice, s));
+                L64::zero_systems(s, op.npad, W.flags.p, x);
+                HIPCHK(hipStreamSynchronize(s));
+            } else if (o.status[i] == kConverged || o.status[i] == kConvergedEarly) {
+                res[i].converged = 1;
+                active &= ~(1u << i);
+            }
+        }
+        if (!active) break;
+        divcorr<NS>(op, W, x, W.phi0.p, active, prm);
+        std::vector<double> tr = true_residual<NS>(op, W, x, b, nullptr);
+        for (int i = 0; i < nsys; ++i) {
+            if (!(active & (1u << i))) continue;
+            if (tr[i] < prm.tol) {
+                res[i].converged = 1;
+                active &= ~(1u << i);
+            } else if (outer > 0 && tr[i] > 0.95 * prev[i]) {
+                active &= ~(1u << i);  // stagnation: accept (:866-873)
+                left |= 1u << i;
+            } else {
+                prev[i] = tr[i];
+            }
+        }
+    }
+    left |= active;  // out of bursts
+    if (unfinished) *unfinished = left;
+}
+
 template <int NS>
 void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const double2* d_rhs,
                 const double2* d_aux, double2* d_x, const m3d_solve_params& prm,
@@ -707,15 +770,38 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
         } else {
             L::fill_zero(s, nv, W.x.p);
         }
-        int steps = 1;
-        BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, mask, prm.maxiter, prm, x_zero, &steps);
-        for (int i = 0; i < nsys; ++i) res[i].guess_used = used[i];
+        bool all_omega = true;
+        for (int i = 0; i < nsys; ++i)
+            if (!(omega[i] > 0)) all_omega = false;
+        if (prm.interleaved_forward && dc && all_omega) {
+            // E_sec by interleaved BiCGStab + DivCorr bursts; systems that do
+            // not converge that way finish with the reference's full BiCGStab
+            // from where they stand.
+            unsigned left = 0;
+            interleaved<NS>(op, W, W.tmp.p, W.x.p, mask, nsys, prm, res, &left);
+            if (left) {
+                int steps = 1;
+                BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, left, prm.maxiter, prm, false, &steps);
+                for (int i = 0; i < nsys; ++i) {
+                    if (!(left & (1u << i))) continue;
+                    const int before = res[i].iterations;
+                    fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
+                    res[i].iterations += before;
+                    res[i].outer += steps;
+                }
+            }
+        } else {
+            int steps = 1;
+            BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, mask, prm.maxiter, prm, x_zero, &steps);
+            for (int i = 0; i < nsys; ++i) {
+                fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
+                res[i].outer = steps;
+            }
+        }
         interp_all<NS>(op, W.x.p);
         L64::forward_total(s, op.npad, op.bc.p, W.aux.p, W.b.p, W.x.p);
         if (dc) divcorr<NS>(op, W, W.x.p, nullptr, mask, prm);
         for (int i = 0; i < nsys; ++i) {
-            fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
-            res[i].outer = steps;
             res[i].guess_used = used[i];
             res[i].guess_residual = guess_res[i];
         }
@@ -762,55 +848,7 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
                 L64::zero_systems(s, op.npad, W.flags.p, W.x.p);
             }
         } else {
-            ensure_dc_vectors<NS>(op, W);
-            L64::phi0(s, op.nn, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p, op.dc_dead.p, W.omega.p, W.b.p,
-                      W.phi0.p);
-            divcorr<NS>(op, W, W.x.p, W.phi0.p, mask, prm);
-            unsigned active = mask;
-            std::vector<double> prev(NS, 1e30);
-            std::vector<int> zero(NS, 0);
-            for (int outer = 0; outer < prm.divcorr_outer_max && active; ++outer) {
-                int steps = 1;
-                BicgOutcome o = solve_linear<NS>(op, W, W.b.p, W.x.p, active, prm.divcorr_iter_per_dc,
-                                                 prm, false, &steps);
-                bool any_zero = false;
-                for (int i = 0; i < nsys; ++i) {
-                    if (!(active & (1u << i))) continue;
-                    res[i].iterations += o.iterations[i];
-                    res[i].residual = o.residual[i];
-                    res[i].info = o.info[i];
-                    res[i].outer = outer + 1;
-                    if (o.residual[i] > 1.0) {
-                        std::fill(zero.begin(), zero.end(), 0);
-                        zero[i] = 1;
-                        res[i].diverged = 1;
-                        active &= ~(1u << i);
-                        any_zero = true;
-                        HIPCHK(hipMemcpyAsync(W.flags.p, zero.data(), NS * sizeof(int),
-                                              hipMemcpyHostToDevice, s));
-                        L64::zero_systems(s, op.npad, W.flags.p, W.x.p);
-                        HIPCHK(hipStreamSynchronize(s));
-                    } else if (o.status[i] == kConverged || o.status[i] == kConvergedEarly) {
-                        res[i].converged = 1;
-                        active &= ~(1u << i);
-                    }
-                }
-                (void)any_zero;
-                if (!active) break;
-                divcorr<NS>(op, W, W.x.p, W.phi0.p, active, prm);
-                std::vector<double> tr = true_residual<NS>(op, W, W.x.p, W.b.p, nullptr);
-                for (int i = 0; i < nsys; ++i) {
-                    if (!(active & (1u << i))) continue;
-                    if (tr[i] < prm.tol) {
-                        res[i].converged = 1;
-                        active &= ~(1u << i);
-                    } else if (outer > 0 && tr[i] > 0.95 * prev[i]) {
-                        active &= ~(1u << i);  // stagnation: accept (:866-873)
-                    } else {
-                        prev[i] = tr[i];
-                    }
-                }
-            }
+            interleaved<NS>(op, W, W.b.p, W.x.p, mask, nsys, prm, res, nullptr);
         }
     } else {
         throw Fail(M3D_ERR_ARG, "unknown solve mode");
@@ -836,6 +874,8 @@ void check_params(const m3d_solve_params* p) {
         throw Fail(M3D_ERR_ARG, "precision must be M3D_PREC_FP64 or M3D_PREC_MIXED");
     if (p->use_initial_guess < 0 || p->use_initial_guess > 2)
         throw Fail(M3D_ERR_ARG, "use_initial_guess must be 0, 1 or 2");
+    if (p->interleaved_forward != 0 && p->interleaved_forward != 1)
+        throw Fail(M3D_ERR_ARG, "interleaved_forward must be 0 or 1");
 }
 
 template <typename F>

@@ -68,6 +68,7 @@ class SolveParamsC(C.Structure):
         ("precision", C.c_int),
         ("inner_tol", C.c_double),
         ("use_initial_guess", C.c_int),
+        ("interleaved_forward", C.c_int),
     ]
 
 
@@ -262,11 +263,13 @@ class SolveParams:
     precision: int = PREC_FP64
     inner_tol: float = 0.0
     use_initial_guess: int = 0   # 0 zero start, 1 guess as given, 2 least-squares scaled guess
+    interleaved_forward: int = 0  # FORWARD E_sec by interleaved BiCGStab + DivCorr bursts
 
     def c(self) -> SolveParamsC:
         return SolveParamsC(self.tol, self.maxiter, int(self.div_correction), self.div_corr_iters,
                             self.div_corr_tol, self.divcorr_outer_max, self.divcorr_iter_per_dc,
-                            self.precision, self.inner_tol, int(self.use_initial_guess))
+                            self.precision, self.inner_tol, int(self.use_initial_guess),
+                            int(self.interleaved_forward))
 
 
 def results_to_dicts(res) -> list:

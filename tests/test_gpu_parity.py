@@ -158,6 +158,21 @@ def test_forward_mixed_precision_parity(block):
         assert dz.max() <= 1e-5 and drho.max() <= 1e-3 and dph.max() <= 0.05
 
 
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_forward_interleaved_divcorr_parity(block, precision):
+    # E_sec by interleaved BiCGStab + DivCorr bursts instead of one full
+    # BiCGStab: the same system to the same tolerance, so Z meets the same bar.
+    wl, m, o = block
+    prm = m.fwd_params.solve_params(precision=precision, inner_tol=1e-3 if precision else 0.0)
+    prm.interleaved_forward = 1
+    E, Z, res = m.solve_frequencies(wl.freqs, prm)
+    assert all(r["converged"] for r in res)
+    for fi, f in enumerate(wl.freqs):
+        _, _, zr, _ = o.solve_frequency(f)
+        dz, drho, dph = _z_errors(Z[fi][:, 1:3], zr[:, 1:3], 2 * np.pi * f)
+        assert dz.max() <= 1e-5 and drho.max() <= 1e-3 and dph.max() <= 0.05, (f, dz.max())
+
+
 def test_divcorr_matches_oracle(block):
     wl, m, o = block
     w = 2 * np.pi * wl.freqs[0]
