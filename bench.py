@@ -7,11 +7,18 @@ relative residual 1e-7 with the reference's forward path (scattered field,
 SSOR-BiCGStab from zero, dead-edge interpolation, post-hoc DivCorr) as one
 batched device call whose inputs (rhs_total, E_pri) already sit in HBM.
 
-value = solved systems per second of the plain FP64 method over the K timed
-steps.  The JSON line also carries:
-  mixed          the same batch with complex64 inner solves in FP64 refinement
-  mixed_learned  mixed, started from the trained network's e0 (inference and
-                 prolongation inside the timed region), when its weights exist
+value = solved systems per second of the plain FP64 method (the reference's
+forward solve, batched) over the K timed steps.  The JSON line also carries,
+under "variants", the same batch solved
+  mixed                      with complex64 inner solves in FP64 refinement
+  fp64_interleaved           with E_sec by BiCGStab bursts of 100 iterations
+                             each followed by DivCorr (solve_with_divcorr_)
+  mixed_interleaved          both: the fast path
+  mixed_interleaved_learned  the fast path started from the trained network's
+                             e0 (inference and prolongation inside the timed
+                             region), when its weights exist
+each with its speed-up over the plain FP64 batch and its largest |dZ|/|Z|
+against it; and
   roofline       the kernel that dominates the timed region (the backward SSOR
                  sweep at the tail batch width), timed with HIP events on the
                  library's stream, against HBM peak
@@ -51,6 +58,8 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--dc-interval", type=int, default=100,
+                    help="BiCGStab iterations between DivCorr steps in the interleaved variants")
     ap.add_argument("--tail-width", type=int, default=2,
                     help="batch width of the roofline kernel (the width the low-frequency tail runs at)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
@@ -181,39 +190,49 @@ def main():
         f"iterations {iters}, true residual max {max(r['true_residual'] for r in res):.2e}")
     z64 = impedances(model, omegas, x64.cpu().numpy(), nf)
 
-    # Mixed precision: complex64 inner BiCGStab inside FP64 refinement.
-    mixed = None
-    if not args.no_mixed:
-        prm32 = wl.fwd.solve_params(precision=PREC_MIXED, inner_tol=1e-3)
-        r32, x32, d32 = timed(prm32, zero_x, "mixed")
-        mixed = {"solves_per_s": ns * args.steps / d32, "ms_per_step": 1e3 * d32 / args.steps,
-                 "speedup_vs_fp64": (ns * args.steps / d32) / value,
-                 "converged": all(r["converged"] for r in r32),
-                 "max_rel_dZ_vs_fp64": rel_dz(impedances(model, omegas, x32.cpu().numpy(), nf), z64),
-                 "iterations": [r["iterations"] for r in r32]}
-        log(f"[bench] mixed: {mixed}")
+    def leg(label, prm, make_x=zero_x):
+        r, x, d = timed(prm, make_x, label)
+        out = {"solves_per_s": ns * args.steps / d, "ms_per_step": 1e3 * d / args.steps,
+               "speedup_vs_fp64": (ns * args.steps / d) / value,
+               "converged": all(q["converged"] for q in r),
+               "max_rel_dZ_vs_fp64": rel_dz(impedances(model, omegas, x.cpu().numpy(), nf), z64),
+               "iterations": [q["iterations"] for q in r]}
+        if prm.use_initial_guess:
+            out["guess_used"] = [int(q["guess_used"]) for q in r]
+            out["guess_residual"] = [round(q["guess_residual"], 4) for q in r]
+        log(f"[bench] {label}: {out}")
+        return out
 
-    # Mixed precision started from the network's e0.
-    learned = None
-    if not args.no_mixed and not args.no_learned and os.path.exists(args.weights):
-        from maple3dmt_amd.learned import Box, CoarseNet, initial_guess
-        wts = np.load(args.weights)
-        box = Box(*wts["box"].tolist())
-        net = CoarseNet(ctx, max_batch=nf, dilations=tuple(int(v) for v in wts["dilations"]))
-        net.set_params(wts["w"], wts["b"])
-        prml = wl.fwd.solve_params(precision=PREC_MIXED, inner_tol=1e-3)
-        prml.use_initial_guess = 2
-        rl, xl, dl = timed(prml, lambda: initial_guess(net, model, wl, box), "mixed+learned")
-        learned = {"solves_per_s": ns * args.steps / dl, "ms_per_step": 1e3 * dl / args.steps,
-                   "speedup_vs_fp64": (ns * args.steps / dl) / value,
-                   "converged": all(r["converged"] for r in rl),
-                   "max_rel_dZ_vs_fp64": rel_dz(impedances(model, omegas, xl.cpu().numpy(), nf), z64),
-                   "guess_used": [int(r["guess_used"]) for r in rl],
-                   "guess_residual": [round(r["guess_residual"], 4) for r in rl],
-                   "iterations": [r["iterations"] for r in rl],
-                   "weights": os.path.relpath(args.weights, ROOT)}
-        log(f"[bench] mixed+learned: {learned}")
-        net.close()
+    def params(precision, interleaved):
+        prm = wl.fwd.solve_params(precision=precision, inner_tol=1e-3 if precision else 0.0)
+        if interleaved:
+            # E_sec by BiCGStab bursts of 100 iterations, each followed by
+            # DivCorr (the interval of the reference's configuration 1).
+            prm.interleaved_forward = 1
+            prm.divcorr_iter_per_dc = args.dc_interval
+            prm.divcorr_outer_max = 400
+        return prm
+
+    legs = {}
+    if not args.no_mixed:
+        # Precision alone: complex64 inner BiCGStab inside FP64 refinement.
+        legs["mixed"] = leg("mixed", params(PREC_MIXED, False))
+        # Interleaved DivCorr alone, FP64.
+        legs["fp64_interleaved"] = leg("fp64+interleaved", params(0, True))
+        # The fast path: mixed precision + interleaved DivCorr.
+        legs["mixed_interleaved"] = leg("mixed+interleaved", params(PREC_MIXED, True))
+        if not args.no_learned and os.path.exists(args.weights):
+            from maple3dmt_amd.learned import Box, CoarseNet, initial_guess
+            wts = np.load(args.weights)
+            box = Box(*wts["box"].tolist())
+            net = CoarseNet(ctx, max_batch=nf, dilations=tuple(int(v) for v in wts["dilations"]))
+            net.set_params(wts["w"], wts["b"])
+            prml = params(PREC_MIXED, True)
+            prml.use_initial_guess = 2
+            legs["mixed_interleaved_learned"] = leg("mixed+interleaved+learned", prml,
+                                                    lambda: initial_guess(net, model, wl, box))
+            legs["mixed_interleaved_learned"]["weights"] = os.path.relpath(args.weights, ROOT)
+            net.close()
 
     # Roofline of the kernel that dominates the timed region: the backward
     # SSOR sweep at the tail width (the low-frequency systems run their
@@ -259,8 +278,7 @@ def main():
         "ssor_apply": {"systems": ns, "avg_ms": ms_ssor, "alg_bytes": bytes_ssor,
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
         "operator": info,
-        "mixed": mixed,
-        "mixed_learned": learned,
+        "variants": legs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

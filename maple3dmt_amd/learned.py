@@ -227,9 +227,13 @@ class Sample:
 def solve_model(model: Model, wl: configs.Workload, params=None):
     """Forward solve of every frequency; returns E_sec per system and results.
     Training targets need far less than the solve tolerance, so by default the
-    mixed-precision solve runs to a relative residual of 1e-6."""
+    mixed-precision solve with interleaved DivCorr runs to a relative residual
+    of 1e-6."""
     if params is None:
         params = wl.fwd.solve_params(precision=PREC_MIXED, inner_tol=1e-3, tol=1e-6)
+        params.interleaved_forward = 1
+        params.divcorr_iter_per_dc = 100
+        params.divcorr_outer_max = 400
     E, Z, res = model.solve_frequencies(wl.freqs, params)
     esec = np.empty_like(E)
     for f, fr in enumerate(wl.freqs):

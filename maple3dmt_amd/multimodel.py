@@ -1,0 +1,98 @@
+"""Misfit gradients of several models (meshes of uneven size) on one GPU.
+
+Each model keeps its own device operator resident and runs the gradient's two
+batched phases (every forward system, then every adjoint system; see
+inversion.py) on the stream of its own context.  A pool of host threads
+drives the models, so the GPU overlaps one model's narrow low-frequency tail
+with the wide phases of others.  ctypes releases the GIL inside every library
+call, and every context, operator and workspace belongs to one thread at a
+time, so the threads share nothing but the GPU.
+
+This replaces the reference's MPI frequency parallelism
+(src/utils/freq_parallel.cpp) for the many-model workload: there, one rank per
+frequency runs the per-frequency loop of InversionFV::compute_gradient_
+(src/inversion/inversion_fv.cpp:230-312) on the CPU.
+"""
+from __future__ import annotations
+
+import threading
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import configs, inversion
+from .native import Context, Model, SolveParams
+
+
+@dataclass
+class ModelJob:
+    """One model of the batch: its workload, observed data and errors."""
+    workload: configs.Workload
+    z_obs: np.ndarray                  # [nf, stations, 4]
+    z_err: np.ndarray                  # [nf, stations, 4]
+    model: Optional[Model] = None      # built by prepare()
+    result: Optional[inversion.GradientResult] = None
+    seconds: float = 0.0
+    worker: int = -1
+
+
+def prepare(jobs: Sequence[ModelJob], contexts: Sequence[Context]) -> None:
+    """Build every model's mesh, operators and device operator (host work,
+    outside any timed region); job i is assigned to context i % len(contexts)."""
+    for i, job in enumerate(jobs):
+        job.worker = i % len(contexts)
+        job.model = configs.build_model(job.workload, contexts[job.worker])
+
+
+def run(jobs: Sequence[ModelJob], n_workers: int, params: Optional[SolveParams] = None,
+        adjoint_params: Optional[SolveParams] = None) -> float:
+    """Gradient of every job; worker w runs the jobs assigned to context w in
+    order.  Returns the wall time in seconds."""
+    errors: List[BaseException] = []
+
+    def work(w: int):
+        try:
+            for job in jobs:
+                if job.worker != w:
+                    continue
+                t = time.perf_counter()
+                job.result = inversion.gradient(job.model, job.workload.freqs, job.z_obs, job.z_err,
+                                                params, adjoint_params)
+                job.seconds = time.perf_counter() - t
+        except BaseException as e:  # re-raised in the caller
+            errors.append(e)
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=work, args=(w,)) for w in range(n_workers)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    wall = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    return wall
+
+
+def systems(jobs: Sequence[ModelJob]) -> int:
+    """Solved systems: two forward and two adjoint per frequency and model."""
+    return sum(4 * len(job.workload.freqs) for job in jobs)
+
+
+def halfspace_data(workload: configs.Workload, rel_err: float = 0.05):
+    """Synthetic observations for benchmarking: the impedance of the
+    workload's background halfspace at every station, Zxy = -Zyx =
+    sqrt(i w mu0 / sigma_bg) (halfspace_1d_test.cpp:124-125), with errors
+    rel_err |Z| on every component."""
+    mu0 = 4e-7 * np.pi
+    ns = len(workload.stations)
+    nf = len(workload.freqs)
+    z = np.zeros((nf, ns, 4), np.complex128)
+    for f, fr in enumerate(workload.freqs):
+        zz = np.sqrt(1j * 2 * np.pi * fr * mu0 / workload.sigma_bg)
+        z[f, :, 1] = zz
+        z[f, :, 2] = -zz
+    err = np.repeat(rel_err * np.abs(z[:, :, 1:2]), 4, axis=2)
+    return z, err

@@ -9,19 +9,19 @@ evaluate what it gains over the zero initial guess.
 Training models are seeded random 3-D models (three layers plus one to four
 blocks, configs.random3d) on their own octree meshes; each contributes its 8
 frequencies.  Targets are the solver's own E_sec = E - E_pri (mixed-precision
-solve to a relative residual of 1e-6; models with an unconverged system are
-skipped) deposited on the network's voxel grid and scaled to unit RMS per
-polarisation: the network predicts the shape, the solve's least-squares
-scaling of the guess restores the amplitude.
+solve with interleaved DivCorr to a relative residual of 1e-6; models with an
+unconverged system are skipped), deposited on the network's voxel grid and
+scaled to unit RMS per polarisation: the network predicts the shape, the
+solve's least-squares scaling of the guess restores the amplitude.
 
 Evaluation solves held-out models (seeds disjoint from training) at the same
-tolerance four ways: plain FP64 from zero, FP64 from the network's e0, mixed
-precision from zero and mixed from e0.  The e0 start is the least-squares
-scaled guess (use_initial_guess = 2): kept per system only where it lowers the
-residual by half, else the zero start.  Reported per model: BiCGStab
-iterations per system, solve wall time (the e0 variants include the network
-inference and prolongation) and the largest |dZ|/|Z| against the FP64
-zero-start solve.
+tolerance: the plain FP64 method from zero (the reference), and the fast path
+(mixed precision + interleaved DivCorr) from zero and from the network's e0.
+The e0 start is the least-squares scaled guess (use_initial_guess = 2), kept
+per system only where it at least halves the residual, else the zero start.
+Reported per model: BiCGStab iterations per system, solve wall time (the e0
+variant includes the network inference and prolongation), the scaled guess's
+residual and the largest |dZ|/|Z| against the reference solve.
 """
 from __future__ import annotations
 
@@ -50,7 +50,8 @@ def cmd_train(a):
     import torch  # noqa: F401  (device memory for the training batches)
     ctx = Context(0)
     box = Box()
-    net = CoarseNet(ctx, max_batch=max(a.batch, 8), seed=a.seed)
+    dil = tuple(int(v) for v in a.dilations.split(","))
+    net = CoarseNet(ctx, max_batch=max(a.batch, 8), dilations=dil, seed=a.seed)
     t0 = time.time()
     samples, its, edges, skipped = [], 0, [], 0
     for k in range(a.models):
@@ -115,9 +116,13 @@ def cmd_eval(a):
         d_aux = torch.from_numpy(aux).to(dev)
         row = {"seed": seed, "edges": m.n}
         zref = None
-        for prec_name, prec in (("fp64", 0), ("mixed", PREC_MIXED)):
-            for start in ("zero", "e0"):
+        for prec_name, prec, starts in (("fp64", 0, ("zero",)), ("fast", PREC_MIXED, ("zero", "e0"))):
+            for start in starts:
                 prm = wl.fwd.solve_params(precision=prec, inner_tol=1e-3 if prec else 0.0)
+                if prec:
+                    prm.interleaved_forward = 1
+                    prm.divcorr_iter_per_dc = 100
+                    prm.divcorr_outer_max = 400
                 torch.cuda.synchronize()
                 t = time.perf_counter()
                 if start == "zero":
@@ -142,15 +147,15 @@ def cmd_eval(a):
                             "converged": all(q["converged"] for q in r),
                             "seconds": dt + dt_net, "net_seconds": dt_net,
                             "max_rel_dZ": float(np.max(np.abs(z[off] - zref[off]) / np.abs(zref[off])))}
-        log(f"[eval] seed {seed}: {m.n} edges; iterations fp64 {sum(row['fp64_zero']['iterations'])} -> "
-            f"{sum(row['fp64_e0']['iterations'])}, mixed {sum(row['mixed_zero']['iterations'])} -> "
-            f"{sum(row['mixed_e0']['iterations'])}; seconds fp64 {row['fp64_zero']['seconds']:.2f} -> "
-            f"{row['fp64_e0']['seconds']:.2f}, mixed {row['mixed_zero']['seconds']:.2f} -> "
-            f"{row['mixed_e0']['seconds']:.2f}")
+        log(f"[eval] seed {seed}: {m.n} edges; iterations fp64 {sum(row['fp64_zero']['iterations'])}, "
+            f"fast {sum(row['fast_zero']['iterations'])} -> e0 {sum(row['fast_e0']['iterations'])}; "
+            f"seconds fp64 {row['fp64_zero']['seconds']:.2f}, fast {row['fast_zero']['seconds']:.2f} -> "
+            f"e0 {row['fast_e0']['seconds']:.2f}; guess residual "
+            f"{np.round(row['fast_e0']['guess_residual'], 3).tolist()}")
         rows.append(row)
         m.close()
     summ = {}
-    for key in ("fp64_zero", "fp64_e0", "mixed_zero", "mixed_e0"):
+    for key in ("fp64_zero", "fast_zero", "fast_e0"):
         summ[key] = {"iterations": int(sum(sum(r[key]["iterations"]) for r in rows)),
                      "seconds": float(sum(r[key]["seconds"] for r in rows)),
                      "solves_per_s": float(16 * len(rows) / sum(r[key]["seconds"] for r in rows)),
@@ -178,6 +183,7 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--lr", type=float, default=5e-4)
     t.add_argument("--seed", type=int, default=1)
+    t.add_argument("--dilations", default="1,2,4,1", help="dilation of each of the four conv layers")
     t.add_argument("--out", default=os.path.join(DATA, "e0_net.npz"))
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=os.path.join(DATA, "e0_net.npz"))
