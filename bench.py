@@ -40,6 +40,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 WEIGHTS = os.path.join(ROOT, "maple3dmt_amd", "data", "e0_net.npz")
+# HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
+# calibrated on a 1 GiB device copy (see the summary beside it).
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (None when it holds no entry for the kernel)."""
+    try:
+        with open(TRAFFIC) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else float(k["traffic_bytes"])
 
 
 def parse():
@@ -272,9 +286,11 @@ def main():
         "roofline": {"bound": "hbm",
                      "kernel": f"k_sweep_bwd<{tw},double> (backward SSOR sweep, one colour per launch)",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None, "alg_bytes_per_launch": bytes_sw, "avg_launch_ms": ms_sw},
+                     "traffic": measured_traffic(f"k_sweep_bwd<{tw}, double>"),
+                     "alg_bytes_per_launch": bytes_sw, "avg_launch_ms": ms_sw},
         "spmv": {"kernel": f"k_spmv<{ns},double>", "avg_launch_ms": ms_spmv, "alg_bytes": bytes_spmv,
-                 "achieved_GBs": ach_spmv, "TBs": ach_spmv / 1e3},
+                 "achieved_GBs": ach_spmv, "TBs": ach_spmv / 1e3,
+                 "traffic": measured_traffic(f"k_spmv<{ns}, double>")},
         "ssor_apply": {"systems": ns, "avg_ms": ms_ssor, "alg_bytes": bytes_ssor,
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
         "operator": info,
