@@ -23,6 +23,8 @@ against it; and
                  sweep at the tail batch width), timed with HIP events on the
                  library's stream, against HBM peak
   spmv           the batched SpMV of all 16 systems (the metric's SpMV TB/s)
+  multi_model    misfit gradients (forward + adjoint systems) of several models
+                 on meshes of uneven size, sequential and concurrent
   cpu_baseline   the CPU oracle timed on a bounded sample of the same workload
 """
 from __future__ import annotations
@@ -72,6 +74,8 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--models", type=int, default=4,
+                    help="models of the multi-model gradient leg (0 skips it)")
     ap.add_argument("--dc-interval", type=int, default=100,
                     help="BiCGStab iterations between DivCorr steps in the interleaved variants")
     ap.add_argument("--tail-width", type=int, default=2,
@@ -261,6 +265,10 @@ def main():
     log(f"[bench] sweep_bwd x{tw} {ms_sw * 1e3:.1f} us/launch, {ach:.0f} GB/s; spmv x{ns} {ms_spmv:.3f} ms, "
         f"{ach_spmv:.0f} GB/s; ssor x{ns} {ms_ssor:.3f} ms; {info}")
 
+    models = None
+    if args.models > 0:
+        models = multi_model_leg(args, model, ctx)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(model, wl, args.cpu_budget, iters)
@@ -295,11 +303,62 @@ def main():
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
         "operator": info,
         "variants": legs,
+        "multi_model": models,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     model.close()
     ctx.close()
+
+
+def multi_model_leg(args, bench_model, ctx):
+    """Misfit gradients of several seeded random models on meshes of uneven
+    size (octree levels 5-7), each with 16 frequencies x 2 polarisations of
+    forward and adjoint systems, by the fast path (mixed precision +
+    interleaved DivCorr): one after another, then concurrently on one stream
+    per model (multimodel.py).  Reports systems/s of both."""
+    from maple3dmt_amd import PREC_MIXED, Context, configs, multimodel
+    levels = [5, 6, 7, 6, 5, 7, 6, 5][:args.models]
+
+    def jobs():
+        out = []
+        for i, lev in enumerate(levels):
+            wl = configs.random3d(100 + i, max_level=lev, n_freq=16, half_width=100e3,
+                                  station_radius=20e3, maxiter=args.maxiter)
+            z, err = multimodel.halfspace_data(wl)
+            out.append(multimodel.ModelJob(wl, z, err))
+        return out
+
+    prm = configs.ForwardParams().solve_params(precision=PREC_MIXED, inner_tol=1e-3)
+    prm.interleaved_forward = 1
+    prm.divcorr_iter_per_dc = args.dc_interval
+    prm.divcorr_outer_max = 400
+    prm.maxiter = args.maxiter
+    contexts = [Context(0) for _ in range(len(levels))]
+    t = time.time()
+    js = jobs()
+    multimodel.prepare(js, contexts)
+    log(f"[bench] multi-model: {len(levels)} models built in {time.time() - t:.1f}s, "
+        f"edges {[j.model.n for j in js]}")
+    multimodel.run(js, 1, prm, prm)                # warm-up: workspaces and graphs
+    t_seq = multimodel.run(js, 1, prm, prm)
+    phi_seq = [j.result.phi for j in js]
+    t_par = multimodel.run(js, len(levels), prm, prm)
+    n_sys = multimodel.systems(js)
+    same = all(a == j.result.phi for a, j in zip(phi_seq, js))
+    seq = js
+    out = {"models": len(levels), "levels": levels, "edges": [j.model.n for j in seq],
+           "systems": n_sys, "frequencies": 16,
+           "method": "mixed precision + interleaved DivCorr, forward then adjoint per model",
+           "sequential_s": t_seq, "concurrent_s": t_par,
+           "systems_per_s_sequential": n_sys / t_seq, "systems_per_s_concurrent": n_sys / t_par,
+           "concurrent_matches_sequential": same}
+    log(f"[bench] multi-model: {out}")
+    for j in js:
+        j.model.close()
+    for c in contexts:
+        c.close()
+    return out
 
 
 def cpu_baseline(model, wl, budget, gpu_iters):

@@ -102,6 +102,30 @@ typedef struct {
 
 int m3d_operator_set_divcorr(m3d_operator* op, const m3d_divcorr_desc* desc);
 
+/* Cell-side data of compute_sensitivity (forward_solver_fv.cpp:989-1059):
+ * per cell the list of its edges (caller numbering, CSR) and per edge the
+ * weight A_dual * L / n_adj, 0 for edges the sensitivity skips (constrained
+ * with the 1 m tolerance of :1017-1029, or without adjacent cells).  Set
+ * again whenever sigma changes. */
+typedef struct {
+    int nc;                   /* cells */
+    const int* cell_ptr;      /* [nc + 1] */
+    const int* cell_edges;    /* [cell_ptr[nc]] */
+    const double* edge_weight; /* [n] */
+    const double* sigma;      /* [nc] */
+} m3d_sensitivity_desc;
+int m3d_operator_set_sensitivity(m3d_operator* op, const m3d_sensitivity_desc* desc);
+
+/* d_grad[c] = sum over the nsys systems of
+ *   Re(i omega_s sigma_c sum_{e in c} lambda_s[e] E_s[e] w_e)
+ * (compute_sensitivity, :989-1059; lambda is not conjugated, :1008), the
+ * gradient of the misfit in ln sigma_c when (E_s, lambda_s) are the forward
+ * and adjoint fields of every frequency and polarisation.  d_E and d_lambda
+ * are device complex128 [nsys][n] (caller order), d_grad device double [nc]
+ * (overwritten).  Returns after the context stream drains. */
+int m3d_sensitivity_device(m3d_operator* op, int nsys, const double* omega, const double* d_E,
+                           const double* d_lambda, double* d_grad);
+
 /* Mirrors the solver fields of ForwardParamsFV (forward_solver_fv.h:31-69). */
 typedef struct {
     double tol;              /* bicgstab_tol */

@@ -7,6 +7,8 @@
 // Vectors of a batch are system-minor, x[row * NS + sys], so the NS lanes of
 // one row read NS consecutive complex values when they gather a column.
 // Thread t of a block works on row (t / NS), system (t % NS).
+#include <algorithm>
+
 #include "dev_common.h"
 #include "kernels.h"
 
@@ -626,6 +628,44 @@ __global__ void k_scale_guess(int npad, const SysState* __restrict__ st, double2
          t += size_t(gridDim.x) * blockDim.x) {
         x[t] = cmul(st[t % NS].guess_scale, x[t]);
     }
+}
+
+// Misfit gradient in ln sigma, summed over systems (compute_sensitivity,
+// forward_solver_fv.cpp:989-1059): g_c = sum_s Re(i w_s sigma_c a_s) with
+// a_s = sum_{e in c} lambda_s[e] E_s[e] w_e (lambda unconjugated); one thread
+// per cell, fields system-major in caller order.
+__global__ void k_sensitivity(int nc, int n, int nsys, const int* __restrict__ cptr,
+                              const int* __restrict__ cedge, const double* __restrict__ wgt,
+                              const double* __restrict__ sigma, const double* __restrict__ omega,
+                              const double2* __restrict__ E, const double2* __restrict__ lam,
+                              double* __restrict__ g) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int s = 0; s < nsys; ++s) {
+            const double2* Es = E + size_t(s) * n;
+            const double2* Ls = lam + size_t(s) * n;
+            double2 a = mk(0.0, 0.0);
+            for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
+                const int e = cedge[k];
+                const double w = wgt[e];
+                if (w == 0.0) continue;
+                const double2 p = cmul(Ls[e], Es[e]);
+                a.x += p.x * w;
+                a.y += p.y * w;
+            }
+            // Re(i w sigma a) = -w sigma Im(a)
+            acc += -omega[s] * sigma[c] * a.y;
+        }
+        g[c] = acc;
+    }
+}
+
+void launch_sensitivity(hipStream_t s, int nc, int n, int nsys, const int* cptr, const int* cedge,
+                        const double* wgt, const double* sigma, const double* omega, const double2* E,
+                        const double2* lam, double* g) {
+    const int blocks = std::max(1, std::min((nc + kBlock - 1) / kBlock, 4096));
+    hipLaunchKernelGGL(k_sensitivity, dim3(blocks), dim3(kBlock), 0, s, nc, n, nsys, cptr, cedge, wgt,
+                       sigma, omega, E, lam, g);
 }
 
 // ---------------------------------------------------------------------------

@@ -222,6 +222,10 @@ struct m3d_operator {
     DBuf<int> nt_ptr, nt_edge, g_col, l_ptr, l_colidx;
     DBuf<double> nt_val, g_val, ms, l_vals, l_diag;
     DBuf<unsigned char> dc_bc_edge, dc_dead, dc_bc_node;
+    // Sensitivity arrays.
+    int nc = 0;
+    DBuf<int> s_cell_ptr, s_cell_edges;
+    DBuf<double> s_weight, s_sigma, s_omega;
     // Workspaces by batch width.
     std::unique_ptr<Work64<1>> w1;
     std::unique_ptr<Work64<2>> w2;
@@ -1317,6 +1321,53 @@ int m3d_operator_set_divcorr(m3d_operator* op, const m3d_divcorr_desc* desc) {
             if (desc->L_colidx[k] < 0 || desc->L_colidx[k] >= desc->nn) throw Fail(M3D_ERR_ARG, "L column out of range");
         HIPCHK(hipSetDevice(op->ctx->device));
         set_divcorr(*op, *desc);
+    });
+}
+
+int m3d_operator_set_sensitivity(m3d_operator* op, const m3d_sensitivity_desc* desc) {
+    return guarded([&] {
+        if (!op || !desc || desc->nc <= 0 || !desc->cell_ptr || !desc->cell_edges || !desc->edge_weight ||
+            !desc->sigma)
+            throw Fail(M3D_ERR_ARG, "incomplete sensitivity description");
+        const int nc = desc->nc;
+        const int total = desc->cell_ptr[nc];
+        if (desc->cell_ptr[0] != 0 || total < 0) throw Fail(M3D_ERR_ARG, "cell_ptr must start at 0");
+        for (int c = 0; c < nc; ++c)
+            if (desc->cell_ptr[c + 1] < desc->cell_ptr[c]) throw Fail(M3D_ERR_ARG, "cell_ptr must not decrease");
+        for (int k = 0; k < total; ++k)
+            if (desc->cell_edges[k] < 0 || desc->cell_edges[k] >= op->n)
+                throw Fail(M3D_ERR_ARG, "cell edge out of range");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        // Host copies stay alive until the stream has drained.
+        const std::vector<int> ptr(desc->cell_ptr, desc->cell_ptr + nc + 1);
+        const std::vector<int> edges(desc->cell_edges, desc->cell_edges + total);
+        const std::vector<double> w(desc->edge_weight, desc->edge_weight + op->n);
+        const std::vector<double> sg(desc->sigma, desc->sigma + nc);
+        op->s_cell_ptr.upload(ptr, s);
+        op->s_cell_edges.upload(edges, s);
+        op->s_weight.upload(w, s);
+        op->s_sigma.upload(sg, s);
+        HIPCHK(hipStreamSynchronize(s));
+        op->nc = nc;
+    });
+}
+
+int m3d_sensitivity_device(m3d_operator* op, int nsys, const double* omega, const double* d_E,
+                           const double* d_lambda, double* d_grad) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        if (op->nc <= 0) throw Fail(M3D_ERR_STATE, "m3d_operator_set_sensitivity has not been called");
+        if (nsys <= 0 || !omega || !d_E || !d_lambda || !d_grad) throw Fail(M3D_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        const std::vector<double> om(omega, omega + nsys);
+        op->s_omega.upload(om, s);
+        launch_sensitivity(s, op->nc, op->n, nsys, op->s_cell_ptr.p, op->s_cell_edges.p, op->s_weight.p,
+                           op->s_sigma.p, op->s_omega.p, reinterpret_cast<const double2*>(d_E),
+                           reinterpret_cast<const double2*>(d_lambda), d_grad);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
     });
 }
 

@@ -80,6 +80,22 @@ void ForwardSolverGPU::set_sigma(const std::vector<double>& sigma) {
     d.L_values = prob_.L_bc.val.data();
     d.L_diag = prob_.L_diag.data();
     check(m3d_operator_set_divcorr(op_, &d));
+    // Sensitivity: cells -> edges, per-edge weights and sigma.
+    const auto& grid = prob_.grid();
+    std::vector<int> cptr(prob_.num_cells() + 1, 0), cedge;
+    for (int c = 0; c < prob_.num_cells(); ++c) {
+        for (const auto& ce : grid.cell_edges[c]) cedge.push_back(ce.edge_id);
+        cptr[c + 1] = static_cast<int>(cedge.size());
+    }
+    std::vector<double> w;
+    prob_.sensitivity_weights(w);
+    m3d_sensitivity_desc sd{};
+    sd.nc = prob_.num_cells();
+    sd.cell_ptr = cptr.data();
+    sd.cell_edges = cedge.data();
+    sd.edge_weight = w.data();
+    sd.sigma = prob_.sigma.data();
+    check(m3d_operator_set_sensitivity(op_, &sd));
 }
 
 void ForwardSolverGPU::map_stations(const data::MTData& observed) {

@@ -389,6 +389,19 @@ void FVProblem::adjoint_rhs(double omega, const std::vector<StationHCache>& cach
 
 // g_c = Re(i omega sum_{e in c, unconstrained} lambda_e E_e sigma_c A_dual L / n_adj)
 // with unconjugated lambda (:989-1059).
+void FVProblem::sensitivity_weights(std::vector<double>& w) const {
+    const int ne = num_edges(), nc = num_cells();
+    w.assign(ne, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        if (sens_constrained[e]) continue;
+        int n_adj = 0;
+        for (int a : grid_.edge_adj[e])
+            if (a >= 0 && a < nc) ++n_adj;
+        if (n_adj == 0) continue;
+        w[e] = ops_.edge_dual_area(e) * grid_.edge_len[e] / n_adj;
+    }
+}
+
 void FVProblem::sensitivity(double omega, const std::vector<cplx>& e_bg,
                             const std::vector<cplx>& lambda, std::vector<double>& sens) const {
     const int nc = num_cells();

@@ -101,6 +101,9 @@ public:
     void adjoint_rhs(double omega, const std::vector<StationHCache>& cache,
                      const std::vector<std::array<cplx, 4>>& wr, std::vector<cplx>& rhs1,
                      std::vector<cplx>& rhs2) const;
+    // Per-edge weight A_dual * L / n_adj of the sensitivity sum (0 for the
+    // edges it skips), the data the device's sensitivity kernel needs.
+    void sensitivity_weights(std::vector<double>& w) const;
     void sensitivity(double omega, const std::vector<cplx>& e_bg, const std::vector<cplx>& lambda,
                      std::vector<double>& sens) const;
     void perturbation_rhs(double omega, const std::vector<cplx>& e_bg,

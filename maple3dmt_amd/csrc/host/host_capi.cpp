@@ -131,6 +131,11 @@ struct m3dh_model {
             }
             return i32(v);
         }
+        if (name == "sens_weight") {
+            std::vector<double> w;
+            P.sensitivity_weights(w);
+            return f64(w);
+        }
         if (name == "cell_edges_ptr" || name == "cell_edges") {
             std::vector<int> ptr(nc + 1, 0), out;
             for (int c = 0; c < nc; ++c) {

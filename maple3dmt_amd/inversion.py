@@ -17,7 +17,7 @@ from typing import Optional
 
 import numpy as np
 
-from .native import Model, SolveParams
+from .native import MODE_FORWARD, MODE_INTERLEAVED, Model, SolveParams
 
 
 @dataclass
@@ -69,3 +69,43 @@ def gradient(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
         g += model.sensitivity(w, E[2 * f], lam[2 * f])
         g += model.sensitivity(w, E[2 * f + 1], lam[2 * f + 1])
     return GradientResult(phi, g, Z, fres, ares)
+
+
+def gradient_device(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
+                    params: Optional[SolveParams] = None,
+                    adjoint_params: Optional[SolveParams] = None) -> GradientResult:
+    """gradient() with the fields resident on the device: the forward batch
+    (FORWARD), the adjoint batch (INTERLEAVED) and the sensitivity sum over
+    every frequency and polarisation (m3d_sensitivity_device) run on the GPU;
+    the host sees E only for the station extraction and builds the station
+    right-hand sides.  torch tensors are the device buffers."""
+    import torch
+    dev = torch.device("cuda:0")
+    freqs = np.asarray(freqs, dtype=np.float64)
+    nf = len(freqs)
+    omegas = np.repeat(2 * np.pi * freqs, 2)
+    fwd_prm = params or model.fwd_params.solve_params()
+    rhs = np.stack([model.rhs_total(model.bc_values(w, p % 2)) for p, w in enumerate(omegas)])
+    aux = np.stack([model.primary_field(w, p % 2) for p, w in enumerate(omegas)])
+    d_rhs = torch.from_numpy(rhs).to(dev)
+    d_aux = torch.from_numpy(aux).to(dev)
+    d_e = torch.zeros_like(d_rhs)
+    torch.cuda.synchronize()
+    fres = model.device.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(),
+                                     d_e.data_ptr(), 2 * nf, fwd_prm)
+    E = d_e.cpu().numpy()
+    Z = np.zeros_like(z_obs)
+    arhs = np.zeros_like(rhs)
+    for f in range(nf):
+        w = omegas[2 * f]
+        Z[f] = model.extract_impedance(w, E[2 * f], E[2 * f + 1])
+        arhs[2 * f], arhs[2 * f + 1] = model.adjoint_rhs(w, weighted_residual(z_obs[f], Z[f], z_err[f]))
+    phi = misfit(z_obs, Z, z_err)
+    d_arhs = torch.from_numpy(arhs).to(dev)
+    d_lam = torch.zeros_like(d_arhs)
+    torch.cuda.synchronize()
+    ares = model.device.solve_device(MODE_INTERLEAVED, omegas, d_arhs.data_ptr(), 0, d_lam.data_ptr(),
+                                     2 * nf, adjoint_params or fwd_prm)
+    d_g = torch.empty(model.nc, dtype=torch.float64, device=dev)
+    model.device.sensitivity_device(omegas, d_e.data_ptr(), d_lam.data_ptr(), d_g.data_ptr())
+    return GradientResult(phi, d_g.cpu().numpy(), Z, fres, ares)

@@ -48,14 +48,15 @@ def prepare(jobs: Sequence[ModelJob], contexts: Sequence[Context]) -> None:
 
 def run(jobs: Sequence[ModelJob], n_workers: int, params: Optional[SolveParams] = None,
         adjoint_params: Optional[SolveParams] = None) -> float:
-    """Gradient of every job; worker w runs the jobs assigned to context w in
-    order.  Returns the wall time in seconds."""
+    """Gradient of every job; with one worker all jobs run in order on the
+    calling thread's schedule, else worker w runs the jobs assigned to
+    context w in order.  Returns the wall time in seconds."""
     errors: List[BaseException] = []
 
     def work(w: int):
         try:
             for job in jobs:
-                if job.worker != w:
+                if n_workers > 1 and job.worker != w:
                     continue
                 t = time.perf_counter()
                 job.result = inversion.gradient(job.model, job.workload.freqs, job.z_obs, job.z_err,

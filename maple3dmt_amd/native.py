@@ -122,6 +122,7 @@ DEVICE_SYMBOLS = [
     "m3d_operator_destroy", "m3d_operator_set_mass", "m3d_operator_set_divcorr", "m3d_solve_batch",
     "m3d_solve_batch_device", "m3d_divcorr", "m3d_spmv", "m3d_ssor_apply",
     "m3d_operator_permutation", "m3d_operator_info", "m3d_context_stream", "m3d_bench_kernel",
+    "m3d_operator_set_sensitivity", "m3d_sensitivity_device",
 ]
 HOST_SYMBOLS = [
     "m3dh_last_error", "m3dh_model_create", "m3dh_model_destroy", "m3dh_get_array",
@@ -167,6 +168,7 @@ def lib() -> C.CDLL:
     L.m3d_operator_permutation.argtypes = [vp, vp, vp]
     L.m3d_operator_info.argtypes = [vp, p(i), p(i), p(C.c_longlong), p(C.c_longlong)]
     L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d)]
+    L.m3d_sensitivity_device.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3dh_model_create.argtypes = [p(MeshParamsC), vp, i, vp, i, p(ForwardParamsC), vp, p(vp)]
     L.m3dh_model_destroy.argtypes = [vp]
     L.m3dh_get_array.argtypes = [vp, C.c_char_p, p(i), p(C.c_longlong), p(vp)]
@@ -339,6 +341,13 @@ class DeviceOperator:
         _check_dev(lib().m3d_bench_kernel(self.handle, kind, om.shape[0], _ptr(om), reps,
                                           C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def sensitivity_device(self, omegas, d_e: int, d_lambda: int, d_grad: int) -> None:
+        """d_grad[c] = sum over systems of Re(i w sigma_c sum_e lambda E w_e),
+        all device pointers (E, lambda complex128 [nsys][n], grad float64 [nc])."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        _check_dev(lib().m3d_sensitivity_device(self.handle, om.shape[0], _ptr(om), C.c_void_p(d_e),
+                                                C.c_void_p(d_lambda), C.c_void_p(d_grad)))
 
     def spmv(self, omegas, x) -> np.ndarray:
         x = _as_c128(np.atleast_2d(x))
