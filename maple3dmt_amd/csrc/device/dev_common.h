@@ -87,10 +87,7 @@ struct SysState {
 template <typename V>
 struct DevOp {
     int n;                 // padded rows
-    const int* l_off;      // per slice: first element of the lower part
-    const int* l_w;        // per slice: lower width
-    const int* u_off;
-    const int* u_w;
+    const int4* slc;       // per slice: {lower offset, lower width, upper offset, upper width}
     const V* l_val;
     const int* l_col;
     const V* u_val;

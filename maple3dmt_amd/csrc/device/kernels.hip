@@ -20,17 +20,45 @@ namespace {
 template <int NS>
 __device__ inline int rows_per_block() { return kBlock / NS; }
 
+// Entries k = k0, k0 + STEP, ... < w of one row of a SELL block, added into
+// (ar, ai) in increasing k.  The column indices and values of U entries are
+// loaded before any of their x entries, so a row waits for two memory round
+// trips per U entries instead of two per entry.  Every lane of a wavefront
+// works on rows of one slice, so off and w are read from the first lane into
+// scalar registers and the loop runs without divergence; entries past w
+// repeat entry w - 1 with value 0 (a valid address, added as +0).
+template <int U, int STEP, typename V>
+__device__ inline void gather_sell(const V* __restrict__ val, const int* __restrict__ col, int off,
+                                   int w, int lane, int k0, const cvec<V>* __restrict__ x, int NS,
+                                   int sys, V& ar, V& ai) {
+    off = __builtin_amdgcn_readfirstlane(off);
+    w = __builtin_amdgcn_readfirstlane(w);
+    for (int k = k0; k < w; k += U * STEP) {
+        int c[U];
+        V a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + u * STEP;
+            const int idx = off + min(kk, w - 1) * kSlice + lane;
+            c[u] = col[idx];
+            a[u] = kk < w ? val[idx] : V(0);
+        }
+        cvec<V> xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[size_t(c[u]) * NS + sys];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ar += a[u] * xv[u].x;
+            ai += a[u] * xv[u].y;
+        }
+    }
+}
+
 template <typename V>
 __device__ inline void row_gather(const V* __restrict__ val, const int* __restrict__ col, int off,
                                   int w, int lane, const cvec<V>* __restrict__ x, int NS, int sys,
                                   V& ar, V& ai) {
-    for (int k = 0; k < w; ++k) {
-        const int idx = off + k * kSlice + lane;
-        const V a = val[idx];
-        const cvec<V> xv = x[size_t(col[idx]) * NS + sys];
-        ar += a * xv.x;
-        ai += a * xv.y;
-    }
+    gather_sell<8, 1, V>(val, col, off, w, lane, 0, x, NS, sys, ar, ai);
 }
 
 // Sweeps of narrow batches split each row over SUB lanes (entries k = sub,
@@ -45,13 +73,7 @@ template <int SUB, typename V>
 __device__ inline void row_gather_split(const V* __restrict__ val, const int* __restrict__ col, int off,
                                         int w, int lane, const cvec<V>* __restrict__ x, int NS, int sys,
                                         int sub, V& ar, V& ai) {
-    for (int k = sub; k < w; k += SUB) {
-        const int idx = off + k * kSlice + lane;
-        const V a = val[idx];
-        const cvec<V> xv = x[size_t(col[idx]) * NS + sys];
-        ar += a * xv.x;
-        ai += a * xv.y;
-    }
+    gather_sell<(SUB == 1 ? 8 : 4), SUB, V>(val, col, off, w, lane, sub, x, NS, sys, ar, ai);
     if (SUB >= 2) {
         ar += __shfl_xor(ar, NS);
         ai += __shfl_xor(ai, NS);
@@ -67,10 +89,11 @@ __device__ inline void row_gather_split(const V* __restrict__ val, const int* __
 template <typename V>
 __device__ inline cvec<V> apply_row(const DevOp<V>& op, int row, int NS, int sys, V w,
                                     const cvec<V>* __restrict__ x) {
-    const int sl = row >> 6, lane = row & 63;
+    const int4 sc = op.slc[row >> 6];
+    const int lane = row & 63;
     V ar = 0, ai = 0;
-    row_gather(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, x, NS, sys, ar, ai);
-    row_gather(op.u_val, op.u_col, op.u_off[sl], op.u_w[sl], lane, x, NS, sys, ar, ai);
+    row_gather(op.l_val, op.l_col, sc.x, sc.y, lane, x, NS, sys, ar, ai);
+    row_gather(op.u_val, op.u_col, sc.z, sc.w, lane, x, NS, sys, ar, ai);
     const V dr = op.diag_s[row], di = w * op.me[row];
     const cvec<V> xi = x[size_t(row) * NS + sys];
     cvec<V> y;
@@ -85,9 +108,10 @@ __device__ inline cvec<V> apply_row(const DevOp<V>& op, int row, int NS, int sys
 template <typename V>
 __device__ inline cvec<V> apply_row_eis(const DevOp<V>& op, int row, int NS, int sys, V w,
                                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ y) {
-    const int sl = row >> 6, lane = row & 63;
+    const int4 sc = op.slc[row >> 6];
+    const int lane = row & 63;
     V ar = 0, ai = 0;
-    row_gather(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, z, NS, sys, ar, ai);
+    row_gather(op.l_val, op.l_col, sc.x, sc.y, lane, z, NS, sys, ar, ai);
     const V dr = op.diag_s[row], di = w * op.me[row];
     const cvec<V> yi = y[size_t(row) * NS + sys];
     cvec<V> out;
@@ -263,9 +287,10 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
             }
             if (sub == 0) p[i] = src;
         }
-        const int sl = row >> 6, lane = row & 63;
+        const int4 sc = op.slc[row >> 6];
+        const int lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather_split<SUB>(op.l_val, op.l_col, op.l_off[sl], op.l_w[sl], lane, y, NS, sys, sub, ar, ai);
+        row_gather_split<SUB>(op.l_val, op.l_col, sc.x, sc.y, lane, y, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(op.diag_s[row], w * op.me[row], dr, di);
         const V er = src.x - ar, ei = src.y - ai;
@@ -294,9 +319,10 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
         const int row = r0 + rb * rpb + tid / (NS * SUB);
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
-        const int sl = row >> 6, lane = row & 63;
+        const int4 sc = op.slc[row >> 6];
+        const int lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather_split<SUB>(op.u_val, op.u_col, op.u_off[sl], op.u_w[sl], lane, z, NS, sys, sub, ar, ai);
+        row_gather_split<SUB>(op.u_val, op.u_col, sc.z, sc.w, lane, z, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(op.diag_s[row], w * op.me[row], dr, di);
         const cvec<V> yi = y[i];
@@ -919,12 +945,33 @@ __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restr
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
         const size_t i = size_t(v) * NS + sys;
+        // Entries in increasing k, indices and values of 8 entries loaded
+        // before their p entries.
         double2 a = mk(0.0, 0.0);
-        for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
-            const double l = lval[k];
-            const double2 pv = p[size_t(lcol[k]) * NS + sys];
-            a.x += l * pv.x;
-            a.y += l * pv.y;
+        const int k1 = lptr[v + 1];
+        for (int k = lptr[v]; k < k1; k += 8) {
+            int c[8];
+            double l[8];
+            double2 pv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                c[u] = 0;
+                l[u] = 0.0;
+                if (k + u < k1) {
+                    c[u] = lcol[k + u];
+                    l[u] = lval[k + u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < k1) pv[u] = p[size_t(c[u]) * NS + sys];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (k + u < k1) {
+                    a.x += l[u] * pv[u].x;
+                    a.y += l[u] * pv[u].y;
+                }
+            }
         }
         ap[i] = a;
         const double2 d = cconjmul(p[i], a);

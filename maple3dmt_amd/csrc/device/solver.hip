@@ -210,7 +210,8 @@ struct m3d_operator {
     std::vector<int> perm, iperm, colour_start;
     std::vector<int> hang_level_start;  // offsets into the hanging triples per level
     // Operator arrays.
-    DBuf<int> d_perm, l_off, l_w, u_off, u_w, l_col, u_col, hang;
+    DBuf<int> d_perm, l_col, u_col, hang;
+    DBuf<int4> slc;
     DBuf<double> l_val, u_val, diag_s, me;
     DBuf<float> l_val32, u_val32, diag_s32, me32;
     DBuf<unsigned char> bc, constrained;
@@ -242,7 +243,7 @@ template <>
 DevOp<double> m3d_operator::view<double>() const {
     DevOp<double> o;
     o.n = npad;
-    o.l_off = l_off.p; o.l_w = l_w.p; o.u_off = u_off.p; o.u_w = u_w.p;
+    o.slc = slc.p;
     o.l_val = l_val.p; o.l_col = l_col.p; o.u_val = u_val.p; o.u_col = u_col.p;
     o.diag_s = diag_s.p; o.me = me.p;
     return o;
@@ -251,7 +252,7 @@ template <>
 DevOp<float> m3d_operator::view<float>() const {
     DevOp<float> o;
     o.n = npad;
-    o.l_off = l_off.p; o.l_w = l_w.p; o.u_off = u_off.p; o.u_w = u_w.p;
+    o.slc = slc.p;
     o.l_val = l_val32.p; o.l_col = l_col.p; o.u_val = u_val32.p; o.u_col = u_col.p;
     o.diag_s = diag_s32.p; o.me = me32.p;
     return o;
@@ -1117,7 +1118,9 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
         cons[r] = d.constrained[e];
     }
     op.d_perm.upload(op.perm, s);
-    op.l_off.upload(loff, s); op.l_w.upload(lw, s); op.u_off.upload(uoff, s); op.u_w.upload(uw, s);
+    std::vector<int4> slc(nsl);
+    for (int sl = 0; sl < nsl; ++sl) slc[sl] = make_int4(loff[sl], lw[sl], uoff[sl], uw[sl]);
+    op.slc.upload(slc, s);
     op.l_val.upload(lval, s); op.u_val.upload(uval, s); op.l_col.upload(lcol, s); op.u_col.upload(ucol, s);
     op.diag_s.upload(ds, s);
     op.bc.upload(bc, s);
