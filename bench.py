@@ -19,6 +19,8 @@ under "variants", the same batch solved
                              region), when its weights exist
 each with its speed-up over the plain FP64 batch and its largest |dZ|/|Z|
 against it; and
+  launch_shapes  the autotuner's launch shape per batch width and precision
+                 (tuned on the bench mesh before any timed region)
   roofline       the kernel that dominates the timed region (the backward SSOR
                  sweep at the tail batch width), timed with HIP events on the
                  library's stream, against HBM peak
@@ -86,6 +88,9 @@ def parse():
                          "kernels (SpMV, SSOR, BiCGStab iterations, backward sweep) and four "
                          "1 GiB device copies (counter calibration)")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of tuning them on the bench "
+                         "mesh before the timed region")
     return ap.parse_args()
 
 
@@ -160,6 +165,15 @@ def main():
     nf = len(wl.freqs)
     ns = 2 * nf
     omegas = np.repeat(2 * np.pi * wl.freqs, 2)
+    shapes = None
+    if not args.no_autotune:
+        # Launch shapes per batch width and precision, timed on this mesh
+        # (outside every timed region).
+        t1 = time.time()
+        tuned = op.autotune(np.resize(omegas, 32), widths=(1, 2, 4, 8, 16), reps=16)
+        shapes = {f"{'c64' if p else 'fp64'}_x{w}": {"ms_per_iteration": round(ms, 4), **sh}
+                  for (p, w), (ms, sh) in tuned.items()}
+        log(f"[bench] autotune {time.time() - t1:.1f}s: {shapes}")
     rhs = np.zeros((ns, n), np.complex128)
     aux = np.zeros((ns, n), np.complex128)
     for f, fr in enumerate(wl.freqs):
@@ -222,14 +236,11 @@ def main():
         return out
 
     def params(precision, interleaved):
-        prm = wl.fwd.solve_params(precision=precision, inner_tol=1e-3 if precision else 0.0)
         if interleaved:
-            # E_sec by BiCGStab bursts of 100 iterations, each followed by
-            # DivCorr (the interval of the reference's configuration 1).
-            prm.interleaved_forward = 1
-            prm.divcorr_iter_per_dc = args.dc_interval
-            prm.divcorr_outer_max = 400
-        return prm
+            # E_sec by BiCGStab bursts of 100 iterations (the interval of the
+            # reference's configuration 1), each followed by DivCorr.
+            return wl.fwd.fast_params(precision=precision, dc_interval=args.dc_interval)
+        return wl.fwd.solve_params(precision=precision, inner_tol=1e-3 if precision else 0.0)
 
     legs = {}
     if not args.no_mixed:
@@ -256,6 +267,8 @@ def main():
     # SSOR sweep at the tail width (the low-frequency systems run their
     # thousands of iterations there); and the 16-system batched SpMV.
     tw = max(1, min(args.tail_width, ns))
+    sub = op.get_shape(0, 1 << (tw - 1).bit_length())["sweep_sub"]
+    sweep_name = f"k_sweep_bwd<{tw}, double, {sub}>"
     ms_sw, bytes_sw = op.bench_kernel(4, omegas[:tw], reps=args.reps)
     ach = bytes_sw / (ms_sw * 1e-3) / 1e9
     ms_spmv, bytes_spmv = op.bench_kernel(0, omegas, reps=args.reps)
@@ -292,9 +305,9 @@ def main():
         "converged": conv,
         "iterations": iters,
         "roofline": {"bound": "hbm",
-                     "kernel": f"k_sweep_bwd<{tw},double> (backward SSOR sweep, one colour per launch)",
+                     "kernel": f"{sweep_name} (backward SSOR sweep, one colour per launch)",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"k_sweep_bwd<{tw}, double>"),
+                     "traffic": measured_traffic(sweep_name),
                      "alg_bytes_per_launch": bytes_sw, "avg_launch_ms": ms_sw},
         "spmv": {"kernel": f"k_spmv<{ns},double>", "avg_launch_ms": ms_spmv, "alg_bytes": bytes_spmv,
                  "achieved_GBs": ach_spmv, "TBs": ach_spmv / 1e3,
@@ -302,6 +315,7 @@ def main():
         "ssor_apply": {"systems": ns, "avg_ms": ms_ssor, "alg_bytes": bytes_ssor,
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
         "operator": info,
+        "launch_shapes": shapes,
         "variants": legs,
         "multi_model": models,
         "cpu_baseline": cpu,
@@ -329,10 +343,7 @@ def multi_model_leg(args, bench_model, ctx):
             out.append(multimodel.ModelJob(wl, z, err))
         return out
 
-    prm = configs.ForwardParams().solve_params(precision=PREC_MIXED, inner_tol=1e-3)
-    prm.interleaved_forward = 1
-    prm.divcorr_iter_per_dc = args.dc_interval
-    prm.divcorr_outer_max = 400
+    prm = configs.ForwardParams().fast_params(dc_interval=args.dc_interval)
     prm.maxiter = args.maxiter
     contexts = [Context(0) for _ in range(len(levels))]
     t = time.time()

@@ -148,6 +148,11 @@ typedef struct {
                                 (:778-888) on E_sec, with a full BiCGStab from the
                                 current E_sec for any system that stagnates or
                                 diverges.  The post-processing is the same. */
+    double divcorr_pcg_tol;  /* PCG stop ||r|| < tol * ||div J|| of the DivCorr steps
+                                between interleaved bursts (INTERLEAVED, and FORWARD
+                                with interleaved_forward); 0 = the reference's 1e-10
+                                (forward_solver_fv.cpp:700-702).  The FORWARD
+                                post-hoc correction and m3d_divcorr always use 1e-10. */
 } m3d_solve_params;
 
 /* Per-system outcome; mirrors BiCGStabResult (bicgstab.h:23-28) plus what the
@@ -211,6 +216,37 @@ int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long*
  * backward-sweep launch (the mean over the colours of a backward sweep). */
 int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
                      double* avg_ms, double* alg_bytes);
+
+/* Launch shape of the row kernels at one batch width and precision (no
+ * reference counterpart: the reference has one CPU thread per system).
+ * Only the assignment of rows to GPU threads changes; the arithmetic of each
+ * row does not, except that sweep_sub > 1 adds a row's couplings in sweep_sub
+ * interleaved partial sums.  Reductions over rows are grouped by workgroup,
+ * so a different shape can change the last bits of dot products, and with
+ * them iteration counts where convergence is sensitive to rounding. */
+typedef struct m3d_launch_shape {
+    int sweep_sub; /* lanes per row in the SSOR sweeps: 1, 2, 4 or 8, with width * sub <= 32 */
+    int sweep_cap; /* grid cap of one sweep launch, workgroups: a multiple of 8 in 8..65536 */
+    int row_cap;   /* grid cap of the SpMV kernels that reduce: a multiple of 8 in 8..2048 */
+} m3d_launch_shape;
+
+/* width is a batch width 1, 2, 4, 8, 16 or 32; precision M3D_PREC_FP64
+ * (complex128 kernels) or M3D_PREC_MIXED (the complex64 inner kernels).  The
+ * default shape of a new operator: sweep_sub 1, sweep_cap 4096, row_cap 2048. */
+int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launch_shape* out);
+int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d_launch_shape* shape);
+
+/* Autotuner (the slice and block shape choice of the batched solver): for
+ * every width whose bit is set in widths_mask (bit i = width 1 << i) and both
+ * precisions, times one BiCGStab iteration (as m3d_bench_kernel kinds 2 and 3,
+ * reps iterations per candidate, frequencies omega[0..width-1]) under candidate
+ * shapes by coordinate descent (sweep_sub, then sweep_cap, then row_cap) and
+ * keeps the fastest.  best_ms, if not NULL, receives 2 * 6 values:
+ * [precision * 6 + log2(width)] = ms per iteration of the kept shape, 0 where
+ * not tuned.  Blocking; run it outside timed regions.  Choices stay with the
+ * operator; read them with m3d_operator_get_shape to pin them elsewhere. */
+int m3d_operator_autotune(m3d_operator* op, int widths_mask, const double* omega, int reps,
+                          double* best_ms);
 
 /* Stream the library launches on (hipStream_t), for callers that time or
  * order work around it. */

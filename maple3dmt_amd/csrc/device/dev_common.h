@@ -73,7 +73,7 @@ struct SysState {
     double tol;
     int status, iter, maxiter, info;
     // DivCorr
-    double div_norm, div_norm0, pcg_rnorm;
+    double div_norm, div_norm0, pcg_rnorm, pcg_tol;
     double2 rz, pcg_alpha, pcg_beta;
     int dc_status, pcg_status, pcg_iter, pcg_max;
     // True residual / refinement
@@ -83,10 +83,24 @@ struct SysState {
     int pad0, pad1;
 };
 
+// Launch shape of the row kernels at one batch width (chosen per operator,
+// precision and width; see m3d_operator_autotune).  Only the assignment of
+// rows to threads and workgroups changes, never the arithmetic of a row.
+struct LaunchShape {
+    int sweep_sub;  // lanes per row in the SSOR sweeps (1, 2, 4 or 8; width * sub <= 32)
+    int sweep_cap;  // grid cap of a sweep launch (workgroups, a multiple of kXcds)
+    int row_cap;    // grid cap of the row kernels that reduce (a multiple of kXcds, <= kMaxBlocks)
+};
+constexpr int kWidths = 6;  // batch widths 1, 2, 4, 8, 16, 32
+__host__ __device__ constexpr int width_index(int ns) {
+    return ns <= 1 ? 0 : ns <= 2 ? 1 : ns <= 4 ? 2 : ns <= 8 ? 3 : ns <= 16 ? 4 : 5;
+}
+
 // Device view of one mesh operator (device row order, padded to slices).
 template <typename V>
 struct DevOp {
     int n;                 // padded rows
+    LaunchShape shape[kWidths];
     const int4* slc;       // per slice: {lower offset, lower width, upper offset, upper width}
     const V* l_val;
     const int* l_col;

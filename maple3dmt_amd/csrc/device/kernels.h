@@ -54,6 +54,8 @@ struct Launch64 {
     static void scale_guess(hipStream_t s, const DevOp<double>& op, const double* omega,
                             const double2* b, double2* x, SysState* st, double* part);
     static void zero_systems(hipStream_t s, int npad, const int* flags, double2* x);
+    // dst = src (0 when src is null) on the flagged systems.
+    static void copy_systems(hipStream_t s, int npad, const int* flags, const double2* src, double2* dst);
     static void forward_total(hipStream_t s, int npad, const unsigned char* bc,
                               const double2* epri, const double2* b, double2* x);
     static void add_correction(hipStream_t s, size_t count, const float2* d, double2* x);
@@ -66,7 +68,7 @@ struct Launch64 {
     static void phi0(hipStream_t s, int nn, const int* nt_ptr, const int* nt_edge,
                      const double* nt_val, const unsigned char* dead, const double* omega,
                      const double2* b, double2* phi0);
-    static void dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, const int* nt_ptr,
+    static void dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, double pcg_tol, const int* nt_ptr,
                         const int* nt_edge, const double* nt_val, const double* ms,
                         const unsigned char* bc_node, const double* ldiag, const double2* E,
                         const double2* phi0, double2* r, double2* z, double2* p, double2* phi,

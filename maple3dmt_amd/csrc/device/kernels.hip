@@ -61,26 +61,22 @@ __device__ inline void row_gather(const V* __restrict__ val, const int* __restri
     gather_sell<8, 1, V>(val, col, off, w, lane, 0, x, NS, sys, ar, ai);
 }
 
-// Sweeps of narrow batches split each row over SUB lanes (entries k = sub,
-// sub + SUB, ...; lanes of one row are NS apart), so that one colour of ~1e5
-// rows still puts enough wavefronts on the GPU to hide gather latency.
-template <int NS>
-__host__ __device__ constexpr int sweep_sub() { return NS >= 4 ? 1 : 4 / NS; }
-template <int NS>
-__host__ __device__ constexpr int sweep_rpb() { return kBlock / (NS * sweep_sub<NS>()); }
+// Sweeps may split each row over SUB lanes (entries k = sub, sub + SUB, ...;
+// lanes of one row are NS apart), so that one colour of ~1e5 rows still puts
+// enough wavefronts on the GPU to hide gather latency at narrow batch widths.
+// SUB is part of the operator's launch shape.
+template <int NS, int SUB>
+__host__ __device__ constexpr int sweep_rpb() { return kBlock / (NS * SUB); }
 
 template <int SUB, typename V>
 __device__ inline void row_gather_split(const V* __restrict__ val, const int* __restrict__ col, int off,
                                         int w, int lane, const cvec<V>* __restrict__ x, int NS, int sys,
                                         int sub, V& ar, V& ai) {
-    gather_sell<(SUB == 1 ? 8 : 4), SUB, V>(val, col, off, w, lane, sub, x, NS, sys, ar, ai);
-    if (SUB >= 2) {
-        ar += __shfl_xor(ar, NS);
-        ai += __shfl_xor(ai, NS);
-    }
-    if (SUB >= 4) {
-        ar += __shfl_xor(ar, 2 * NS);
-        ai += __shfl_xor(ai, 2 * NS);
+    gather_sell<(SUB == 1 ? 8 : SUB >= 8 ? 2 : 4), SUB, V>(val, col, off, w, lane, sub, x, NS, sys, ar, ai);
+#pragma unroll
+    for (int m = 1; m < SUB; m <<= 1) {
+        ar += __shfl_xor(ar, NS * m);
+        ai += __shfl_xor(ai, NS * m);
     }
 }
 
@@ -255,14 +251,13 @@ __global__ __launch_bounds__(kFinBlock) void k_bicg_init_fin(SysState* st, const
 //   y_i = (in_i - sum_{j in earlier colours} a_ij y_j) / d_i   (ssor.h:58-65)
 // With PUPD the input is the search direction, updated first:
 //   p = r (first iteration) or p = r + beta (p - omega v)   (bicgstab.cpp:60-67)
-template <int NS, typename V, bool PUPD>
+template <int NS, typename V, int SUB, bool PUPD>
 __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
                             const cvec<V>* __restrict__ in, cvec<V>* __restrict__ p,
                             const cvec<V>* __restrict__ v, cvec<V>* __restrict__ y,
                             const SysState* __restrict__ st) {
-    constexpr int SUB = sweep_sub<NS>();
     const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
-    constexpr int rpb = sweep_rpb<NS>();
+    constexpr int rpb = sweep_rpb<NS, SUB>();
     const SysState& S = st[sys];
     if (S.status != kRunning) return;
     const V w = V(omega[sys]);
@@ -303,13 +298,12 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
 
 // Backward sweep over one colour: z_i = y_i - (sum_{j in later colours} a_ij z_j) / d_i
 // (ssor.h:71-78).
-template <int NS, typename V>
+template <int NS, typename V, int SUB>
 __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
                             const cvec<V>* __restrict__ y, cvec<V>* __restrict__ z,
                             const SysState* __restrict__ st) {
-    constexpr int SUB = sweep_sub<NS>();
     const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
-    constexpr int rpb = sweep_rpb<NS>();
+    constexpr int rpb = sweep_rpb<NS, SUB>();
     if (st[sys].status != kRunning) return;
     const V w = V(omega[sys]);
     const int nrows = r1 - r0;
@@ -741,6 +735,17 @@ __global__ void k_zero_systems(int npad, const int* __restrict__ flags, double2*
     }
 }
 
+// dst = src (0 when src is null) on the systems whose flag is set.
+template <int NS>
+__global__ void k_copy_systems(int npad, const int* __restrict__ flags, const double2* __restrict__ src,
+                               double2* __restrict__ dst) {
+    const size_t total = size_t(npad) * NS;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        if (flags[t % NS]) dst[t] = src ? src[t] : mk(0.0, 0.0);
+    }
+}
+
 // E[coarse] = 0.5 (E[child1] + E[child2]) for one independent level of
 // hanging pairs (forward_solver_fv.cpp:258-261).
 template <int NS, typename V>
@@ -909,7 +914,8 @@ __global__ void k_dc_divj(int nn, const int* __restrict__ nt_ptr, const int* __r
 }
 
 template <int NS>
-__global__ __launch_bounds__(kFinBlock) void k_dc_divj_fin(int dc_iter, double dc_tol, int cg_max, SysState* st, const double* __restrict__ part, int nb) {
+__global__ __launch_bounds__(kFinBlock) void k_dc_divj_fin(int dc_iter, double dc_tol, int cg_max, double pcg_tol,
+                                                            SysState* st, const double* __restrict__ part, int nb) {
     double out[3];
     if (!finish_partials<NS, 3>(part, nb, out)) return;
     const int tid = threadIdx.x;
@@ -928,6 +934,7 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_divj_fin(int dc_iter, double d
     S.pcg_status = kRunning;
     S.pcg_iter = 0;
     S.pcg_max = cg_max;
+    S.pcg_tol = pcg_tol;
 }
 
 // Ap = L p, (p, Ap); alpha = rz / pAp (:681-692).
@@ -1044,7 +1051,7 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_update_fin(SysState* st, const
     if (S.pcg_status != kRunning) return;
     S.pcg_iter += 1;
     S.pcg_rnorm = sqrt(out[0]);
-    if (S.pcg_rnorm < 1e-10 * S.div_norm || S.pcg_iter >= S.pcg_max) {
+    if (S.pcg_rnorm < S.pcg_tol * S.div_norm || S.pcg_iter >= S.pcg_max) {
         S.pcg_status = kConverged;
         return;
     }
@@ -1137,29 +1144,53 @@ void Launch<NS, V>::bicg_init(hipStream_t s, const DevOp<V>& op, const double* o
                        part, x_zero);
     hipLaunchKernelGGL((k_bicg_init_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
+// Sweep launch at lanes-per-row SUB; widths where NS * SUB > 32 fall back to
+// one lane per row.
+template <int NS, int SUB>
+constexpr bool sub_ok() { return NS * SUB <= 32; }
+
+template <int NS, typename V, int SUB>
+void sweep_fwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1, bool pupd,
+                  const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, const SysState* st) {
+    const dim3 g(grid_rows(r1 - r0, sweep_rpb<NS, SUB>(), op.shape[width_index(NS)].sweep_cap));
+    if (pupd)
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, true>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
+                           in, p, v, y, st);
+    else
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, false>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
+                           in, p, v, y, st);
+}
+template <int NS, typename V, int SUB>
+void sweep_bwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
+                  const cvec<V>* y, cvec<V>* z, const SysState* st) {
+    const dim3 g(grid_rows(r1 - r0, sweep_rpb<NS, SUB>(), op.shape[width_index(NS)].sweep_cap));
+    hipLaunchKernelGGL((k_sweep_bwd<NS, V, SUB>), g, dim3(kBlock), 0, s, op, omega, r0, r1, y, z, st);
+}
+
 template <int NS, typename V>
 void Launch<NS, V>::sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
                               int r1, bool pupd, const cvec<V>* in, cvec<V>* p, const cvec<V>* v,
                               cvec<V>* y, const SysState* st) {
-    const dim3 g(grid_rows(r1 - r0, sweep_rpb<NS>(), kStreamCap));
-    if (pupd)
-        hipLaunchKernelGGL((k_sweep_fwd<NS, V, true>), g, dim3(kBlock), 0, s, op, omega, r0, r1, in,
-                           p, v, y, st);
-    else
-        hipLaunchKernelGGL((k_sweep_fwd<NS, V, false>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
-                           in, p, v, y, st);
+    const int sub = op.shape[width_index(NS)].sweep_sub;
+    if (sub == 8 && sub_ok<NS, 8>()) sweep_fwd_at<NS, V, (sub_ok<NS, 8>() ? 8 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
+    else if (sub == 4 && sub_ok<NS, 4>()) sweep_fwd_at<NS, V, (sub_ok<NS, 4>() ? 4 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
+    else if (sub == 2 && sub_ok<NS, 2>()) sweep_fwd_at<NS, V, (sub_ok<NS, 2>() ? 2 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
+    else sweep_fwd_at<NS, V, 1>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
 }
 template <int NS, typename V>
 void Launch<NS, V>::sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
                               int r1, const cvec<V>* y, cvec<V>* z, const SysState* st) {
-    hipLaunchKernelGGL((k_sweep_bwd<NS, V>), dim3(grid_rows(r1 - r0, sweep_rpb<NS>(), kStreamCap)),
-                       dim3(kBlock), 0, s, op, omega, r0, r1, y, z, st);
+    const int sub = op.shape[width_index(NS)].sweep_sub;
+    if (sub == 8 && sub_ok<NS, 8>()) sweep_bwd_at<NS, V, (sub_ok<NS, 8>() ? 8 : 1)>(s, op, omega, r0, r1, y, z, st);
+    else if (sub == 4 && sub_ok<NS, 4>()) sweep_bwd_at<NS, V, (sub_ok<NS, 4>() ? 4 : 1)>(s, op, omega, r0, r1, y, z, st);
+    else if (sub == 2 && sub_ok<NS, 2>()) sweep_bwd_at<NS, V, (sub_ok<NS, 2>() ? 2 : 1)>(s, op, omega, r0, r1, y, z, st);
+    else sweep_bwd_at<NS, V, 1>(s, op, omega, r0, r1, y, z, st);
 }
 template <int NS, typename V>
 void Launch<NS, V>::spmv_v(hipStream_t s, const DevOp<V>& op, const double* omega,
                            const cvec<V>* z, const cvec<V>* ys, const cvec<V>* r0, cvec<V>* v,
                            SysState* st, double* part) {
-    const int g = row_grid<NS>(op.n);
+    const int g = row_grid<NS>(op.n, op.shape[width_index(NS)].row_cap);
     hipLaunchKernelGGL((k_spmv_v<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, r0, v, st, part);
     hipLaunchKernelGGL((k_spmv_v_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
@@ -1174,7 +1205,7 @@ template <int NS, typename V>
 void Launch<NS, V>::spmv_t(hipStream_t s, const DevOp<V>& op, const double* omega,
                            const cvec<V>* z, const cvec<V>* ys, const cvec<V>* sv, cvec<V>* t,
                            SysState* st, double* part) {
-    const int g = row_grid<NS>(op.n);
+    const int g = row_grid<NS>(op.n, op.shape[width_index(NS)].row_cap);
     hipLaunchKernelGGL((k_spmv_t<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, sv, t, st, part);
     hipLaunchKernelGGL((k_spmv_t_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
@@ -1228,6 +1259,11 @@ void Launch64<NS>::zero_systems(hipStream_t s, int npad, const int* flags, doubl
                        dim3(kBlock), 0, s, npad, flags, x);
 }
 template <int NS>
+void Launch64<NS>::copy_systems(hipStream_t s, int npad, const int* flags, const double2* src, double2* dst) {
+    hipLaunchKernelGGL((k_copy_systems<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)), dim3(kBlock), 0, s,
+                       npad, flags, src, dst);
+}
+template <int NS>
 void Launch64<NS>::forward_total(hipStream_t s, int npad, const unsigned char* bc,
                                  const double2* epri, const double2* b, double2* x) {
     hipLaunchKernelGGL((k_forward_total<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
@@ -1264,7 +1300,7 @@ void Launch64<NS>::phi0(hipStream_t s, int nn, const int* nt_ptr, const int* nt_
                        nn, nt_ptr, nt_edge, nt_val, dead, omega, b, phi0);
 }
 template <int NS>
-void Launch64<NS>::dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, const int* nt_ptr,
+void Launch64<NS>::dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, double pcg_tol, const int* nt_ptr,
                            const int* nt_edge, const double* nt_val, const double* ms,
                            const unsigned char* bc_node, const double* ldiag, const double2* E,
                            const double2* phi0, double2* r, double2* z, double2* p, double2* phi,
@@ -1272,7 +1308,7 @@ void Launch64<NS>::dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max
     const int g = row_grid<NS>(nn);
     hipLaunchKernelGGL((k_dc_divj<NS>), dim3(g), dim3(kBlock), 0, s, nn, nt_ptr, nt_edge, nt_val, ms,
                        bc_node, ldiag, E, phi0, r, z, p, phi, st, part);
-    hipLaunchKernelGGL((k_dc_divj_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, it, tol, cg_max, st, part, g);
+    hipLaunchKernelGGL((k_dc_divj_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, it, tol, cg_max, pcg_tol, st, part, g);
 }
 template <int NS>
 void Launch64<NS>::dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,

@@ -227,6 +227,8 @@ struct m3d_operator {
     int nc = 0;
     DBuf<int> s_cell_ptr, s_cell_edges;
     DBuf<double> s_weight, s_sigma, s_omega;
+    // Launch shapes by precision (0 = complex128, 1 = complex64) and width.
+    LaunchShape shape[2][kWidths];
     // Workspaces by batch width.
     std::unique_ptr<Work64<1>> w1;
     std::unique_ptr<Work64<2>> w2;
@@ -243,6 +245,7 @@ template <>
 DevOp<double> m3d_operator::view<double>() const {
     DevOp<double> o;
     o.n = npad;
+    for (int i = 0; i < kWidths; ++i) o.shape[i] = shape[0][i];
     o.slc = slc.p;
     o.l_val = l_val.p; o.l_col = l_col.p; o.u_val = u_val.p; o.u_col = u_col.p;
     o.diag_s = diag_s.p; o.me = me.p;
@@ -252,6 +255,7 @@ template <>
 DevOp<float> m3d_operator::view<float>() const {
     DevOp<float> o;
     o.n = npad;
+    for (int i = 0; i < kWidths; ++i) o.shape[i] = shape[1][i];
     o.slc = slc.p;
     o.l_val = l_val32.p; o.l_col = l_col.p; o.u_val = u_val32.p; o.u_col = u_col.p;
     o.diag_s = diag_s32.p; o.me = me32.p;
@@ -555,9 +559,10 @@ void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, u
     hipStream_t s = op.ctx->stream;
     using L = Launch64<NS>;
     const int cg_max = std::max(500, op.nn);
+    const double pcg_tol = prm.divcorr_pcg_tol > 0 ? prm.divcorr_pcg_tol : 1e-10;
     launch_dc_begin(s, W.st.p, mask);
     for (int it = 0; it < prm.div_corr_iters; ++it) {
-        L::dc_divj(s, op.nn, it, prm.div_corr_tol, cg_max, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p,
+        L::dc_divj(s, op.nn, it, prm.div_corr_tol, cg_max, pcg_tol, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p,
                    op.ms.p, op.dc_bc_node.p, op.l_diag.p, E, phi0, W.dr.p, W.dz.p, W.dp.p, W.dphi.p,
                    W.st.p, W.part.p);
         HIPCHK(hipGetLastError());
@@ -708,6 +713,50 @@ BicgOutcome solve_linear(m3d_operator& op, Work64<NS>& W, const double2* b, doub
     return total;
 }
 
+// Mixed precision only.  Refinement hands a system whose FP64 true residual
+// stops falling to FP64 BiCGStab from the refined x; at low frequency that
+// finish can still diverge or break down, where the same BiCGStab from the
+// original start converges (which trajectory happens depends on rounding).
+// Systems of `mask` that end unconverged with a true residual above tol are
+// therefore solved again by the reference's FP64 BiCGStab from their start:
+// zero, or the caller's system-major guess `caller_x` (packed again).  So the
+// mixed path never ends worse than the FP64 method from the same start.  The
+// repeat's outcome replaces the mixed one; iterations add up.
+template <int NS>
+void restart_failed(m3d_operator& op, Work64<NS>& W, const double2* b, double2* x, unsigned mask,
+                    int nsys, const double2* caller_x, const m3d_solve_params& prm, BicgOutcome& o) {
+    if (prm.precision != M3D_PREC_MIXED) return;
+    unsigned cand = 0;
+    for (int i = 0; i < nsys; ++i)
+        if ((mask & (1u << i)) && o.status[i] != kConverged && o.status[i] != kConvergedEarly) cand |= 1u << i;
+    if (!cand) return;
+    std::vector<double> tr = true_residual<NS>(op, W, x, b, nullptr);
+    unsigned failed = 0;
+    for (int i = 0; i < nsys; ++i)
+        if ((cand & (1u << i)) && !(tr[i] < prm.tol)) failed |= 1u << i;
+    if (!failed) return;
+    hipStream_t s = op.ctx->stream;
+    const double2* x0 = nullptr;
+    if (caller_x) {
+        Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, caller_x, W.rres.p);
+        x0 = W.rres.p;
+    }
+    std::vector<int> flag(NS, 0);
+    for (int i = 0; i < NS; ++i) flag[i] = int((failed >> i) & 1u);
+    HIPCHK(hipMemcpyAsync(W.flags.p, flag.data(), NS * sizeof(int), hipMemcpyHostToDevice, s));
+    Launch64<NS>::copy_systems(s, op.npad, W.flags.p, x0, x);
+    HIPCHK(hipGetLastError());
+    auto& w64 = vec64<NS>(op, W);
+    BicgOutcome r = bicgstab<NS, double>(op, W, w64, b, x, failed, prm.maxiter, prm.tol, x0 == nullptr);
+    for (int i = 0; i < nsys; ++i) {
+        if (!(failed & (1u << i))) continue;
+        o.iterations[i] += r.iterations[i];
+        o.residual[i] = r.residual[i];
+        o.status[i] = r.status[i];
+        o.info[i] = r.info[i];
+    }
+}
+
 // ModEM-style interleaved BiCGStab + DivCorr on A x = b from the current x
 // (solve_with_divcorr_, forward_solver_fv.cpp:778-888): phi0 = G^T b / (i w),
 // an initial DivCorr, then bursts of divcorr_iter_per_dc BiCGStab iterations,
@@ -834,6 +883,7 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
             if (left) {
                 int steps = 1;
                 BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, left, prm.maxiter, prm, false, &steps);
+                restart_failed<NS>(op, W, W.tmp.p, W.x.p, left, nsys, nullptr, prm, o);
                 for (int i = 0; i < nsys; ++i) {
                     if (!(left & (1u << i))) continue;
                     const int before = res[i].iterations;
@@ -845,6 +895,7 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
         } else {
             int steps = 1;
             BicgOutcome o = solve_linear<NS>(op, W, W.tmp.p, W.x.p, mask, prm.maxiter, prm, x_zero, &steps);
+            restart_failed<NS>(op, W, W.tmp.p, W.x.p, mask, nsys, nullptr, prm, o);
             for (int i = 0; i < nsys; ++i) {
                 fill_result(res[i], o.iterations[i], o.residual[i], o.status[i], o.info[i]);
                 res[i].outer = steps;
@@ -852,7 +903,13 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
         }
         interp_all<NS>(op, W.x.p);
         L64::forward_total(s, op.npad, op.bc.p, W.aux.p, W.b.p, W.x.p);
-        if (dc) divcorr<NS>(op, W, W.x.p, nullptr, mask, prm);
+        if (dc) {
+            // The post-hoc correction of the total field is the reference's
+            // (PCG to 1e-10) whatever the bursts used.
+            m3d_solve_params post = prm;
+            post.divcorr_pcg_tol = 0.0;
+            divcorr<NS>(op, W, W.x.p, nullptr, mask, post);
+        }
         for (int i = 0; i < nsys; ++i) {
             res[i].guess_used = used[i];
             res[i].guess_residual = guess_res[i];
@@ -861,6 +918,7 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
         L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
         int steps = 1;
         BicgOutcome o = solve_linear<NS>(op, W, W.b.p, W.x.p, mask, prm.maxiter, prm, false, &steps);
+        restart_failed<NS>(op, W, W.b.p, W.x.p, mask, nsys, d_x, prm, o);
         if (mode == M3D_MODE_INNER) interp_all<NS>(op, W.x.p);
         std::vector<int> zero(NS, 0);
         bool any = false;
@@ -886,6 +944,7 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
             // solve_system_ path (forward_solver_fv.cpp:921-923).
             int steps = 1;
             BicgOutcome o = solve_linear<NS>(op, W, W.b.p, W.x.p, mask, prm.maxiter, prm, true, &steps);
+            restart_failed<NS>(op, W, W.b.p, W.x.p, mask, nsys, nullptr, prm, o);
             std::vector<int> zero(NS, 0);
             bool any = false;
             for (int i = 0; i < nsys; ++i) {
@@ -928,6 +987,8 @@ void check_params(const m3d_solve_params* p) {
         throw Fail(M3D_ERR_ARG, "use_initial_guess must be 0, 1 or 2");
     if (p->interleaved_forward != 0 && p->interleaved_forward != 1)
         throw Fail(M3D_ERR_ARG, "interleaved_forward must be 0 or 1");
+    if (!(p->divcorr_pcg_tol >= 0.0 && p->divcorr_pcg_tol < 1.0))
+        throw Fail(M3D_ERR_ARG, "divcorr_pcg_tol must be in [0, 1)");
 }
 
 template <typename F>
@@ -1011,10 +1072,21 @@ std::vector<int> colour_rows(int n, const int* ptr, const int* col, const unsign
     return c;
 }
 
+LaunchShape default_shape(int width) {
+    LaunchShape sh;
+    (void)width;
+    sh.sweep_sub = 1;  // with grouped gathers one lane per row keeps enough loads in flight
+    sh.sweep_cap = 4096;
+    sh.row_cap = kMaxBlocks;
+    return sh;
+}
+
 void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
     const int n = d.n;
     hipStream_t s = op.ctx->stream;
     op.n = n;
+    for (int p = 0; p < 2; ++p)
+        for (int i = 0; i < kWidths; ++i) op.shape[p][i] = default_shape(1 << i);
     op.h_constrained.assign(d.constrained, d.constrained + n);
     int K = 0;
     std::vector<int> colour = colour_rows(n, d.rowptr, d.colidx, d.constrained, &K);
@@ -1556,21 +1628,41 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
         }
     };
     one();  // warm
+    // BiCGStab iterations run as the solver runs them: kGraphIters per graph
+    // launch (unless graphs are switched off).
+    int per = 1;
+    hipGraphExec_t ex = nullptr;
+    if ((kind == 2 || kind == 3) && use_graphs()) {
+        per = kGraphIters;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < per; ++k) one();
+        HIPCHK(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) throw Fail(M3D_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        HIPCHK(hipGraphLaunch(ex, s));  // warm
+    }
+    const int units = (reps + per - 1) / per;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r) one();
+    for (int r = 0; r < units; ++r) {
+        if (ex) HIPCHK(hipGraphLaunch(ex, s));
+        else one();
+    }
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (ex) (void)hipGraphExecDestroy(ex);
     launch_bicg_begin(s, W.st.p, 0u, 1, 1.0);
     HIPCHK(hipStreamSynchronize(s));
     // Kind 4 reports one launch: the mean over the colours of a sweep.
-    *avg_ms = double(ms) / reps / (kind == 4 ? op.ncolours : 1);
+    *avg_ms = double(ms) / (units * per) / (kind == 4 ? op.ncolours : 1);
     long long nfree = 0;
     for (int e = 0; e < op.n; ++e) nfree += op.h_constrained[e] ? 0 : 1;
     const double m = double(op.nnz), n = double(nfree), k = double(nsys);
@@ -1593,6 +1685,82 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
              : kind == 2 ? iter_b(8, 16)
              : kind == 3 ? iter_b(4, 8)
                          : bwd_b(8, 16);
+}
+
+bool valid_shape(int width, const LaunchShape& sh) {
+    const int sub = sh.sweep_sub;
+    const bool sub_ok = (sub == 1 || sub == 2 || sub == 4 || sub == 8) && width * sub <= 32;
+    return sub_ok && sh.sweep_cap >= kXcds && sh.sweep_cap <= 65536 && sh.sweep_cap % kXcds == 0 &&
+           sh.row_cap >= kXcds && sh.row_cap <= kMaxBlocks && sh.row_cap % kXcds == 0;
+}
+
+int width_slot(int width) {
+    for (int i = 0; i < kWidths; ++i)
+        if (width == (1 << i)) return i;
+    throw Fail(M3D_ERR_ARG, "width must be 1, 2, 4, 8, 16 or 32");
+}
+
+// ms per BiCGStab iteration at batch width 1 << slot in precision p.
+double time_iteration(m3d_operator& op, int p, int slot, const double* omega, int reps) {
+    const int kind = p == 0 ? 2 : 3, k = 1 << slot;
+    double ms = 0.0, bytes = 0.0;
+    switch (k) {
+        case 1: bench_impl<1>(op, kind, k, omega, reps, &ms, &bytes); break;
+        case 2: bench_impl<2>(op, kind, k, omega, reps, &ms, &bytes); break;
+        case 4: bench_impl<4>(op, kind, k, omega, reps, &ms, &bytes); break;
+        case 8: bench_impl<8>(op, kind, k, omega, reps, &ms, &bytes); break;
+        case 16: bench_impl<16>(op, kind, k, omega, reps, &ms, &bytes); break;
+        default: bench_impl<32>(op, kind, k, omega, reps, &ms, &bytes); break;
+    }
+    return ms;
+}
+
+// Coordinate descent over the launch shape of one (precision, width): each
+// candidate is timed twice (the faster counts) and replaces the incumbent
+// only when at least 2 % faster, so timing noise keeps the default.
+double tune_one(m3d_operator& op, int p, int slot, const double* omega, int reps) {
+    const int width = 1 << slot;
+    auto time_shape = [&](const LaunchShape& c) {
+        op.shape[p][slot] = c;
+        clear_graphs(op);
+        const double a = time_iteration(op, p, slot, omega, reps);
+        const double b = time_iteration(op, p, slot, omega, reps);
+        return std::min(a, b);
+    };
+    LaunchShape best = op.shape[p][slot];
+    double tb = time_shape(best);
+    auto consider = [&](LaunchShape c) {
+        if (!valid_shape(width, c)) return;
+        const double t = time_shape(c);
+        if (t < 0.98 * tb) {
+            tb = t;
+            best = c;
+        }
+    };
+    const LaunchShape base = best;
+    for (int sub : {1, 2, 4, 8}) {
+        if (sub == base.sweep_sub) continue;
+        LaunchShape c = best;
+        c.sweep_sub = sub;
+        consider(c);
+    }
+    const LaunchShape after_sub = best;
+    for (int cap : {1024, 2048, 4096, 8192, 16384}) {
+        if (cap == after_sub.sweep_cap) continue;
+        LaunchShape c = best;
+        c.sweep_cap = cap;
+        consider(c);
+    }
+    const LaunchShape after_cap = best;
+    for (int cap : {512, 1024, 2048}) {
+        if (cap == after_cap.row_cap) continue;
+        LaunchShape c = best;
+        c.row_cap = cap;
+        consider(c);
+    }
+    op.shape[p][slot] = best;
+    clear_graphs(op);
+    return tb;
 }
 
 template <typename F>
@@ -1671,6 +1839,52 @@ int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, 
     });
 }
 
+int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launch_shape* out) {
+    return guarded([&] {
+        if (!op || !out) throw Fail(M3D_ERR_ARG, "null argument");
+        if (precision != M3D_PREC_FP64 && precision != M3D_PREC_MIXED) throw Fail(M3D_ERR_ARG, "unknown precision");
+        const LaunchShape& sh = op->shape[precision == M3D_PREC_MIXED ? 1 : 0][width_slot(width)];
+        out->sweep_sub = sh.sweep_sub;
+        out->sweep_cap = sh.sweep_cap;
+        out->row_cap = sh.row_cap;
+    });
+}
+
+int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d_launch_shape* shape) {
+    return guarded([&] {
+        if (!op || !shape) throw Fail(M3D_ERR_ARG, "null argument");
+        if (precision != M3D_PREC_FP64 && precision != M3D_PREC_MIXED) throw Fail(M3D_ERR_ARG, "unknown precision");
+        const int slot = width_slot(width);
+        LaunchShape sh;
+        sh.sweep_sub = shape->sweep_sub;
+        sh.sweep_cap = shape->sweep_cap;
+        sh.row_cap = shape->row_cap;
+        if (!valid_shape(width, sh)) throw Fail(M3D_ERR_ARG, "invalid launch shape for this width");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        HIPCHK(hipStreamSynchronize(op->ctx->stream));
+        op->shape[precision == M3D_PREC_MIXED ? 1 : 0][slot] = sh;
+        clear_graphs(*op);
+    });
+}
+
+int m3d_operator_autotune(m3d_operator* op, int widths_mask, const double* omega, int reps,
+                          double* best_ms) {
+    return guarded([&] {
+        check_ready(op);
+        if (!omega || reps <= 0 || (widths_mask & ~((1 << kWidths) - 1)) != 0)
+            throw Fail(M3D_ERR_ARG, "invalid autotune arguments");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        if (best_ms)
+            for (int i = 0; i < 2 * kWidths; ++i) best_ms[i] = 0.0;
+        for (int p = 0; p < 2; ++p)
+            for (int slot = 0; slot < kWidths; ++slot) {
+                if (!(widths_mask & (1 << slot))) continue;
+                const double t = tune_one(*op, p, slot, omega, reps);
+                if (best_ms) best_ms[p * kWidths + slot] = t;
+            }
+    });
+}
+
 int m3d_divcorr(m3d_operator* op, int nsys, const double* omega, double* E, const double* phi0,
                 const m3d_solve_params* params) {
     return guarded([&] {
@@ -1699,12 +1913,14 @@ int m3d_divcorr(m3d_operator* op, int nsys, const double* omega, double* E, cons
                 HIPCHK(hipMemcpyAsync(st.aux.p, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, s));
             }
             const double2* p0 = phi0 ? st.aux.p : nullptr;
-            if (NS == 1) divcorr_host_impl<1>(*op, k, omega, st.x.p, p0, *params);
-            else if (NS == 2) divcorr_host_impl<2>(*op, k, omega, st.x.p, p0, *params);
-            else if (NS == 4) divcorr_host_impl<4>(*op, k, omega, st.x.p, p0, *params);
-            else if (NS == 8) divcorr_host_impl<8>(*op, k, omega, st.x.p, p0, *params);
-            else if (NS == 16) divcorr_host_impl<16>(*op, k, omega, st.x.p, p0, *params);
-            else divcorr_host_impl<32>(*op, k, omega, st.x.p, p0, *params);
+            m3d_solve_params prm = *params;
+            prm.divcorr_pcg_tol = 0.0;  // divcorr_poisson_ as the reference runs it
+            if (NS == 1) divcorr_host_impl<1>(*op, k, omega, st.x.p, p0, prm);
+            else if (NS == 2) divcorr_host_impl<2>(*op, k, omega, st.x.p, p0, prm);
+            else if (NS == 4) divcorr_host_impl<4>(*op, k, omega, st.x.p, p0, prm);
+            else if (NS == 8) divcorr_host_impl<8>(*op, k, omega, st.x.p, p0, prm);
+            else if (NS == 16) divcorr_host_impl<16>(*op, k, omega, st.x.p, p0, prm);
+            else divcorr_host_impl<32>(*op, k, omega, st.x.p, p0, prm);
             HIPCHK(hipMemcpyAsync(E, st.x.p, bytes, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
         });

@@ -19,7 +19,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import configs
-from .native import PREC_MIXED, Context, Model, NativeError, lib, _ptr
+from .native import Context, Model, NativeError, lib, _ptr
 
 G = 32
 NCH = 12
@@ -230,10 +230,7 @@ def solve_model(model: Model, wl: configs.Workload, params=None):
     mixed-precision solve with interleaved DivCorr runs to a relative residual
     of 1e-6."""
     if params is None:
-        params = wl.fwd.solve_params(precision=PREC_MIXED, inner_tol=1e-3, tol=1e-6)
-        params.interleaved_forward = 1
-        params.divcorr_iter_per_dc = 100
-        params.divcorr_outer_max = 400
+        params = wl.fwd.fast_params(tol=1e-6)
     E, Z, res = model.solve_frequencies(wl.freqs, params)
     esec = np.empty_like(E)
     for f, fr in enumerate(wl.freqs):

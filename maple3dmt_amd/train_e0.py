@@ -118,11 +118,7 @@ def cmd_eval(a):
         zref = None
         for prec_name, prec, starts in (("fp64", 0, ("zero",)), ("fast", PREC_MIXED, ("zero", "e0"))):
             for start in starts:
-                prm = wl.fwd.solve_params(precision=prec, inner_tol=1e-3 if prec else 0.0)
-                if prec:
-                    prm.interleaved_forward = 1
-                    prm.divcorr_iter_per_dc = 100
-                    prm.divcorr_outer_max = 400
+                prm = wl.fwd.fast_params() if prec else wl.fwd.solve_params()
                 torch.cuda.synchronize()
                 t = time.perf_counter()
                 if start == "zero":

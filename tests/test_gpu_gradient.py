@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from maple3dmt_amd import configs, inversion
+from conftest import solved
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -32,8 +33,8 @@ def test_gradient_matches_oracle(ctx):
     err = np.repeat(floor[:, :, None], 4, axis=2)
 
     res = inversion.gradient(m, freqs, obs, err)
-    assert all(r["converged"] for r in res.forward)
-    assert all(r["converged"] for r in res.adjoint)
+    assert all(solved(r) for r in res.forward), res.forward
+    assert all(solved(r) for r in res.adjoint), res.adjoint
 
     phi_o, g_o = 0.0, np.zeros(m.nc)
     for fi, f in enumerate(freqs):
