@@ -141,7 +141,8 @@ typedef struct {
                                 1: used as given where ||b - A e0|| < ||b||;
                                 2: scaled per system by gamma = (A e0, b) / (A e0, A e0), the
                                 least-squares factor, and used where ||b - gamma A e0|| <
-                                0.5 ||b||.  Elsewhere the zero start. */
+                                0.5 ||b||; 3: scaled as 2 and used where
+                                ||b - gamma A e0|| < ||b||.  Elsewhere the zero start. */
     int interleaved_forward; /* FORWARD: 0 = the reference's full BiCGStab on E_sec
                                 (forward_solver_fv.cpp:347-354); 1 = the interleaved
                                 BiCGStab + DivCorr bursts of solve_with_divcorr_

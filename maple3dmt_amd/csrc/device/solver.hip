@@ -845,21 +845,21 @@ void solve_impl(m3d_operator& op, int nsys, int mode, const double* omega, const
         std::vector<int> used(NS, 0);
         std::vector<double> guess_res(NS, 0.0);
         if (prm.use_initial_guess) {
-            // Caller's E_sec guess (mode 2: scaled by the least-squares factor
-            // first), kept only where it lowers the residual below ||b||
-            // (0.5 ||b|| when scaled: a guess that removes less than half the
-            // residual mostly adds rough error that BiCGStab must undo);
+            // Caller's E_sec guess (modes 2 and 3: scaled by the least-squares
+            // factor first), kept only where it lowers the residual below
+            // ||b|| (mode 2: 0.5 ||b||; a guess that removes less than half the
+            // residual often adds rough error that BiCGStab must undo);
             // elsewhere the reference's zero start.
             L::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_x, W.x.p);
             L64::fix_constrained(s, op.npad, op.constrained.p, W.tmp.p, W.x.p);
-            const bool scaled = prm.use_initial_guess == 2;
+            const bool scaled = prm.use_initial_guess >= 2;
             if (scaled) {
                 L64::scale_guess(s, op.view<double>(), W.omega.p, W.tmp.p, W.x.p, W.st.p, W.part.p);
                 L64::fix_constrained(s, op.npad, op.constrained.p, W.tmp.p, W.x.p);
             }
             std::vector<double> tr = true_residual<NS>(op, W, W.x.p, W.tmp.p, nullptr);
             std::vector<int> zero(NS, 0);
-            const double keep = scaled ? 0.5 : 1.0;
+            const double keep = prm.use_initial_guess == 2 ? 0.5 : 1.0;
             for (int i = 0; i < NS; ++i) {
                 used[i] = (i < nsys && tr[i] < keep) ? 1 : 0;
                 zero[i] = used[i] ? 0 : 1;
@@ -983,8 +983,8 @@ void check_params(const m3d_solve_params* p) {
         throw Fail(M3D_ERR_ARG, "invalid DivCorr parameters");
     if (p->precision != M3D_PREC_FP64 && p->precision != M3D_PREC_MIXED)
         throw Fail(M3D_ERR_ARG, "precision must be M3D_PREC_FP64 or M3D_PREC_MIXED");
-    if (p->use_initial_guess < 0 || p->use_initial_guess > 2)
-        throw Fail(M3D_ERR_ARG, "use_initial_guess must be 0, 1 or 2");
+    if (p->use_initial_guess < 0 || p->use_initial_guess > 3)
+        throw Fail(M3D_ERR_ARG, "use_initial_guess must be 0, 1, 2 or 3");
     if (p->interleaved_forward != 0 && p->interleaved_forward != 1)
         throw Fail(M3D_ERR_ARG, "interleaved_forward must be 0 or 1");
     if (!(p->divcorr_pcg_tol >= 0.0 && p->divcorr_pcg_tol < 1.0))
@@ -1628,12 +1628,14 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
         }
     };
     one();  // warm
-    // BiCGStab iterations run as the solver runs them: kGraphIters per graph
-    // launch (unless graphs are switched off).
+    // Launched as the solver launches its work: from a graph (kGraphIters
+    // BiCGStab iterations, or all reps of the other kinds, per graph launch)
+    // unless graphs are switched off, so the event time per launch is close
+    // to the kernels' own duration.
     int per = 1;
     hipGraphExec_t ex = nullptr;
-    if ((kind == 2 || kind == 3) && use_graphs()) {
-        per = kGraphIters;
+    if (use_graphs()) {
+        per = (kind == 2 || kind == 3) ? kGraphIters : reps;
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < per; ++k) one();

@@ -293,7 +293,7 @@ class SolveParams:
     divcorr_iter_per_dc: int = 500
     precision: int = PREC_FP64
     inner_tol: float = 0.0
-    use_initial_guess: int = 0   # 0 zero start, 1 guess as given, 2 least-squares scaled guess
+    use_initial_guess: int = 0   # 0 zero start, 1 guess as given, 2 or 3 least-squares scaled guess (kept below 0.5 or 1 relative residual)
     interleaved_forward: int = 0  # FORWARD E_sec by interleaved BiCGStab + DivCorr bursts
     divcorr_pcg_tol: float = 0.0  # DivCorr PCG relative stop; 0 = the reference's 1e-10
 

@@ -5,6 +5,7 @@ evaluate what it gains over the zero initial guess.
         --out maple3dmt_amd/data/e0_net.npz
     python -m maple3dmt_amd.train_e0 eval --weights maple3dmt_amd/data/e0_net.npz \
         --models 12 --level 6 --report e0_eval.json
+    python -m maple3dmt_amd.train_e0 bound --seed 1 --level 7 --report e0_bound.json
 
 Training models are seeded random 3-D models (three layers plus one to four
 blocks, configs.random3d) on their own octree meshes; each contributes its 8
@@ -22,6 +23,9 @@ per system only where it at least halves the residual, else the zero start.
 Reported per model: BiCGStab iterations per system, solve wall time (the e0
 variant includes the network inference and prolongation), the scaled guess's
 residual and the largest |dZ|/|Z| against the reference solve.
+
+`bound` measures what any coarse-grid e0 can gain: the converged solution
+itself, restricted to the voxel grid and prolonged back, as the start.
 """
 from __future__ import annotations
 
@@ -169,6 +173,61 @@ def cmd_eval(a):
     ctx.close()
 
 
+def cmd_bound(a):
+    """Upper bound of any coarse-grid e0: the converged E_sec itself,
+    deposited on the 32^3 voxel grid (the network's output space) and prolonged
+    back to the edges, used as the start of the plain FP64 method and of the
+    fast path (unscaled and least-squares scaled).  A trained network cannot
+    start closer to the solution than this restriction of the solution."""
+    import torch
+    from .learned import Box, CoarseNet, deposit
+    dev = torch.device("cuda:0")
+    wl = configs.random3d(a.seed, max_level=a.level, half_width=a.half_width,
+                          station_radius=a.station_radius, maxiter=a.maxiter)
+    ctx = Context(0)
+    m = configs.build_model(wl, ctx)
+    nf = len(wl.freqs)
+    omegas = np.repeat(2 * np.pi * wl.freqs, 2)
+    rhs = np.stack([m.rhs_total(m.bc_values(w, p % 2)) for p, w in enumerate(omegas)])
+    aux = np.stack([m.primary_field(w, p % 2) for p, w in enumerate(omegas)])
+    d_rhs = torch.from_numpy(rhs).to(dev)
+    d_aux = torch.from_numpy(aux).to(dev)
+    report = {"workload": wl.name, "edges": m.n, "starts": {}}
+    x = torch.zeros_like(d_rhs)
+    r, dt = _solve(m, omegas, d_rhs, d_aux, x, wl.fwd.fast_params())
+    esec = x.cpu().numpy() - aux
+    net = CoarseNet(ctx, max_batch=nf)
+    exyz = torch.from_numpy(np.stack([m.array("edge_x"), m.array("edge_y"), m.array("edge_z")],
+                                     1).astype(np.float32)).to(dev)
+    edir = torch.from_numpy(m.array("edge_dir")).to(dev)
+    boxes = {"station_box": Box(), "whole_domain": Box(*list(m.mesh_params.domain))}
+    for bname, box in boxes.items():
+        dep = np.stack([deposit(m, box, esec[2 * f], esec[2 * f + 1]) for f in range(nf)])
+        d_dep = torch.from_numpy(dep.astype(np.float32)).to(dev)
+        x0 = torch.empty((2 * nf, m.n), dtype=torch.complex128, device=dev)
+        net.prolong(d_dep.data_ptr(), nf, m.n, exyz.data_ptr(), edir.data_ptr(), box, x0.data_ptr())
+        torch.cuda.synchronize()
+        g = x0.cpu().numpy()
+        entry = {"rel_l2_error": [round(float(np.linalg.norm(g[i] - esec[i]) / np.linalg.norm(esec[i])), 4)
+                                  for i in range(2 * nf)]}
+        for method, mk in (("fp64", wl.fwd.solve_params), ("fast", wl.fwd.fast_params)):
+            for start in (0, 1, 3):
+                prm = mk()
+                prm.use_initial_guess = start
+                xx = x0.clone() if start else torch.zeros_like(d_rhs)
+                r, dt = _solve(m, omegas, d_rhs, d_aux, xx, prm)
+                entry[f"{method}_mode{start}"] = {
+                    "seconds": dt, "iterations": [q["iterations"] for q in r],
+                    "guess_used": [int(q["guess_used"]) for q in r],
+                    "guess_residual": [round(q["guess_residual"], 4) for q in r]}
+                print(bname, method, start, round(dt, 3), entry[f"{method}_mode{start}"]["iterations"], flush=True)
+        report["starts"][bname] = entry
+    net.close()
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(report, f, indent=1)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -187,8 +246,15 @@ def main(argv=None):
     e.add_argument("--level", type=int, default=6)
     e.add_argument("--maxiter", type=int, default=40000)
     e.add_argument("--report", default=None)
+    bd = sub.add_parser("bound")
+    bd.add_argument("--seed", type=int, default=1)
+    bd.add_argument("--level", type=int, default=7)
+    bd.add_argument("--half-width", type=float, default=100e3)
+    bd.add_argument("--station-radius", type=float, default=80e3)
+    bd.add_argument("--maxiter", type=int, default=40000)
+    bd.add_argument("--report", default=None)
     a = ap.parse_args(argv)
-    cmd_train(a) if a.cmd == "train" else cmd_eval(a)
+    {"train": cmd_train, "eval": cmd_eval, "bound": cmd_bound}[a.cmd](a)
 
 
 if __name__ == "__main__":
