@@ -11,10 +11,6 @@ void launch_sensitivity(hipStream_t s, int nc, int n, int nsys, const int* cptr,
                         const double* wgt, const double* sigma, const double* omega, const double2* E,
                         const double2* lam, double* g);
 
-// Row schedule of the row kernels on the current device (0 = grid stride,
-// 1 = one contiguous eighth of the rows per XCD).
-hipError_t set_row_schedule(int mode);
-
 template <int NS, typename V>
 struct Launch {
     using C = cvec<V>;

@@ -189,15 +189,15 @@ __global__ void k_bicg_begin(SysState* st, unsigned mask, int maxiter, double to
 template <int NS, typename V>
 __global__ void k_bicg_init(DevOp<V> op, const double* __restrict__ omega,
                             const cvec<V>* __restrict__ b, const cvec<V>* __restrict__ x,
-                            cvec<V>* __restrict__ r, cvec<V>* __restrict__ r0, SysState* st,
-                            double* part, int x_zero) {
+                            cvec<V>* __restrict__ r, cvec<V>* __restrict__ r0,
+                            const SysState* __restrict__ st, double* part, int x_zero) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const bool run = st[sys].status == kRunning;
+    const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
     const V w = V(omega[sys]);
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n || !run) continue;
@@ -258,18 +258,24 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
                             const SysState* __restrict__ st) {
     const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
     constexpr int rpb = sweep_rpb<NS, SUB>();
+    if (!((status_mask<NS>(st, kRunning) >> sys) & 1u)) return;
+    unsigned first_m = 0;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) first_m |= (st[q].iter == 0 ? 1u : 0u) << q;
+    const bool first = (first_m >> sys) & 1u;
     const SysState& S = st[sys];
-    if (S.status != kRunning) return;
     const V w = V(omega[sys]);
     const int nrows = r1 - r0;
     const int nrb = (nrows + rpb - 1) / rpb;
-    const bool first = S.iter == 0;
     const V br = V(S.beta.x), bi = V(S.beta.y), wr = V(S.omega.x), wi = V(S.omega.y);
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = r0 + rb * rpb + tid / (NS * SUB);
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
+        // Every load the row needs besides the gather goes out before it.
+        const int4 sc = op.slc[row >> 6];
+        const V ds = op.diag_s[row], me = op.me[row];
         cvec<V> src = in[i];
         if (PUPD) {
             if (!first) {
@@ -282,12 +288,11 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
             }
             if (sub == 0) p[i] = src;
         }
-        const int4 sc = op.slc[row >> 6];
         const int lane = row & 63;
         V ar = 0, ai = 0;
         row_gather_split<SUB>(op.l_val, op.l_col, sc.x, sc.y, lane, y, NS, sys, sub, ar, ai);
         V dr, di;
-        cinv(op.diag_s[row], w * op.me[row], dr, di);
+        cinv(ds, w * me, dr, di);
         const V er = src.x - ar, ei = src.y - ai;
         cvec<V> out;
         out.x = er * dr - ei * di;
@@ -304,22 +309,24 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
                             const SysState* __restrict__ st) {
     const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
     constexpr int rpb = sweep_rpb<NS, SUB>();
-    if (st[sys].status != kRunning) return;
+    if (!((status_mask<NS>(st, kRunning) >> sys) & 1u)) return;
     const V w = V(omega[sys]);
     const int nrows = r1 - r0;
     const int nrb = (nrows + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = r0 + rb * rpb + tid / (NS * SUB);
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
+        // Every load the row needs besides the gather goes out before it.
         const int4 sc = op.slc[row >> 6];
+        const V ds = op.diag_s[row], me = op.me[row];
+        const cvec<V> yi = y[i];
         const int lane = row & 63;
         V ar = 0, ai = 0;
         row_gather_split<SUB>(op.u_val, op.u_col, sc.z, sc.w, lane, z, NS, sys, sub, ar, ai);
         V dr, di;
-        cinv(op.diag_s[row], w * op.me[row], dr, di);
-        const cvec<V> yi = y[i];
+        cinv(ds, w * me, dr, di);
         cvec<V> out;
         out.x = yi.x - (ar * dr - ai * di);
         out.y = yi.y - (ar * di + ai * dr);
@@ -332,15 +339,15 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
 template <int NS, typename V>
 __global__ void k_spmv_v(DevOp<V> op, const double* __restrict__ omega,
                          const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys,
-                         const cvec<V>* __restrict__ r0, cvec<V>* __restrict__ v, SysState* st,
-                         double* part) {
+                         const cvec<V>* __restrict__ r0, cvec<V>* __restrict__ v,
+                         const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const bool run = st[sys].status == kRunning;
+    const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
     const V w = V(omega[sys]);
     double acc[2] = {0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n || !run) continue;
@@ -376,14 +383,14 @@ __global__ __launch_bounds__(kFinBlock) void k_spmv_v_fin(SysState* st, const do
 // (bicgstab.cpp:89-103); x += alpha p_hat is applied by k_bicg_xr.
 template <int NS, typename V>
 __global__ void k_bicg_s(int n, const cvec<V>* __restrict__ r, const cvec<V>* __restrict__ v,
-                         cvec<V>* __restrict__ s, SysState* st, double* part) {
+                         cvec<V>* __restrict__ s, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const bool run = st[sys].status == kRunning;
+    const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
     const V ar = V(st[sys].alpha.x), ai = V(st[sys].alpha.y);
     double acc[1] = {0.0};
     const int nrb = (n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= n || !run) continue;
@@ -419,15 +426,15 @@ __global__ __launch_bounds__(kFinBlock) void k_bicg_s_fin(SysState* st, const do
 template <int NS, typename V>
 __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
                          const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys,
-                         const cvec<V>* __restrict__ s, cvec<V>* __restrict__ t, SysState* st,
-                         double* part) {
+                         const cvec<V>* __restrict__ s, cvec<V>* __restrict__ t,
+                         const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const bool run = st[sys].status == kRunning;
+    const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
     const V w = V(omega[sys]);
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n || !run) continue;
@@ -467,17 +474,23 @@ template <int NS, typename V>
 __global__ void k_bicg_xr(int n, cvec<V>* __restrict__ x, const cvec<V>* __restrict__ ph,
                           const cvec<V>* __restrict__ sh, const cvec<V>* __restrict__ s,
                           const cvec<V>* __restrict__ t, const cvec<V>* __restrict__ r0,
-                          cvec<V>* __restrict__ r, SysState* st, double* part) {
+                          cvec<V>* __restrict__ r, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const int status = st[sys].status;
-    const bool run = status == kRunning;
-    const bool pend = status == kPendConvEarly || status == kPendBreakTT;
+    unsigned run_m = 0, pend_m = 0;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const int sq = st[q].status;
+        run_m |= (sq == kRunning ? 1u : 0u) << q;
+        pend_m |= (sq == kPendConvEarly || sq == kPendBreakTT ? 1u : 0u) << q;
+    }
+    const bool run = (run_m >> sys) & 1u;
+    const bool pend = (pend_m >> sys) & 1u;
     const V ar = V(st[sys].alpha.x), ai = V(st[sys].alpha.y);
     const V wr = V(st[sys].omega.x), wi = V(st[sys].omega.y);
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= n || !(run || pend)) continue;
@@ -556,7 +569,7 @@ __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
     const int rpb = rows_per_block<NS>();
     const V w = V(omega[sys]);
     const int nrb = (op.n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
@@ -582,7 +595,7 @@ __global__ void k_true_res(DevOp<double> op, const double* __restrict__ omega,
     const double w = omega[sys];
     double acc[2] = {0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
@@ -618,7 +631,7 @@ __global__ void k_guess_dots(DevOp<double> op, const double* __restrict__ omega,
     const double w = omega[sys];
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
@@ -858,6 +871,16 @@ __global__ void k_phi0(int nn, const int* __restrict__ nt_ptr, const int* __rest
     }
 }
 
+// Bit s set when DivCorr (pcg = false) or its PCG (pcg = true) of system s
+// is running; scalar loads, as status_mask.
+template <int NS>
+__device__ inline unsigned dc_mask(const SysState* __restrict__ st, bool pcg) {
+    unsigned m = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) m |= ((pcg ? st[s].pcg_status : st[s].dc_status) == kRunning ? 1u : 0u) << s;
+    return m;
+}
+
 __global__ void k_dc_begin(SysState* st, unsigned mask) {
     const int s = threadIdx.x;
     if (s >= kMaxSys) return;
@@ -873,13 +896,13 @@ __global__ void k_dc_divj(int nn, const int* __restrict__ nt_ptr, const int* __r
                           const unsigned char* __restrict__ bc_node, const double* __restrict__ ldiag,
                           const double2* __restrict__ E, const double2* __restrict__ phi0,
                           double2* __restrict__ r, double2* __restrict__ z, double2* __restrict__ p,
-                          double2* __restrict__ phi, SysState* st, double* part) {
+                          double2* __restrict__ phi, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const bool run = st[sys].dc_status == kRunning;
+    const bool run = (dc_mask<NS>(st, false) >> sys) & 1u;
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (nn + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
@@ -941,13 +964,13 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_divj_fin(int dc_iter, double d
 template <int NS>
 __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restrict__ lcol,
                         const double* __restrict__ lval, const double2* __restrict__ p,
-                        double2* __restrict__ ap, SysState* st, double* part) {
+                        double2* __restrict__ ap, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const bool run = st[sys].pcg_status == kRunning;
+    const bool run = (dc_mask<NS>(st, true) >> sys) & 1u;
     double acc[2] = {0.0, 0.0};
     const int nrb = (nn + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
@@ -1008,15 +1031,15 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_lp_fin(SysState* st, const dou
 template <int NS>
 __global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const double2* __restrict__ p,
                             const double2* __restrict__ ap, double2* __restrict__ phi,
-                            double2* __restrict__ r, double2* __restrict__ z, SysState* st,
-                            double* part) {
+                            double2* __restrict__ r, double2* __restrict__ z,
+                            const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const bool run = st[sys].pcg_status == kRunning;
+    const bool run = (dc_mask<NS>(st, true) >> sys) & 1u;
     const double2 al = st[sys].pcg_alpha;
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (nn + rpb - 1) / rpb;
-    const XcdRange R = xcd_range(nrb);
+    const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
@@ -1065,10 +1088,11 @@ template <int NS>
 __global__ void k_dc_p(int nn, const double2* __restrict__ z, double2* __restrict__ p,
                        const SysState* __restrict__ st) {
     const size_t total = size_t(nn) * NS;
+    const unsigned run = dc_mask<NS>(st, true);
     for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
          t += size_t(gridDim.x) * blockDim.x) {
         const int sys = int(t % NS);
-        if (st[sys].pcg_status != kRunning) continue;
+        if (!((run >> sys) & 1u)) continue;
         const double2 b = st[sys].pcg_beta;
         const double2 bp = cmul(b, p[t]);
         const double2 zi = z[t];
@@ -1083,10 +1107,11 @@ __global__ void k_dc_correct(int npad, const int* __restrict__ g_col, const doub
                              const double2* __restrict__ phi, double2* __restrict__ E,
                              const SysState* __restrict__ st) {
     const size_t total = size_t(npad) * NS;
+    const unsigned run = dc_mask<NS>(st, false);
     for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
          t += size_t(gridDim.x) * blockDim.x) {
         const int row = int(t / NS), sys = int(t % NS);
-        if (st[sys].dc_status != kRunning || bc_edge[row]) continue;
+        if (!((run >> sys) & 1u) || bc_edge[row]) continue;
         const int a = g_col[2 * row], b = g_col[2 * row + 1];
         if (a < 0) continue;  // padding row
         const double2 pa = phi[size_t(a) * NS + sys], pb = phi[size_t(b) * NS + sys];
@@ -1108,7 +1133,8 @@ inline int grid_for(size_t work_items, int cap) {
     return int(b < size_t(cap) ? (b > 0 ? b : 1) : cap);
 }
 // Grid of a row kernel: one block per row block up to cap, rounded up to a
-// multiple of the XCD count (xcd_range needs it; surplus blocks find no rows).
+// multiple of the XCD count (so every XCD gets the same number of blocks;
+// surplus blocks find no rows).
 inline int grid_rows(int rows, int rpb, int cap) {
     int b = (rows + rpb - 1) / rpb;
     b = b < cap ? b : cap;
@@ -1347,10 +1373,6 @@ void launch_regroup(hipStream_t s, int rows, int na, int nb, const int* map, cve
 }
 template void launch_regroup<double>(hipStream_t, int, int, int, const int*, double2*, double2*, bool);
 template void launch_regroup<float>(hipStream_t, int, int, int, const int*, float2*, float2*, bool);
-
-hipError_t set_row_schedule(int mode) {
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_row_sched), &mode, sizeof(int));
-}
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol) {
     hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(64), 0, s, st, mask, maxiter, tol);

@@ -165,14 +165,6 @@ bool row_sort() {
     return on;
 }
 
-int row_schedule() {
-    static const int mode = [] {
-        const char* v = std::getenv("M3D_ROW_SCHED");
-        return (v && v[0] == '1') ? 1 : 0;
-    }();
-    return mode;
-}
-
 bool use_compaction() {
     static const bool on = [] {
         const char* v = std::getenv("M3D_COMPACT");
@@ -1324,7 +1316,6 @@ int m3d_context_create(int device, m3d_context** out) {
         HIPCHK(hipGetDeviceCount(&count));
         if (device < 0 || device >= count) throw Fail(M3D_ERR_ARG, "device index out of range");
         HIPCHK(hipSetDevice(device));
-        HIPCHK(set_row_schedule(row_schedule()));
         auto* c = new m3d_context;
         c->device = device;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
