@@ -19,6 +19,8 @@ under "variants", the same batch solved
                              region), when its weights exist
 each with its speed-up over the plain FP64 batch and its largest |dZ|/|Z|
 against it; and
+  sigma_update   one line-search sigma update: host recompute + upload against
+                 the recompute on the device (m3d_operator_update_sigma)
   launch_shapes  the autotuner's launch shape per batch width and precision
                  (tuned on the bench mesh before any timed region)
   roofline       the kernel that dominates the timed region (the backward SSOR
@@ -278,6 +280,21 @@ def main():
     log(f"[bench] sweep_bwd x{tw} {ms_sw * 1e3:.1f} us/launch, {ach:.0f} GB/s; spmv x{ns} {ms_spmv:.3f} ms, "
         f"{ach_spmv:.0f} GB/s; ssor x{ns} {ms_ssor:.3f} ms; {info}")
 
+    # A line-search trial's sigma update: host recompute of Me and the DivCorr
+    # Laplacian plus upload, against the recompute on the device.
+    s0 = model.array("sigma").copy()
+    s1 = s0 * np.exp(np.random.default_rng(5).uniform(-0.5, 0.5, s0.shape))
+    t = time.perf_counter()
+    model.set_sigma(s1)
+    host_ms = 1e3 * (time.perf_counter() - t)
+    model.set_sigma(s0)
+    t = time.perf_counter()
+    model.update_sigma_device(s1)
+    dev_ms = 1e3 * (time.perf_counter() - t)
+    model.set_sigma(s0)
+    sigma_update = {"cells": model.nc, "host_ms": host_ms, "device_ms": dev_ms}
+    log(f"[bench] sigma update: host {host_ms:.1f} ms, device {dev_ms:.2f} ms")
+
     models = None
     if args.models > 0:
         models = multi_model_leg(args, model, ctx)
@@ -316,6 +333,7 @@ def main():
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
         "operator": info,
         "launch_shapes": shapes,
+        "sigma_update": sigma_update,
         "variants": legs,
         "multi_model": models,
         "cpu_baseline": cpu,

@@ -218,6 +218,46 @@ int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long*
 int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
                      double* avg_ms, double* alg_bytes);
 
+/* Edge-side data of the edge mass, fixed per mesh, for sigma updates on the
+ * device.  build_edge_mass (operators.cpp:481-518) gives
+ *   Me[e]       = mean(sigma over adj(e)) * (A_dual[e] * L[e]),
+ *                 0 for massless edges (hanging coarse edges) and edges
+ *                 without adjacent cells,
+ * and the DivCorr weight (:956-973, :1034-1048) is
+ *   Me_sigma[e] = (mean(sigma over adj(e)) * A_dual[e]) * L[e], 0 on dead
+ *                 edges,
+ * where the mean runs over the listed cells c with 0 <= c < nc, and is 1e-8
+ * when there is none. */
+typedef struct {
+    int nc;                        /* cells */
+    const int* adj_ptr;            /* [n + 1] cells adjacent to each edge (CSR) */
+    const int* adj_cell;           /* [adj_ptr[n]] in the order the mean adds them */
+    const double* dual_area;       /* [n] A_dual */
+    const double* length;          /* [n] L */
+    const unsigned char* massless; /* [n] edges whose Me is 0 (hanging coarse edges) */
+    const unsigned char* dead;     /* [n] edges whose DivCorr weight is 0 */
+} m3d_mass_structure;
+int m3d_operator_set_mass_structure(m3d_operator* op, const m3d_mass_structure* desc);
+
+/* Sigma update on the device (a line-search trial of the inversion,
+ * inversion_fv.cpp:427-436, where only Me changes): from sigma [nc] it
+ * recomputes Me (the operator diagonal, FP64 and the FP32 copy), the DivCorr
+ * edge weights, the values of the DivCorr Laplacian L = G^T diag(Me_sigma) G
+ * (operators.cpp:952-1019) with its Dirichlet rewrite and Jacobi diagonal
+ * (forward_solver_fv.cpp:581-606), and the sigma of the sensitivity — the
+ * same values, bit for bit, as recomputing them on the host and calling
+ * m3d_operator_set_mass, m3d_operator_set_divcorr and
+ * m3d_operator_set_sensitivity.  Needs m3d_operator_set_mass_structure and a
+ * previous m3d_operator_set_divcorr (its sparsity pattern is kept).  Returns
+ * M3D_ERR_STATE, with the device arrays undefined until the next host update,
+ * if the new sigma would move an edge weight across the 1e-30 threshold that
+ * decides the pattern of L.  sigma is host memory for
+ * m3d_operator_update_sigma and device memory for
+ * m3d_operator_update_sigma_device (ordered after earlier work on the
+ * context stream; returns after the update is done). */
+int m3d_operator_update_sigma(m3d_operator* op, const double* sigma);
+int m3d_operator_update_sigma_device(m3d_operator* op, const double* d_sigma);
+
 /* Launch shape of the row kernels at one batch width and precision (no
  * reference counterpart: the reference has one CPU thread per system).
  * Only the assignment of rows to GPU threads changes; the arithmetic of each
@@ -242,7 +282,7 @@ int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d
  * precisions, times one BiCGStab iteration (as m3d_bench_kernel kinds 2 and 3,
  * reps iterations per candidate, frequencies omega[0..width-1]) under candidate
  * shapes by coordinate descent (sweep_sub, then sweep_cap, then row_cap) and
- * keeps the fastest.  best_ms, if not NULL, receives 2 * 6 values:
+ * keeps a candidate only when it is at least 5 % faster than the incumbent.  best_ms, if not NULL, receives 2 * 6 values:
  * [precision * 6 + log2(width)] = ms per iteration of the kept shape, 0 where
  * not tuned.  Blocking; run it outside timed regions.  Choices stay with the
  * operator; read them with m3d_operator_get_shape to pin them elsewhere. */

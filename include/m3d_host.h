@@ -9,6 +9,9 @@
  *                       DiscreteOperators::build (operators.cpp:19-170),
  *                       ForwardSolverFV::setup (forward_solver_fv.cpp:28-47)
  *   m3dh_set_sigma      ForwardSolverFV::set_sigma / update_sigma (:49-58)
+ *   m3dh_update_sigma_device  the same update with Me and the DivCorr
+ *                       Laplacian recomputed on the device (line-search
+ *                       trials, inversion_fv.cpp:427-436)
  *   m3dh_find_stations  find_stations_ (:1393-1485)
  *   m3dh_solve_frequencies   solve_frequency (:84-531) for several frequencies
  *   m3dh_extract_impedance   extract_impedance_ (:1490-1727)
@@ -74,6 +77,9 @@ int m3dh_get_array(m3dh_model* m, const char* name, int* dtype, long long* count
 long long m3dh_get_count(m3dh_model* m, const char* name);
 
 int m3dh_set_sigma(m3dh_model* m, const double* sigma);
+/* After one m3dh_set_sigma: a new sigma [cells] applied by
+ * m3d_operator_update_sigma (device arrays equal to m3dh_set_sigma's). */
+int m3dh_update_sigma_device(m3dh_model* m, const double* sigma);
 int m3dh_find_stations(m3dh_model* m, const double* xy, int n);
 
 /* Host producers for one polarisation (dir 0 = pol 1 / Ex, 1 = pol 2 / Ey). */

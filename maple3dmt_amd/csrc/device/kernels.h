@@ -81,6 +81,18 @@ struct Launch64 {
 };
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol);
+// Sigma update: per device row the operator's Me (FP64, FP32) and the
+// DivCorr weight; per node the Laplacian values and Jacobi diagonal.
+void launch_sigma_edges(hipStream_t s, int npad, int nc, const int* perm, const int* adj_ptr,
+                        const int* adj_cell, const double* area, const double* len,
+                        const unsigned char* massless, const unsigned char* dead,
+                        const unsigned char* constrained, const unsigned char* dc_dead,
+                        const unsigned char* in_pattern, const double* sigma, double* me, float* me32,
+                        double* ms, int* err);
+void launch_sigma_laplacian(hipStream_t s, int nn, const int* nt_ptr, const int* nt_edge, const double* nt_val,
+                            const double* ms, const int* lptr, const int* lcol, const int* ledge,
+                            const int* g_col, const double* g_val, const unsigned char* bc_node, double* lval,
+                            double* ldiag);
 template <typename V>
 void launch_regroup(hipStream_t s, int rows, int na, int nb, const int* map, cvec<V>* wide,
                     cvec<V>* narrow, bool to_narrow);

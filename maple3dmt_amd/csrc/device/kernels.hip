@@ -1125,6 +1125,97 @@ __global__ void k_dc_correct(int npad, const int* __restrict__ g_col, const doub
 }
 
 // ---------------------------------------------------------------------------
+// Sigma update on the device: values only, the sparsity patterns stay.  Every
+// sum runs in the host builders' order and products are not contracted into
+// FMAs, so the results equal the host path's bit for bit.
+// ---------------------------------------------------------------------------
+
+// Per device row r (edge e = perm[r]): the operator's Me (edge_mass,
+// operators.cpp:481-518; 0 on massless, adjacency-free, constrained and
+// padding rows) in FP64 and FP32, and the DivCorr weight Me_sigma
+// (edge_mass_divcorr; 0 on dead edges).  *err |= 1 when a weight crosses the
+// 1e-30 threshold that decided the Laplacian pattern (in_pattern[r]).
+__global__ void k_sigma_edges(int npad, int nc, const int* __restrict__ perm, const int* __restrict__ adj_ptr,
+                              const int* __restrict__ adj_cell, const double* __restrict__ area,
+                              const double* __restrict__ len, const unsigned char* __restrict__ massless,
+                              const unsigned char* __restrict__ dead, const unsigned char* __restrict__ constrained,
+                              const unsigned char* __restrict__ dc_dead, const unsigned char* __restrict__ in_pattern,
+                              const double* __restrict__ sigma, double* __restrict__ me, float* __restrict__ me32,
+                              double* __restrict__ ms, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < npad; r += gridDim.x * blockDim.x) {
+        const int e = perm[r];
+        double m = 0.0, w = 0.0;
+        if (e >= 0) {
+            const int a = adj_ptr[e], b = adj_ptr[e + 1];
+            double sum = 0.0;
+            int count = 0;
+            for (int k = a; k < b; ++k) {
+                const int c = adj_cell[k];
+                if (c >= 0 && c < nc) {
+                    sum += sigma[c];
+                    ++count;
+                }
+            }
+            const double avg = count > 0 ? sum / count : 1e-8;
+            if (b > a && !massless[e] && !constrained[r]) {
+                const double hodge = area[e] * len[e];
+                m = avg * hodge;
+            }
+            if (!dead[e] && !dc_dead[r]) w = avg * area[e] * len[e];
+            if ((w >= 1e-30) != (in_pattern[r] != 0)) atomicOr(err, 1);
+        }
+        me[r] = m;
+        me32[r] = float(m);
+        ms[r] = w;
+    }
+}
+
+// Per node v: the values of row v of L = G^T diag(Me_sigma) G
+// (div_laplacian, operators.cpp:952-1019: the diagonal adds g w g over the
+// incident edges in ascending edge order, an off-diagonal entry is the one
+// edge joining the two nodes) with the Dirichlet rewrite and the Jacobi
+// diagonal taken before it (compute_bc_flags_, forward_solver_fv.cpp:581-606).
+// ledge[k] is the device row of the edge of off-diagonal entry k.
+__global__ void k_sigma_laplacian(int nn, const int* __restrict__ nt_ptr, const int* __restrict__ nt_edge,
+                                  const double* __restrict__ nt_val, const double* __restrict__ ms,
+                                  const int* __restrict__ lptr, const int* __restrict__ lcol,
+                                  const int* __restrict__ ledge, const int* __restrict__ g_col,
+                                  const double* __restrict__ g_val, const unsigned char* __restrict__ bc_node,
+                                  double* __restrict__ lval, double* __restrict__ ldiag) {
+#pragma clang fp contract(off)
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nn; v += gridDim.x * blockDim.x) {
+        double d = 0.0;
+        for (int q = nt_ptr[v]; q < nt_ptr[v + 1]; ++q) {
+            const double w = ms[nt_edge[q]];
+            if (w < 1e-30) continue;
+            const double g = nt_val[q];
+            d += g * w * g;
+        }
+        const bool bv = bc_node[v] != 0;
+        bool has_diag = false;
+        for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+            const int u = lcol[k];
+            double val;
+            if (u == v) {
+                val = bv ? 1.0 : d;
+                has_diag = true;
+            } else if (bv || bc_node[u]) {
+                val = 0.0;
+            } else {
+                const int r = ledge[k];
+                const double w = ms[r];
+                const double gv = g_col[2 * r] == v ? g_val[2 * r] : g_val[2 * r + 1];
+                const double gu = g_col[2 * r] == u ? g_val[2 * r] : g_val[2 * r + 1];
+                val = gv * w * gu;
+            }
+            lval[k] = val;
+        }
+        ldiag[v] = (bv || !has_diag) ? 1.0 : d;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers (explicit instantiation per batch width).
 // ---------------------------------------------------------------------------
 namespace {
@@ -1373,6 +1464,24 @@ void launch_regroup(hipStream_t s, int rows, int na, int nb, const int* map, cve
 }
 template void launch_regroup<double>(hipStream_t, int, int, int, const int*, double2*, double2*, bool);
 template void launch_regroup<float>(hipStream_t, int, int, int, const int*, float2*, float2*, bool);
+
+void launch_sigma_edges(hipStream_t s, int npad, int nc, const int* perm, const int* adj_ptr,
+                        const int* adj_cell, const double* area, const double* len,
+                        const unsigned char* massless, const unsigned char* dead,
+                        const unsigned char* constrained, const unsigned char* dc_dead,
+                        const unsigned char* in_pattern, const double* sigma, double* me, float* me32,
+                        double* ms, int* err) {
+    hipLaunchKernelGGL(k_sigma_edges, dim3(grid_for(size_t(npad), kStreamCap)), dim3(kBlock), 0, s, npad, nc,
+                       perm, adj_ptr, adj_cell, area, len, massless, dead, constrained, dc_dead, in_pattern,
+                       sigma, me, me32, ms, err);
+}
+void launch_sigma_laplacian(hipStream_t s, int nn, const int* nt_ptr, const int* nt_edge, const double* nt_val,
+                            const double* ms, const int* lptr, const int* lcol, const int* ledge,
+                            const int* g_col, const double* g_val, const unsigned char* bc_node, double* lval,
+                            double* ldiag) {
+    hipLaunchKernelGGL(k_sigma_laplacian, dim3(grid_for(size_t(nn), kStreamCap)), dim3(kBlock), 0, s, nn, nt_ptr,
+                       nt_edge, nt_val, ms, lptr, lcol, ledge, g_col, g_val, bc_node, lval, ldiag);
+}
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol) {
     hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(64), 0, s, st, mask, maxiter, tol);

@@ -57,6 +57,34 @@ void ForwardSolverGPU::setup(const OctreeMesh& mesh, const ForwardParams& params
     d.n_hanging = static_cast<int>(tri.size() / 3);
     d.hanging = tri.empty() ? nullptr : tri.data();
     check(m3d_operator_create(ctx_, &d, &op_));
+    // Edge-side data of the edge mass, for sigma updates on the device.
+    const auto& ops = prob_.ops();
+    const auto& adj = ops.edge_adjacency();
+    std::vector<int> aptr(d.n + 1, 0), acell;
+    for (int e = 0; e < d.n; ++e) {
+        acell.insert(acell.end(), adj[e].begin(), adj[e].end());
+        aptr[e + 1] = static_cast<int>(acell.size());
+    }
+    std::vector<unsigned char> massless(d.n, 0), dead(d.n, 0);
+    for (const auto& hp : ops.hanging_pairs()) massless[hp.coarse] = 1;
+    for (int e = 0; e < d.n; ++e) dead[e] = ops.is_dead_edge(e) ? 1 : 0;
+    m3d_mass_structure ms{};
+    ms.nc = prob_.num_cells();
+    ms.adj_ptr = aptr.data();
+    ms.adj_cell = acell.empty() ? nullptr : acell.data();
+    ms.dual_area = ops.edge_dual_areas().data();
+    ms.length = ops.edge_lengths().data();
+    ms.massless = massless.data();
+    ms.dead = dead.data();
+    check(m3d_operator_set_mass_structure(op_, &ms));
+}
+
+void ForwardSolverGPU::update_sigma_device(const std::vector<double>& sigma) {
+    if (static_cast<int>(sigma.size()) != prob_.num_cells())
+        throw std::invalid_argument("sigma size does not match the number of cells");
+    if (!op_) throw std::runtime_error("update_sigma_device needs a device operator");
+    check(m3d_operator_update_sigma(op_, sigma.data()));
+    prob_.sigma = sigma;
 }
 
 void ForwardSolverGPU::set_sigma(const std::vector<double>& sigma) {

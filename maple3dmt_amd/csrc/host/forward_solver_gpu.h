@@ -32,6 +32,12 @@ public:
     // set_sigma / update_sigma (:49-58): host sigma data and the device mass
     // and DivCorr arrays.
     void set_sigma(const std::vector<double>& sigma);
+    // update_sigma for a line-search trial: the device recomputes Me, the
+    // DivCorr weights and Laplacian values and the sensitivity sigma itself
+    // (m3d_operator_update_sigma; equal to set_sigma's device arrays).  The
+    // host keeps the new sigma; its own Me and Laplacian copies are not
+    // refreshed.  Needs one set_sigma first (the Laplacian pattern).
+    void update_sigma_device(const std::vector<double>& sigma);
 
     void compute_responses(const data::MTData& observed, data::MTData& predicted);
     void compute_single_frequency(int freq_idx, const data::MTData& observed,

@@ -307,6 +307,14 @@ int m3dh_set_sigma(m3dh_model* m, const double* sigma) {
     });
 }
 
+int m3dh_update_sigma_device(m3dh_model* m, const double* sigma) {
+    return guarded([&] {
+        const int nc = m->mesh.num_cells();
+        m->fwd.update_sigma_device(std::vector<double>(sigma, sigma + nc));
+        m->cache.clear();
+    });
+}
+
 int m3dh_find_stations(m3dh_model* m, const double* xy, int n) {
     return guarded([&] {
         std::vector<std::array<double, 2>> v;

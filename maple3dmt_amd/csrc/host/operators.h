@@ -51,6 +51,10 @@ public:
     bool is_dead_face(int f) const { return dead_face_[f] != 0; }
     const std::vector<HangingPair>& hanging_pairs() const { return hanging_pairs_; }
     double edge_dual_area(int e) const { return edge_dual_area_[e]; }
+    const std::vector<double>& edge_dual_areas() const { return edge_dual_area_; }
+    const std::vector<double>& edge_lengths() const { return edge_len_; }
+    // Cells adjacent to each edge, in the order the edge mass averages them.
+    const std::vector<std::vector<int>>& edge_adjacency() const { return edge_adj_; }
     double face_dual_length(int f) const { return face_dual_len_[f]; }
     const std::vector<double>& node_x() const { return node_x_; }
     const std::vector<double>& node_y() const { return node_y_; }

@@ -132,13 +132,15 @@ DEVICE_SYMBOLS = [
     "m3d_solve_batch_device", "m3d_divcorr", "m3d_spmv", "m3d_ssor_apply",
     "m3d_operator_permutation", "m3d_operator_info", "m3d_context_stream", "m3d_bench_kernel",
     "m3d_operator_set_sensitivity", "m3d_sensitivity_device", "m3d_operator_get_shape",
-    "m3d_operator_set_shape", "m3d_operator_autotune",
+    "m3d_operator_set_shape", "m3d_operator_autotune", "m3d_operator_set_mass_structure",
+    "m3d_operator_update_sigma", "m3d_operator_update_sigma_device",
 ]
 HOST_SYMBOLS = [
     "m3dh_last_error", "m3dh_model_create", "m3dh_model_destroy", "m3dh_get_array",
     "m3dh_get_count", "m3dh_set_sigma", "m3dh_find_stations", "m3dh_primary_field",
     "m3dh_bc_values", "m3dh_rhs_total", "m3dh_phi0", "m3dh_extract_impedance", "m3dh_adjoint_rhs",
     "m3dh_sensitivity", "m3dh_solve_frequencies", "m3dh_adjoint_solve", "m3dh_device_operator",
+    "m3dh_update_sigma_device",
 ]
 
 
@@ -188,6 +190,9 @@ def lib() -> C.CDLL:
     L.m3dh_get_count.argtypes = [vp, C.c_char_p]
     L.m3dh_get_count.restype = C.c_longlong
     L.m3dh_set_sigma.argtypes = [vp, vp]
+    L.m3dh_update_sigma_device.argtypes = [vp, vp]
+    L.m3d_operator_update_sigma.argtypes = [vp, vp]
+    L.m3d_operator_update_sigma_device.argtypes = [vp, vp]
     L.m3dh_find_stations.argtypes = [vp, vp, i]
     L.m3dh_primary_field.argtypes = [vp, d, i, vp]
     L.m3dh_bc_values.argtypes = [vp, d, i, vp]
@@ -372,6 +377,10 @@ class DeviceOperator:
                                           C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def update_sigma_device(self, d_sigma: int) -> None:
+        """m3d_operator_update_sigma_device: sigma [cells] in device memory."""
+        _check_dev(lib().m3d_operator_update_sigma_device(self.handle, C.c_void_p(d_sigma)))
+
     def get_shape(self, precision: int, width: int) -> dict:
         """Launch shape of the row kernels at one precision and batch width."""
         sh = LaunchShapeC()
@@ -541,6 +550,14 @@ class Model:
         s = np.ascontiguousarray(sigma, dtype=np.float64)
         assert s.shape == (self.nc,)
         _check_host(lib().m3dh_set_sigma(self.handle, _ptr(s)))
+
+    def update_sigma_device(self, sigma) -> None:
+        """set_sigma for a line-search trial after one set_sigma: the device
+        recomputes Me and the DivCorr Laplacian (the same arrays set_sigma
+        would upload); the host keeps the new sigma."""
+        s = np.ascontiguousarray(sigma, dtype=np.float64)
+        assert s.shape == (self.nc,)
+        _check_host(lib().m3dh_update_sigma_device(self.handle, _ptr(s)))
 
     def find_stations(self, xy) -> np.ndarray:
         a = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
