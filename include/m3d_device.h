@@ -269,11 +269,15 @@ typedef struct m3d_launch_shape {
     int sweep_sub; /* lanes per row in the SSOR sweeps: 1, 2, 4 or 8, with width * sub <= 32 */
     int sweep_cap; /* grid cap of one sweep launch, workgroups: a multiple of 8 in 8..65536 */
     int row_cap;   /* grid cap of the SpMV kernels that reduce: a multiple of 8 in 8..2048 */
+    int sweep_chunk; /* 1: a sweep's workgroups grid-stride over its row blocks (sweep_cap
+                        applies); 2, 4, 8 or 16: each workgroup takes that many consecutive
+                        row blocks (sweep_cap is then unused) */
 } m3d_launch_shape;
 
 /* width is a batch width 1, 2, 4, 8, 16 or 32; precision M3D_PREC_FP64
  * (complex128 kernels) or M3D_PREC_MIXED (the complex64 inner kernels).  The
- * default shape of a new operator: sweep_sub 1, sweep_cap 4096, row_cap 2048. */
+ * default shape of a new operator: sweep_sub 1, sweep_cap 4096, row_cap 2048,
+ * sweep_chunk 1. */
 int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launch_shape* out);
 int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d_launch_shape* shape);
 
@@ -281,7 +285,7 @@ int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d
  * every width whose bit is set in widths_mask (bit i = width 1 << i) and both
  * precisions, times one BiCGStab iteration (as m3d_bench_kernel kinds 2 and 3,
  * reps iterations per candidate, frequencies omega[0..width-1]) under candidate
- * shapes by coordinate descent (sweep_sub, then sweep_cap, then row_cap) and
+ * shapes by coordinate descent (sweep_sub, sweep_cap, row_cap, sweep_chunk) and
  * keeps a candidate only when it is at least 5 % faster than the incumbent.  best_ms, if not NULL, receives 2 * 6 values:
  * [precision * 6 + log2(width)] = ms per iteration of the kept shape, 0 where
  * not tuned.  Blocking; run it outside timed regions.  Choices stay with the

@@ -90,6 +90,8 @@ struct LaunchShape {
     int sweep_sub;  // lanes per row in the SSOR sweeps (1, 2, 4 or 8; width * sub <= 32)
     int sweep_cap;  // grid cap of a sweep launch (workgroups, a multiple of kXcds)
     int row_cap;    // grid cap of the row kernels that reduce (a multiple of kXcds, <= kMaxBlocks)
+    int sweep_chunk;  // 1: sweeps grid-stride over row blocks; k > 1: each workgroup
+                      // takes k consecutive row blocks (neighbouring rows on one CU)
 };
 constexpr int kWidths = 6;  // batch widths 1, 2, 4, 8, 16, 32
 __host__ __device__ constexpr int width_index(int ns) {
@@ -124,6 +126,17 @@ __device__ inline RowRange row_range(int nrb) {
     r.first = blockIdx.x;
     r.end = nrb;
     r.step = gridDim.x;
+    return r;
+}
+
+// Row blocks of a sweep workgroup: the grid stride, or `chunk` consecutive
+// row blocks per workgroup (the grid then holds ceil(nrb / chunk) workgroups).
+__device__ inline RowRange row_range_chunk(int nrb, int chunk) {
+    if (chunk <= 1) return row_range(nrb);
+    RowRange r;
+    r.first = blockIdx.x * chunk;
+    r.end = min(nrb, r.first + chunk);
+    r.step = 1;
     return r;
 }
 

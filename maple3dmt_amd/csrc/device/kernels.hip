@@ -268,7 +268,7 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
     const int nrows = r1 - r0;
     const int nrb = (nrows + rpb - 1) / rpb;
     const V br = V(S.beta.x), bi = V(S.beta.y), wr = V(S.omega.x), wi = V(S.omega.y);
-    const RowRange R = row_range(nrb);
+    const RowRange R = row_range_chunk(nrb, op.shape[width_index(NS)].sweep_chunk);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = r0 + rb * rpb + tid / (NS * SUB);
         if (row >= r1) continue;
@@ -313,7 +313,7 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
     const V w = V(omega[sys]);
     const int nrows = r1 - r0;
     const int nrb = (nrows + rpb - 1) / rpb;
-    const RowRange R = row_range(nrb);
+    const RowRange R = row_range_chunk(nrb, op.shape[width_index(NS)].sweep_chunk);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = r0 + rb * rpb + tid / (NS * SUB);
         if (row >= r1) continue;
@@ -1266,10 +1266,20 @@ void Launch<NS, V>::bicg_init(hipStream_t s, const DevOp<V>& op, const double* o
 template <int NS, int SUB>
 constexpr bool sub_ok() { return NS * SUB <= 32; }
 
+// Workgroups of a sweep launch over `rows` rows under the operator's shape.
+template <int NS, int SUB, typename V>
+int sweep_grid(const DevOp<V>& op, int rows) {
+    const LaunchShape& sh = op.shape[width_index(NS)];
+    constexpr int rpb = sweep_rpb<NS, SUB>();
+    if (sh.sweep_chunk <= 1) return grid_rows(rows, rpb, sh.sweep_cap);
+    const int nrb = (rows + rpb - 1) / rpb;
+    return grid_rows(nrb, sh.sweep_chunk, 1 << 30);
+}
+
 template <int NS, typename V, int SUB>
 void sweep_fwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1, bool pupd,
                   const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, const SysState* st) {
-    const dim3 g(grid_rows(r1 - r0, sweep_rpb<NS, SUB>(), op.shape[width_index(NS)].sweep_cap));
+    const dim3 g(sweep_grid<NS, SUB>(op, r1 - r0));
     if (pupd)
         hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, true>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
                            in, p, v, y, st);
@@ -1280,7 +1290,7 @@ void sweep_fwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0
 template <int NS, typename V, int SUB>
 void sweep_bwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
                   const cvec<V>* y, cvec<V>* z, const SysState* st) {
-    const dim3 g(grid_rows(r1 - r0, sweep_rpb<NS, SUB>(), op.shape[width_index(NS)].sweep_cap));
+    const dim3 g(sweep_grid<NS, SUB>(op, r1 - r0));
     hipLaunchKernelGGL((k_sweep_bwd<NS, V, SUB>), g, dim3(kBlock), 0, s, op, omega, r0, r1, y, z, st);
 }
 

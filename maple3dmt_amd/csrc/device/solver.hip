@@ -1077,6 +1077,7 @@ LaunchShape default_shape(int width) {
     sh.sweep_sub = 1;  // with grouped gathers one lane per row keeps enough loads in flight
     sh.sweep_cap = 4096;
     sh.row_cap = kMaxBlocks;
+    sh.sweep_chunk = 1;
     return sh;
 }
 
@@ -1799,8 +1800,10 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
 bool valid_shape(int width, const LaunchShape& sh) {
     const int sub = sh.sweep_sub;
     const bool sub_ok = (sub == 1 || sub == 2 || sub == 4 || sub == 8) && width * sub <= 32;
+    const int ch = sh.sweep_chunk;
     return sub_ok && sh.sweep_cap >= kXcds && sh.sweep_cap <= 65536 && sh.sweep_cap % kXcds == 0 &&
-           sh.row_cap >= kXcds && sh.row_cap <= kMaxBlocks && sh.row_cap % kXcds == 0;
+           sh.row_cap >= kXcds && sh.row_cap <= kMaxBlocks && sh.row_cap % kXcds == 0 &&
+           (ch == 1 || ch == 2 || ch == 4 || ch == 8 || ch == 16);
 }
 
 int width_slot(int width) {
@@ -1867,6 +1870,13 @@ double tune_one(m3d_operator& op, int p, int slot, const double* omega, int reps
         if (cap == after_cap.row_cap) continue;
         LaunchShape c = best;
         c.row_cap = cap;
+        consider(c);
+    }
+    const LaunchShape after_row = best;
+    for (int ch : {1, 2, 4, 8}) {
+        if (ch == after_row.sweep_chunk) continue;
+        LaunchShape c = best;
+        c.sweep_chunk = ch;
         consider(c);
     }
     op.shape[p][slot] = best;
@@ -1958,6 +1968,7 @@ int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launc
         out->sweep_sub = sh.sweep_sub;
         out->sweep_cap = sh.sweep_cap;
         out->row_cap = sh.row_cap;
+        out->sweep_chunk = sh.sweep_chunk;
     });
 }
 
@@ -1970,6 +1981,7 @@ int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d
         sh.sweep_sub = shape->sweep_sub;
         sh.sweep_cap = shape->sweep_cap;
         sh.row_cap = shape->row_cap;
+        sh.sweep_chunk = shape->sweep_chunk;
         if (!valid_shape(width, sh)) throw Fail(M3D_ERR_ARG, "invalid launch shape for this width");
         HIPCHK(hipSetDevice(op->ctx->device));
         HIPCHK(hipStreamSynchronize(op->ctx->stream));

@@ -122,6 +122,7 @@ class LaunchShapeC(C.Structure):
         ("sweep_sub", C.c_int),
         ("sweep_cap", C.c_int),
         ("row_cap", C.c_int),
+        ("sweep_chunk", C.c_int),
     ]
 
 
@@ -385,10 +386,12 @@ class DeviceOperator:
         """Launch shape of the row kernels at one precision and batch width."""
         sh = LaunchShapeC()
         _check_dev(lib().m3d_operator_get_shape(self.handle, precision, width, C.byref(sh)))
-        return {"sweep_sub": sh.sweep_sub, "sweep_cap": sh.sweep_cap, "row_cap": sh.row_cap}
+        return {"sweep_sub": sh.sweep_sub, "sweep_cap": sh.sweep_cap, "row_cap": sh.row_cap,
+                "sweep_chunk": sh.sweep_chunk}
 
-    def set_shape(self, precision: int, width: int, sweep_sub: int, sweep_cap: int, row_cap: int) -> None:
-        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap)
+    def set_shape(self, precision: int, width: int, sweep_sub: int, sweep_cap: int, row_cap: int,
+                  sweep_chunk: int = 1) -> None:
+        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap, sweep_chunk)
         _check_dev(lib().m3d_operator_set_shape(self.handle, precision, width, C.byref(sh)))
 
     def autotune(self, omegas, widths=(1, 2, 4, 8, 16), reps: int = 16) -> dict:

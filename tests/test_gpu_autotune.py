@@ -39,14 +39,18 @@ def test_shape_defaults_and_validation(block):
     dev = m.device
     for prec in (PREC_FP64, PREC_MIXED):
         for w in (1, 2, 4, 8, 16, 32):
-            assert dev.get_shape(prec, w) == {"sweep_sub": 1, "sweep_cap": 4096, "row_cap": 2048}
+            assert dev.get_shape(prec, w) == {"sweep_sub": 1, "sweep_cap": 4096, "row_cap": 2048,
+                                              "sweep_chunk": 1}
     for bad in ((3, 4096, 2048), (8, 4096, 2048), (1, 100, 2048), (1, 4096, 4096), (1, 0, 2048)):
         with pytest.raises(NativeError):
             dev.set_shape(PREC_FP64, 8, *bad)   # width 8: sub 8 gives 64 lanes per row
     with pytest.raises(NativeError):
         dev.get_shape(PREC_FP64, 3)
-    dev.set_shape(PREC_FP64, 8, 4, 1024, 512)
-    assert dev.get_shape(PREC_FP64, 8) == {"sweep_sub": 4, "sweep_cap": 1024, "row_cap": 512}
+    with pytest.raises(NativeError):
+        dev.set_shape(PREC_FP64, 8, 1, 4096, 2048, 3)   # chunk 3
+    dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4)
+    assert dev.get_shape(PREC_FP64, 8) == {"sweep_sub": 4, "sweep_cap": 1024, "row_cap": 512,
+                                           "sweep_chunk": 4}
     _reset(dev)
 
 
@@ -67,9 +71,9 @@ def test_every_sweep_split_matches_oracle_ssor(block, width):
         for sub in (1, 2, 4, 8):
             if width * sub > 32:
                 continue
-            for cap in (8, 4096):
-                dev.set_shape(PREC_FP64, width, sub, cap, 2048)
-                print(f"width {width} sweep_sub {sub} sweep_cap {cap}", flush=True)
+            for cap, chunk in ((8, 1), (4096, 1), (4096, 4), (4096, 16)):
+                dev.set_shape(PREC_FP64, width, sub, cap, 2048, chunk)
+                print(f"width {width} sweep_sub {sub} sweep_cap {cap} sweep_chunk {chunk}", flush=True)
                 z = dev.ssor_apply(np.full(width, w), r)
                 for i in range(width):
                     assert np.linalg.norm(z[i] - ref[i]) <= 1e-12 * np.linalg.norm(ref[i]), (sub, cap, i)
