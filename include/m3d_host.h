@@ -16,6 +16,7 @@
  *   m3dh_solve_frequencies   solve_frequency (:84-531) for several frequencies
  *   m3dh_extract_impedance   extract_impedance_ (:1490-1727)
  *   m3dh_adjoint_rhs    build_adjoint_rhs_from_residual (:1804-2013)
+ *   m3dh_*_compact      the same two on the station cells' edges only
  *   m3dh_adjoint_solve  adjoint_solve_complex / solve_rhs_complex (:914-940)
  *   m3dh_sensitivity    compute_sensitivity (:989-1059)
  */
@@ -96,6 +97,19 @@ int m3dh_adjoint_rhs(m3dh_model* m, double omega, const double* wr_c128, double*
                      double* rhs2);
 int m3dh_sensitivity(m3dh_model* m, double omega, const double* e_bg, const double* lambda,
                      double* sens);
+
+/* The same station steps on compact arrays, for fields that stay on the
+ * device: m3dh_station_edges gives the sorted edges of the station cells
+ * (count values; the pointer stays valid until the next call or
+ * m3dh_find_stations); e1c / e2c and rhs1c / rhs2c hold one complex value per
+ * listed edge.  Z and the station H cache equal m3dh_extract_impedance's on
+ * the full fields, and the right-hand sides equal m3dh_adjoint_rhs's on the
+ * listed edges (zero elsewhere). */
+int m3dh_station_edges(m3dh_model* m, int* count, const int** edges);
+int m3dh_extract_impedance_compact(m3dh_model* m, double omega, const double* e1c_c128,
+                                   const double* e2c_c128, double* z_out);
+int m3dh_adjoint_rhs_compact(m3dh_model* m, double omega, const double* wr_c128, double* rhs1c,
+                             double* rhs2c);
 
 /* Device solves through the model's device operator. */
 int m3dh_solve_frequencies(m3dh_model* m, const double* freqs, int nf, const m3d_solve_params* prm,

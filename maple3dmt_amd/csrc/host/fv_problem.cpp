@@ -206,10 +206,9 @@ struct CellGroups {
 
 }  // namespace
 
-void FVProblem::extract_impedance(double omega, const std::vector<cplx>& e1,
-                                  const std::vector<cplx>& e2,
-                                  std::vector<std::array<cplx, 4>>& z,
-                                  std::vector<StationHCache>& cache) const {
+template <typename Get1, typename Get2>
+void FVProblem::extract_impedance_core(double omega, Get1 e1, Get2 e2, std::vector<std::array<cplx, 4>>& z,
+                                       std::vector<StationHCache>& cache) const {
     const OctreeMesh& m = *mesh_;
     const cplx iwmu(0.0, omega * kMu0);
     const int ns = static_cast<int>(station_cell_.size());
@@ -229,16 +228,16 @@ void FVProblem::extract_impedance(double omega, const std::vector<cplx>& e1,
             const int e = ce.edge_id;
             const int d = grid_.edge_dir[e];
             if (d == 0) {
-                if (grid_.edge_z[e] > cz) { x1t += e1[e]; x2t += e2[e]; ++g.nxt; }
-                else { x1b += e1[e]; x2b += e2[e]; ++g.nxb; }
+                if (grid_.edge_z[e] > cz) { x1t += e1(e); x2t += e2(e); ++g.nxt; }
+                else { x1b += e1(e); x2b += e2(e); ++g.nxb; }
             } else if (d == 1) {
-                if (grid_.edge_z[e] > cz) { y1t += e1[e]; y2t += e2[e]; ++g.nyt; }
-                else { y1b += e1[e]; y2b += e2[e]; ++g.nyb; }
+                if (grid_.edge_z[e] > cz) { y1t += e1(e); y2t += e2(e); ++g.nyt; }
+                else { y1b += e1(e); y2b += e2(e); ++g.nyb; }
             } else {
-                if (grid_.edge_x[e] > cx) { z1px += e1[e]; z2px += e2[e]; ++g.nzpx; }
-                else { z1mx += e1[e]; z2mx += e2[e]; ++g.nzmx; }
-                if (grid_.edge_y[e] > cy) { z1py += e1[e]; z2py += e2[e]; ++g.nzpy; }
-                else { z1my += e1[e]; z2my += e2[e]; ++g.nzmy; }
+                if (grid_.edge_x[e] > cx) { z1px += e1(e); z2px += e2(e); ++g.nzpx; }
+                else { z1mx += e1(e); z2mx += e2(e); ++g.nzmx; }
+                if (grid_.edge_y[e] > cy) { z1py += e1(e); z2py += e2(e); ++g.nzpy; }
+                else { z1my += e1(e); z2my += e2(e); ++g.nzmy; }
             }
         }
         auto avg = [](cplx& a, cplx& b, int n) {
@@ -287,14 +286,20 @@ void FVProblem::extract_impedance(double omega, const std::vector<cplx>& e1,
     }
 }
 
+void FVProblem::extract_impedance(double omega, const std::vector<cplx>& e1,
+                                  const std::vector<cplx>& e2,
+                                  std::vector<std::array<cplx, 4>>& z,
+                                  std::vector<StationHCache>& cache) const {
+    auto g1 = [&](int e) -> const cplx& { return e1[e]; };
+    auto g2 = [&](int e) -> const cplx& { return e2[e]; };
+    extract_impedance_core(omega, g1, g2, z, cache);
+}
+
 // Q^T of Z = E H^-1: E part via H^-T and H part via -Z^T and the transpose
 // of the station-cell curl (:1804-2013).
-void FVProblem::adjoint_rhs(double omega, const std::vector<StationHCache>& cache,
-                            const std::vector<std::array<cplx, 4>>& wr, std::vector<cplx>& rhs1,
-                            std::vector<cplx>& rhs2) const {
-    const int ne = num_edges();
-    rhs1.assign(ne, cplx(0, 0));
-    rhs2.assign(ne, cplx(0, 0));
+template <typename Add>
+void FVProblem::adjoint_rhs_core(double omega, const std::vector<StationHCache>& cache,
+                                 const std::vector<std::array<cplx, 4>>& wr, Add add) const {
     const cplx iwmu(0.0, omega * kMu0);
     const OctreeMesh& m = *mesh_;
     for (size_t s = 0; s < station_cell_.size(); ++s) {
@@ -344,47 +349,93 @@ void FVProblem::adjoint_rhs(double omega, const std::vector<StationHCache>& cach
             if (d == 0) {
                 const bool top = grid_.edge_z[e] > cz;
                 const double we = top ? wxt : wxb;
-                rhs1[e] += sx1 * we;
-                rhs2[e] += sx2 * we;
+                add(e, sx1 * we, sx2 * we);
                 if (top && g.nxt > 0) {
                     const cplx wh = iz / double(g.nxt);
-                    rhs1[e] += aHy1 * wh; rhs2[e] += aHy2 * wh;
+                    add(e, aHy1 * wh, aHy2 * wh);
                 } else if (!top && g.nxb > 0) {
                     const cplx wh = -iz / double(g.nxb);
-                    rhs1[e] += aHy1 * wh; rhs2[e] += aHy2 * wh;
+                    add(e, aHy1 * wh, aHy2 * wh);
                 }
             } else if (d == 1) {
                 const bool top = grid_.edge_z[e] > cz;
                 const double we = top ? wyt : wyb;
-                rhs1[e] += sy1 * we;
-                rhs2[e] += sy2 * we;
+                add(e, sy1 * we, sy2 * we);
                 if (top && g.nyt > 0) {
                     const cplx wh = -iz / double(g.nyt);
-                    rhs1[e] += aHx1 * wh; rhs2[e] += aHx2 * wh;
+                    add(e, aHx1 * wh, aHx2 * wh);
                 } else if (!top && g.nyb > 0) {
                     const cplx wh = iz / double(g.nyb);
-                    rhs1[e] += aHx1 * wh; rhs2[e] += aHx2 * wh;
+                    add(e, aHx1 * wh, aHx2 * wh);
                 }
             } else {
                 const bool py = grid_.edge_y[e] > cy;
                 if (py && g.nzpy > 0) {
                     const cplx wh = iy / double(g.nzpy);
-                    rhs1[e] += aHx1 * wh; rhs2[e] += aHx2 * wh;
+                    add(e, aHx1 * wh, aHx2 * wh);
                 } else if (!py && g.nzmy > 0) {
                     const cplx wh = -iy / double(g.nzmy);
-                    rhs1[e] += aHx1 * wh; rhs2[e] += aHx2 * wh;
+                    add(e, aHx1 * wh, aHx2 * wh);
                 }
                 const bool px = grid_.edge_x[e] > cx;
                 if (px && g.nzpx > 0) {
                     const cplx wh = -ix / double(g.nzpx);
-                    rhs1[e] += aHy1 * wh; rhs2[e] += aHy2 * wh;
+                    add(e, aHy1 * wh, aHy2 * wh);
                 } else if (!px && g.nzmx > 0) {
                     const cplx wh = ix / double(g.nzmx);
-                    rhs1[e] += aHy1 * wh; rhs2[e] += aHy2 * wh;
+                    add(e, aHy1 * wh, aHy2 * wh);
                 }
             }
         }
     }
+}
+
+void FVProblem::adjoint_rhs(double omega, const std::vector<StationHCache>& cache,
+                            const std::vector<std::array<cplx, 4>>& wr, std::vector<cplx>& rhs1,
+                            std::vector<cplx>& rhs2) const {
+    const int ne = num_edges();
+    rhs1.assign(ne, cplx(0, 0));
+    rhs2.assign(ne, cplx(0, 0));
+    adjoint_rhs_core(omega, cache, wr, [&](int e, const cplx& v1, const cplx& v2) {
+        rhs1[e] += v1;
+        rhs2[e] += v2;
+    });
+}
+
+std::vector<int> FVProblem::station_edges() const {
+    std::vector<int> out;
+    for (int cell : station_cell_)
+        if (cell >= 0)
+            for (const CellEdge& ce : grid_.cell_edges[cell]) out.push_back(ce.edge_id);
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+}
+
+void FVProblem::extract_impedance_compact(double omega, const std::vector<int>& edges, const cplx* e1,
+                                          const cplx* e2, std::vector<std::array<cplx, 4>>& z,
+                                          std::vector<StationHCache>& cache) const {
+    auto pos = [&](int e) {
+        const auto it = std::lower_bound(edges.begin(), edges.end(), e);
+        if (it == edges.end() || *it != e) throw std::invalid_argument("edge outside the station edge list");
+        return static_cast<size_t>(it - edges.begin());
+    };
+    auto g1 = [&](int e) -> const cplx& { return e1[pos(e)]; };
+    auto g2 = [&](int e) -> const cplx& { return e2[pos(e)]; };
+    extract_impedance_core(omega, g1, g2, z, cache);
+}
+
+void FVProblem::adjoint_rhs_compact(double omega, const std::vector<StationHCache>& cache,
+                                    const std::vector<std::array<cplx, 4>>& wr, const std::vector<int>& edges,
+                                    cplx* rhs1, cplx* rhs2) const {
+    for (size_t k = 0; k < edges.size(); ++k) rhs1[k] = rhs2[k] = cplx(0, 0);
+    adjoint_rhs_core(omega, cache, wr, [&](int e, const cplx& v1, const cplx& v2) {
+        const auto it = std::lower_bound(edges.begin(), edges.end(), e);
+        if (it == edges.end() || *it != e) throw std::invalid_argument("edge outside the station edge list");
+        const size_t k = static_cast<size_t>(it - edges.begin());
+        rhs1[k] += v1;
+        rhs2[k] += v2;
+    });
 }
 
 // g_c = Re(i omega sum_{e in c, unconstrained} lambda_e E_e sigma_c A_dual L / n_adj)

@@ -101,6 +101,16 @@ public:
     void adjoint_rhs(double omega, const std::vector<StationHCache>& cache,
                      const std::vector<std::array<cplx, 4>>& wr, std::vector<cplx>& rhs1,
                      std::vector<cplx>& rhs2) const;
+    // The same two steps on compact arrays over the station cells' edges
+    // (station_edges(): sorted, unique), for fields that stay on the device:
+    // value k belongs to edge edges[k].  Results equal the full-vector forms.
+    std::vector<int> station_edges() const;
+    void extract_impedance_compact(double omega, const std::vector<int>& edges, const cplx* e1,
+                                   const cplx* e2, std::vector<std::array<cplx, 4>>& z,
+                                   std::vector<StationHCache>& cache) const;
+    void adjoint_rhs_compact(double omega, const std::vector<StationHCache>& cache,
+                             const std::vector<std::array<cplx, 4>>& wr, const std::vector<int>& edges,
+                             cplx* rhs1, cplx* rhs2) const;
     // Per-edge weight A_dual * L / n_adj of the sensitivity sum (0 for the
     // edges it skips), the data the device's sensitivity kernel needs.
     void sensitivity_weights(std::vector<double>& w) const;
@@ -113,6 +123,13 @@ public:
                          std::vector<std::array<cplx, 4>>& dz) const;
 
 private:
+    template <typename Get1, typename Get2>
+    void extract_impedance_core(double omega, Get1 e1, Get2 e2, std::vector<std::array<cplx, 4>>& z,
+                                std::vector<StationHCache>& cache) const;
+    template <typename Add>
+    void adjoint_rhs_core(double omega, const std::vector<StationHCache>& cache,
+                          const std::vector<std::array<cplx, 4>>& wr, Add add) const;
+
     const OctreeMesh* mesh_ = nullptr;
     ForwardParams params_;
     StaggeredGrid grid_;

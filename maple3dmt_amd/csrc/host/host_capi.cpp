@@ -64,6 +64,7 @@ struct m3dh_model {
     std::map<std::string, Array> cache;
     std::vector<std::array<cplx, 4>> last_z;
     std::vector<StationHCache> hcache;
+    std::vector<int> station_edges;  // sorted station-cell edges (m3dh_station_edges)
 
     Array build(const std::string& name) {
         const FVProblem& P = fwd.problem();
@@ -321,6 +322,7 @@ int m3dh_find_stations(m3dh_model* m, const double* xy, int n) {
         for (int i = 0; i < n; ++i) v.push_back({xy[2 * i], xy[2 * i + 1]});
         m->fwd.find_stations(v);
         m->cache.erase("station_cell");
+        m->station_edges.clear();
     });
 }
 
@@ -379,6 +381,37 @@ int m3dh_adjoint_rhs(m3dh_model* m, double omega, const double* wr, double* rhs1
         m->fwd.problem().adjoint_rhs(omega, m->hcache, w, r1, r2);
         std::memcpy(rhs1, r1.data(), r1.size() * sizeof(cplx));
         std::memcpy(rhs2, r2.data(), r2.size() * sizeof(cplx));
+    });
+}
+
+int m3dh_station_edges(m3dh_model* m, int* count, const int** edges) {
+    return guarded([&] {
+        if (!count || !edges) throw std::invalid_argument("null argument");
+        m->station_edges = m->fwd.problem().station_edges();
+        *count = static_cast<int>(m->station_edges.size());
+        *edges = m->station_edges.data();
+    });
+}
+
+int m3dh_extract_impedance_compact(m3dh_model* m, double omega, const double* e1c, const double* e2c,
+                                   double* z_out) {
+    return guarded([&] {
+        if (m->station_edges.empty()) m->station_edges = m->fwd.problem().station_edges();
+        m->fwd.problem().extract_impedance_compact(omega, m->station_edges, reinterpret_cast<const cplx*>(e1c),
+                                                   reinterpret_cast<const cplx*>(e2c), m->last_z, m->hcache);
+        m->fwd.set_station_cache(m->hcache);
+        std::memcpy(z_out, static_cast<const void*>(m->last_z.data()), m->last_z.size() * sizeof(std::array<cplx, 4>));
+    });
+}
+
+int m3dh_adjoint_rhs_compact(m3dh_model* m, double omega, const double* wr, double* rhs1c, double* rhs2c) {
+    return guarded([&] {
+        if (m->station_edges.empty()) m->station_edges = m->fwd.problem().station_edges();
+        const size_t ns = m->fwd.problem().station_cells().size();
+        std::vector<std::array<cplx, 4>> w(ns);
+        std::memcpy(static_cast<void*>(w.data()), wr, ns * sizeof(std::array<cplx, 4>));
+        m->fwd.problem().adjoint_rhs_compact(omega, m->hcache, w, m->station_edges, reinterpret_cast<cplx*>(rhs1c),
+                                             reinterpret_cast<cplx*>(rhs2c));
     });
 }
 

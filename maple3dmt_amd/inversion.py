@@ -76,9 +76,12 @@ def gradient_device(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
                     adjoint_params: Optional[SolveParams] = None) -> GradientResult:
     """gradient() with the fields resident on the device: the forward batch
     (FORWARD), the adjoint batch (INTERLEAVED) and the sensitivity sum over
-    every frequency and polarisation (m3d_sensitivity_device) run on the GPU;
-    the host sees E only for the station extraction and builds the station
-    right-hand sides.  torch tensors are the device buffers."""
+    every frequency and polarisation (m3d_sensitivity_device) run on the GPU.
+    Only the station cells' edges cross to the host: E is gathered there on
+    the device, Z and the station right-hand sides come from the compact
+    values (the reference's formulas, equal to the full-field path), and the
+    right-hand sides are scattered into zeroed device vectors.  torch
+    tensors are the device buffers."""
     import torch
     dev = torch.device("cuda:0")
     freqs = np.asarray(freqs, dtype=np.float64)
@@ -93,15 +96,18 @@ def gradient_device(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
     torch.cuda.synchronize()
     fres = model.device.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(),
                                      d_e.data_ptr(), 2 * nf, fwd_prm)
-    E = d_e.cpu().numpy()
+    idx = torch.from_numpy(model.station_edges().astype(np.int64)).to(dev)
+    Ec = d_e.index_select(1, idx).cpu().numpy()          # [2 nf, station edges]
     Z = np.zeros_like(z_obs)
-    arhs = np.zeros_like(rhs)
+    arhs_c = np.zeros_like(Ec)
     for f in range(nf):
         w = omegas[2 * f]
-        Z[f] = model.extract_impedance(w, E[2 * f], E[2 * f + 1])
-        arhs[2 * f], arhs[2 * f + 1] = model.adjoint_rhs(w, weighted_residual(z_obs[f], Z[f], z_err[f]))
+        Z[f] = model.extract_impedance_compact(w, Ec[2 * f], Ec[2 * f + 1])
+        arhs_c[2 * f], arhs_c[2 * f + 1] = model.adjoint_rhs_compact(
+            w, weighted_residual(z_obs[f], Z[f], z_err[f]))
     phi = misfit(z_obs, Z, z_err)
-    d_arhs = torch.from_numpy(arhs).to(dev)
+    d_arhs = torch.zeros_like(d_e)
+    d_arhs.index_copy_(1, idx, torch.from_numpy(arhs_c).to(dev))
     d_lam = torch.zeros_like(d_arhs)
     torch.cuda.synchronize()
     ares = model.device.solve_device(MODE_INTERLEAVED, omegas, d_arhs.data_ptr(), 0, d_lam.data_ptr(),

@@ -141,7 +141,8 @@ HOST_SYMBOLS = [
     "m3dh_get_count", "m3dh_set_sigma", "m3dh_find_stations", "m3dh_primary_field",
     "m3dh_bc_values", "m3dh_rhs_total", "m3dh_phi0", "m3dh_extract_impedance", "m3dh_adjoint_rhs",
     "m3dh_sensitivity", "m3dh_solve_frequencies", "m3dh_adjoint_solve", "m3dh_device_operator",
-    "m3dh_update_sigma_device",
+    "m3dh_update_sigma_device", "m3dh_station_edges", "m3dh_extract_impedance_compact",
+    "m3dh_adjoint_rhs_compact",
 ]
 
 
@@ -192,6 +193,9 @@ def lib() -> C.CDLL:
     L.m3dh_get_count.restype = C.c_longlong
     L.m3dh_set_sigma.argtypes = [vp, vp]
     L.m3dh_update_sigma_device.argtypes = [vp, vp]
+    L.m3dh_station_edges.argtypes = [vp, p(i), p(C.POINTER(C.c_int))]
+    L.m3dh_extract_impedance_compact.argtypes = [vp, d, vp, vp, vp]
+    L.m3dh_adjoint_rhs_compact.argtypes = [vp, d, vp, vp, vp]
     L.m3d_operator_update_sigma.argtypes = [vp, vp]
     L.m3d_operator_update_sigma_device.argtypes = [vp, vp]
     L.m3dh_find_stations.argtypes = [vp, vp, i]
@@ -602,6 +606,30 @@ class Model:
         r1 = np.zeros(self.n, np.complex128)
         r2 = np.zeros(self.n, np.complex128)
         _check_host(lib().m3dh_adjoint_rhs(self.handle, omega, _ptr(w), _ptr(r1), _ptr(r2)))
+        return r1, r2
+
+    def station_edges(self) -> np.ndarray:
+        """Sorted edges of the station cells (the compact station arrays'
+        index)."""
+        n = C.c_int()
+        ptr = C.POINTER(C.c_int)()
+        _check_host(lib().m3dh_station_edges(self.handle, C.byref(n), C.byref(ptr)))
+        return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy() if n.value else np.zeros(0, np.int32)
+
+    def extract_impedance_compact(self, omega: float, e1c, e2c) -> np.ndarray:
+        """extract_impedance from the values on station_edges() only."""
+        a, b = _as_c128(e1c), _as_c128(e2c)
+        z = np.zeros((self.count("stations"), 4), np.complex128)
+        _check_host(lib().m3dh_extract_impedance_compact(self.handle, omega, _ptr(a), _ptr(b), _ptr(z)))
+        return z
+
+    def adjoint_rhs_compact(self, omega: float, wr):
+        """adjoint_rhs on station_edges() only (the rest of both RHS is 0)."""
+        w = _as_c128(wr)
+        k = len(self.station_edges())
+        r1 = np.zeros(k, np.complex128)
+        r2 = np.zeros(k, np.complex128)
+        _check_host(lib().m3dh_adjoint_rhs_compact(self.handle, omega, _ptr(w), _ptr(r1), _ptr(r2)))
         return r1, r2
 
     def sensitivity(self, omega: float, e_bg, lam) -> np.ndarray:

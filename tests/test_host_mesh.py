@@ -167,3 +167,29 @@ def test_random_workload_mesh_is_balanced():
     assert np.abs(lev[fm[both]] - lev[fp[both]]).max() <= 1
     s = m.array("sigma")
     assert s.min() == pytest.approx(1e-8) and len(np.unique(s)) >= 3
+
+
+def test_compact_station_steps_equal_full_field_forms():
+    """Z and the adjoint right-hand sides from the values on the station
+    cells' edges only (the device-resident gradient's path) equal the
+    full-field forms bit for bit (extract_impedance_, :1490-1727;
+    build_adjoint_rhs_from_residual, :1804-2013)."""
+    wl = configs.random3d(7, min_level=3, max_level=5, n_freq=2, half_width=50e3,
+                          station_radius=5e3)
+    m = configs.build_model(wl)
+    idx = m.station_edges()
+    assert len(idx) > 0 and np.all(np.diff(idx) > 0)
+    rng = np.random.default_rng(4)
+    e1 = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    e2 = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    w = 2 * np.pi * 0.7
+    z_full = m.extract_impedance(w, e1, e2)
+    wr = rng.standard_normal(z_full.shape) + 1j * rng.standard_normal(z_full.shape)
+    r1, r2 = m.adjoint_rhs(w, wr)
+    z_c = m.extract_impedance_compact(w, e1[idx], e2[idx])
+    c1, c2 = m.adjoint_rhs_compact(w, wr)
+    assert np.array_equal(z_c, z_full)
+    assert np.array_equal(c1, r1[idx]) and np.array_equal(c2, r2[idx])
+    rest = np.ones(m.n, bool)
+    rest[idx] = False
+    assert not np.any(r1[rest]) and not np.any(r2[rest])
