@@ -19,6 +19,8 @@ under "variants", the same batch solved
                              region), when its weights exist
 each with its speed-up over the plain FP64 batch and its largest |dZ|/|Z|
 against it; and
+  gradient       one misfit gradient of the bench model (fast path): full fields
+                 through the host against the device-resident path
   sigma_update   one line-search sigma update: host recompute + upload against
                  the recompute on the device (m3d_operator_update_sigma)
   launch_shapes  the autotuner's launch shape per batch width and precision
@@ -295,6 +297,27 @@ def main():
     sigma_update = {"cells": model.nc, "host_ms": host_ms, "device_ms": dev_ms}
     log(f"[bench] sigma update: host {host_ms:.1f} ms, device {dev_ms:.2f} ms")
 
+    # Misfit gradient of the bench model (16 forward + 16 adjoint systems,
+    # fast path): full fields through the host against the device-resident
+    # path that moves only the station cells' edges.
+    gradient = None
+    if not args.no_mixed:
+        from maple3dmt_amd import inversion
+        from maple3dmt_amd.multimodel import halfspace_data
+        obs, err = halfspace_data(wl)
+        gp = wl.fwd.fast_params(dc_interval=args.dc_interval)
+        inversion.gradient_device(model, wl.freqs, obs, err, gp, gp)      # warm-up
+        t = time.perf_counter()
+        g_host = inversion.gradient(model, wl.freqs, obs, err, gp, gp)
+        t_host = time.perf_counter() - t
+        t = time.perf_counter()
+        g_dev = inversion.gradient_device(model, wl.freqs, obs, err, gp, gp)
+        t_dev = time.perf_counter() - t
+        gradient = {"systems": 4 * nf, "host_fields_s": t_host, "device_resident_s": t_dev,
+                    "station_edges": int(len(model.station_edges())),
+                    "max_rel_diff": float(np.abs(g_dev.grad - g_host.grad).max() / np.abs(g_host.grad).max())}
+        log(f"[bench] gradient: {gradient}")
+
     models = None
     if args.models > 0:
         models = multi_model_leg(args, model, ctx)
@@ -334,6 +357,7 @@ def main():
         "operator": info,
         "launch_shapes": shapes,
         "sigma_update": sigma_update,
+        "gradient": gradient,
         "variants": legs,
         "multi_model": models,
         "cpu_baseline": cpu,

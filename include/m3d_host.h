@@ -106,6 +106,18 @@ int m3dh_sensitivity(m3dh_model* m, double omega, const double* e_bg, const doub
  * the full fields, and the right-hand sides equal m3dh_adjoint_rhs's on the
  * listed edges (zero elsewhere). */
 int m3dh_station_edges(m3dh_model* m, int* count, const int** edges);
+
+/* GN-CG J*v pieces (ImplicitJtJOperatorFV::Mult, implicit_jtj_fv.cpp:78-192):
+ * the perturbation right-hand side -i omega sum_c dMe/dsigma_c dsigma_c E_bg
+ * with the reference's harmonic-mean L^3 mass derivative
+ * (build_perturbation_rhs, forward_solver_fv.cpp:945-984; dsigma in sigma
+ * space, [cells]), and delta Z at the stations from delta E on the station
+ * edges (extract_delta_impedance_complex, :2018-2065; polarization 1 or 2;
+ * uses the station H cache of the last impedance extraction). */
+int m3dh_perturbation_rhs(m3dh_model* m, double omega, const double* e_bg_c128, const double* dsigma,
+                          double* rhs_c128);
+int m3dh_delta_impedance_compact(m3dh_model* m, const double* dec_c128, int polarization,
+                                 double* dz_out);
 int m3dh_extract_impedance_compact(m3dh_model* m, double omega, const double* e1c_c128,
                                    const double* e2c_c128, double* z_out);
 int m3dh_adjoint_rhs_compact(m3dh_model* m, double omega, const double* wr_c128, double* rhs1c,

@@ -504,9 +504,9 @@ void FVProblem::perturbation_rhs(double omega, const std::vector<cplx>& e_bg,
     }
 }
 
-void FVProblem::delta_impedance(const std::vector<cplx>& de, int polarization,
-                                const std::vector<StationHCache>& cache,
-                                std::vector<std::array<cplx, 4>>& dz) const {
+template <typename Get>
+void FVProblem::delta_impedance_core(Get de, int polarization, const std::vector<StationHCache>& cache,
+                                     std::vector<std::array<cplx, 4>>& dz) const {
     const int ns = static_cast<int>(station_cell_.size());
     dz.assign(ns, {cplx(0, 0), cplx(0, 0), cplx(0, 0), cplx(0, 0)});
     for (int s = 0; s < ns; ++s) {
@@ -516,8 +516,8 @@ void FVProblem::delta_impedance(const std::vector<cplx>& de, int polarization,
         int nx = 0, ny = 0;
         for (const CellEdge& ce : grid_.cell_edges[cell]) {
             const int e = ce.edge_id;
-            if (grid_.edge_dir[e] == 0) { dex += de[e]; ++nx; }
-            else if (grid_.edge_dir[e] == 1) { dey += de[e]; ++ny; }
+            if (grid_.edge_dir[e] == 0) { dex += de(e); ++nx; }
+            else if (grid_.edge_dir[e] == 1) { dey += de(e); ++ny; }
         }
         if (nx > 0) dex /= cplx(nx, 0);
         if (ny > 0) dey /= cplx(ny, 0);
@@ -530,6 +530,24 @@ void FVProblem::delta_impedance(const std::vector<cplx>& de, int polarization,
             dz[s] = {dex * (-hc.Hy1) * inv, dex * hc.Hx1 * inv, dey * (-hc.Hy1) * inv, dey * hc.Hx1 * inv};
         }
     }
+}
+
+void FVProblem::delta_impedance(const std::vector<cplx>& de, int polarization,
+                                const std::vector<StationHCache>& cache,
+                                std::vector<std::array<cplx, 4>>& dz) const {
+    delta_impedance_core([&](int e) -> const cplx& { return de[e]; }, polarization, cache, dz);
+}
+
+void FVProblem::delta_impedance_compact(const std::vector<int>& edges, const cplx* de, int polarization,
+                                        const std::vector<StationHCache>& cache,
+                                        std::vector<std::array<cplx, 4>>& dz) const {
+    delta_impedance_core(
+        [&](int e) -> const cplx& {
+            const auto it = std::lower_bound(edges.begin(), edges.end(), e);
+            if (it == edges.end() || *it != e) throw std::invalid_argument("edge outside the station edge list");
+            return de[it - edges.begin()];
+        },
+        polarization, cache, dz);
 }
 
 }  // namespace m3d

@@ -121,11 +121,17 @@ public:
     void delta_impedance(const std::vector<cplx>& de, int polarization,
                          const std::vector<StationHCache>& cache,
                          std::vector<std::array<cplx, 4>>& dz) const;
+    void delta_impedance_compact(const std::vector<int>& edges, const cplx* de, int polarization,
+                                 const std::vector<StationHCache>& cache,
+                                 std::vector<std::array<cplx, 4>>& dz) const;
 
 private:
     template <typename Get1, typename Get2>
     void extract_impedance_core(double omega, Get1 e1, Get2 e2, std::vector<std::array<cplx, 4>>& z,
                                 std::vector<StationHCache>& cache) const;
+    template <typename Get>
+    void delta_impedance_core(Get de, int polarization, const std::vector<StationHCache>& cache,
+                              std::vector<std::array<cplx, 4>>& dz) const;
     template <typename Add>
     void adjoint_rhs_core(double omega, const std::vector<StationHCache>& cache,
                           const std::vector<std::array<cplx, 4>>& wr, Add add) const;

@@ -415,6 +415,28 @@ int m3dh_adjoint_rhs_compact(m3dh_model* m, double omega, const double* wr, doub
     });
 }
 
+int m3dh_perturbation_rhs(m3dh_model* m, double omega, const double* e_bg, const double* dsigma,
+                          double* rhs) {
+    return guarded([&] {
+        const int n = m->fwd.problem().num_edges(), nc = m->fwd.problem().num_cells();
+        const cplx* a = reinterpret_cast<const cplx*>(e_bg);
+        std::vector<cplx> e(a, a + n), r;
+        m->fwd.problem().perturbation_rhs(omega, e, std::vector<double>(dsigma, dsigma + nc), r);
+        std::memcpy(rhs, r.data(), r.size() * sizeof(cplx));
+    });
+}
+
+int m3dh_delta_impedance_compact(m3dh_model* m, const double* dec, int polarization, double* dz_out) {
+    return guarded([&] {
+        if (polarization != 1 && polarization != 2) throw std::invalid_argument("polarization must be 1 or 2");
+        if (m->station_edges.empty()) m->station_edges = m->fwd.problem().station_edges();
+        std::vector<std::array<cplx, 4>> dz;
+        m->fwd.problem().delta_impedance_compact(m->station_edges, reinterpret_cast<const cplx*>(dec),
+                                                 polarization, m->hcache, dz);
+        std::memcpy(dz_out, static_cast<const void*>(dz.data()), dz.size() * sizeof(std::array<cplx, 4>));
+    });
+}
+
 int m3dh_sensitivity(m3dh_model* m, double omega, const double* e_bg, const double* lambda, double* sens) {
     return guarded([&] {
         const int n = m->fwd.problem().num_edges();

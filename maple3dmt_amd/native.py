@@ -142,7 +142,7 @@ HOST_SYMBOLS = [
     "m3dh_bc_values", "m3dh_rhs_total", "m3dh_phi0", "m3dh_extract_impedance", "m3dh_adjoint_rhs",
     "m3dh_sensitivity", "m3dh_solve_frequencies", "m3dh_adjoint_solve", "m3dh_device_operator",
     "m3dh_update_sigma_device", "m3dh_station_edges", "m3dh_extract_impedance_compact",
-    "m3dh_adjoint_rhs_compact",
+    "m3dh_adjoint_rhs_compact", "m3dh_perturbation_rhs", "m3dh_delta_impedance_compact",
 ]
 
 
@@ -196,6 +196,8 @@ def lib() -> C.CDLL:
     L.m3dh_station_edges.argtypes = [vp, p(i), p(C.POINTER(C.c_int))]
     L.m3dh_extract_impedance_compact.argtypes = [vp, d, vp, vp, vp]
     L.m3dh_adjoint_rhs_compact.argtypes = [vp, d, vp, vp, vp]
+    L.m3dh_perturbation_rhs.argtypes = [vp, d, vp, vp, vp]
+    L.m3dh_delta_impedance_compact.argtypes = [vp, vp, i, vp]
     L.m3d_operator_update_sigma.argtypes = [vp, vp]
     L.m3d_operator_update_sigma_device.argtypes = [vp, vp]
     L.m3dh_find_stations.argtypes = [vp, vp, i]
@@ -631,6 +633,24 @@ class Model:
         r2 = np.zeros(k, np.complex128)
         _check_host(lib().m3dh_adjoint_rhs_compact(self.handle, omega, _ptr(w), _ptr(r1), _ptr(r2)))
         return r1, r2
+
+    def perturbation_rhs(self, omega: float, e_bg, dsigma) -> np.ndarray:
+        """build_perturbation_rhs: -i w sum_c dMe/dsigma_c dsigma_c E_bg with the
+        reference's harmonic-mean L^3 mass derivative (dsigma in sigma space)."""
+        e = _as_c128(e_bg)
+        ds = np.ascontiguousarray(dsigma, dtype=np.float64)
+        assert ds.shape == (self.nc,)
+        out = np.zeros(self.n, np.complex128)
+        _check_host(lib().m3dh_perturbation_rhs(self.handle, omega, _ptr(e), _ptr(ds), _ptr(out)))
+        return out
+
+    def delta_impedance_compact(self, dec, polarization: int) -> np.ndarray:
+        """extract_delta_impedance_complex from delta E on station_edges()
+        (polarization 1 or 2; the H cache of the last impedance extraction)."""
+        a = _as_c128(dec)
+        dz = np.zeros((self.count("stations"), 4), np.complex128)
+        _check_host(lib().m3dh_delta_impedance_compact(self.handle, _ptr(a), polarization, _ptr(dz)))
+        return dz
 
     def sensitivity(self, omega: float, e_bg, lam) -> np.ndarray:
         a, b = _as_c128(e_bg), _as_c128(lam)
