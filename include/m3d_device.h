@@ -126,6 +126,22 @@ int m3d_operator_set_sensitivity(m3d_operator* op, const m3d_sensitivity_desc* d
 int m3d_sensitivity_device(m3d_operator* op, int nsys, const double* omega, const double* d_E,
                            const double* d_lambda, double* d_grad);
 
+/* Perturbation right-hand side of the Gauss-Newton product J*v
+ * (build_perturbation_rhs, forward_solver_fv.cpp:945-984) on the device:
+ * for each system s
+ *   rhs_s[e] = -sum_{cells c containing e} i omega_s dMe_ce dsigma_c E_bg,s[e],
+ *   dMe_ce   = L_e^3 savg_e^2 / (n_adj(e) sigma_c^2)
+ * with savg_e the harmonic mean of sigma over the edge's adjacent cells (the
+ * reference's mass derivative; it equals the derivative of the edge mass
+ * only in a uniform medium on a uniform mesh), the current sigma of the
+ * sensitivity, and the reference's operation order, so the values equal the
+ * host's bit for bit.  d_ebg, d_rhs: device complex128 [nsys][n] (caller
+ * order); d_dsigma: device double [cells], sigma space.  Needs
+ * m3d_operator_set_mass_structure and m3d_operator_set_sensitivity.  Returns
+ * after the context stream drains. */
+int m3d_perturbation_rhs_device(m3d_operator* op, int nsys, const double* omega, const double* d_ebg,
+                                const double* d_dsigma, double* d_rhs);
+
 /* Mirrors the solver fields of ForwardParamsFV (forward_solver_fv.h:31-69). */
 typedef struct {
     double tol;              /* bicgstab_tol */

@@ -81,6 +81,10 @@ struct Launch64 {
 };
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol);
+// Gauss-Newton perturbation right-hand side, caller order [nsys][n].
+void launch_perturbation_rhs(hipStream_t s, int n, int nsys, int nc, const int* e_ptr, const int* e_cells,
+                             const int* adj_ptr, const int* adj_cell, const double* len, const double* sigma,
+                             const double* dsig, const double* omega, const double2* ebg, double2* rhs);
 // Sigma update: per device row the operator's Me (FP64, FP32) and the
 // DivCorr weight; per node the Laplacian values and Jacobi diagonal.
 void launch_sigma_edges(hipStream_t s, int npad, int nc, const int* perm, const int* adj_ptr,

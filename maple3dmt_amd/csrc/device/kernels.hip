@@ -1215,6 +1215,62 @@ __global__ void k_sigma_laplacian(int nn, const int* __restrict__ nt_ptr, const 
     }
 }
 
+// Perturbation right-hand side of the Gauss-Newton product
+// (build_perturbation_rhs, forward_solver_fv.cpp:945-984) for edge e of
+// system s (caller order, t = s * n + e):
+//   rhs = -sum over cells c containing e with |dsig_c| >= 1e-30 of
+//         (i w)(dsig_c dMe_ce) E_bg[e],  dMe_ce = L^3 savg^2 / (n_adj sigma_c^2),
+// savg the harmonic mean of sigma over the edge's adjacent cells.  Cells in
+// ascending order (e_cells), complex products written out in the host's
+// operation order, nothing contracted: equal to the host's values.
+__global__ void k_perturbation_rhs(int n, int nsys, int nc, const int* __restrict__ e_ptr,
+                                   const int* __restrict__ e_cells, const int* __restrict__ adj_ptr,
+                                   const int* __restrict__ adj_cell, const double* __restrict__ len,
+                                   const double* __restrict__ sigma, const double* __restrict__ dsig,
+                                   const double* __restrict__ omega, const double2* __restrict__ ebg,
+                                   double2* __restrict__ rhs) {
+#pragma clang fp contract(off)
+    const size_t total = size_t(n) * nsys;
+    for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int s = int(t / n), e = int(t % n);
+        double ax = 0.0, ay = 0.0;
+        const int a = e_ptr[e], b = e_ptr[e + 1];
+        if (a < b) {
+            double sum_inv = 0.0;
+            int n_adj = 0;
+            for (int k = adj_ptr[e]; k < adj_ptr[e + 1]; ++k) {
+                const int c = adj_cell[k];
+                if (c >= 0 && c < nc && sigma[c] > 1e-30) {
+                    sum_inv += 1.0 / sigma[c];
+                    ++n_adj;
+                }
+            }
+            const double savg = (n_adj > 0 && sum_inv > 1e-30) ? n_adj / sum_inv : 1e-8;
+            const double L = len[e];
+            const double vol = L * L * L;
+            const double w = omega[s];
+            const double2 E = ebg[t];
+            for (int k = a; k < b; ++k) {
+                const int c = e_cells[k];
+                const double ds = dsig[c];
+                if (fabs(ds) < 1e-30) continue;
+                const double sc = sigma[c];
+                const double dme = vol * savg * savg / (n_adj * sc * sc);
+                const double av = ds * dme;
+                // (i w) * (av + 0i), then * E
+                const double t1r = 0.0 * av - w * 0.0;
+                const double t1i = 0.0 * 0.0 + w * av;
+                const double pr = t1r * E.x - t1i * E.y;
+                const double pi = t1r * E.y + t1i * E.x;
+                ax -= pr;
+                ay -= pi;
+            }
+        }
+        rhs[t] = mk(ax, ay);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers (explicit instantiation per batch width).
 // ---------------------------------------------------------------------------
@@ -1491,6 +1547,13 @@ void launch_sigma_laplacian(hipStream_t s, int nn, const int* nt_ptr, const int*
                             double* ldiag) {
     hipLaunchKernelGGL(k_sigma_laplacian, dim3(grid_for(size_t(nn), kStreamCap)), dim3(kBlock), 0, s, nn, nt_ptr,
                        nt_edge, nt_val, ms, lptr, lcol, ledge, g_col, g_val, bc_node, lval, ldiag);
+}
+
+void launch_perturbation_rhs(hipStream_t s, int n, int nsys, int nc, const int* e_ptr, const int* e_cells,
+                             const int* adj_ptr, const int* adj_cell, const double* len, const double* sigma,
+                             const double* dsig, const double* omega, const double2* ebg, double2* rhs) {
+    hipLaunchKernelGGL(k_perturbation_rhs, dim3(grid_for(size_t(n) * nsys, kStreamCap)), dim3(kBlock), 0, s, n,
+                       nsys, nc, e_ptr, e_cells, adj_ptr, adj_cell, len, sigma, dsig, omega, ebg, rhs);
 }
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol) {

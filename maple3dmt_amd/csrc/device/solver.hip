@@ -225,6 +225,7 @@ struct m3d_operator {
     // Sensitivity arrays.
     int nc = 0;
     DBuf<int> s_cell_ptr, s_cell_edges;
+    DBuf<int> s_edge_ptr, s_edge_cells;   // edge -> cells containing it, ascending
     DBuf<double> s_weight, s_sigma, s_omega;
     // Launch shapes by precision (0 = complex128, 1 = complex64) and width.
     LaunchShape shape[2][kWidths];
@@ -1529,10 +1530,40 @@ int m3d_operator_set_sensitivity(m3d_operator* op, const m3d_sensitivity_desc* d
         const std::vector<double> sg(desc->sigma, desc->sigma + nc);
         op->s_cell_ptr.upload(ptr, s);
         op->s_cell_edges.upload(edges, s);
+        // The transpose: per edge, the cells whose edge lists hold it, in
+        // ascending cell order (with multiplicity).
+        std::vector<int> eptr(op->n + 1, 0);
+        for (int k = 0; k < total; ++k) eptr[edges[k] + 1]++;
+        for (int e = 0; e < op->n; ++e) eptr[e + 1] += eptr[e];
+        std::vector<int> ecells(std::max(total, 1), 0), epos(eptr.begin(), eptr.end() - 1);
+        for (int c = 0; c < nc; ++c)
+            for (int k = ptr[c]; k < ptr[c + 1]; ++k) ecells[epos[edges[k]]++] = c;
+        op->s_edge_ptr.upload(eptr, s);
+        op->s_edge_cells.upload(ecells, s);
         op->s_weight.upload(w, s);
         op->s_sigma.upload(sg, s);
         HIPCHK(hipStreamSynchronize(s));
         op->nc = nc;
+    });
+}
+
+int m3d_perturbation_rhs_device(m3d_operator* op, int nsys, const double* omega, const double* d_ebg,
+                                const double* d_dsigma, double* d_rhs) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        if (op->nc <= 0) throw Fail(M3D_ERR_STATE, "m3d_operator_set_sensitivity has not been called");
+        if (!op->mass_struct) throw Fail(M3D_ERR_STATE, "m3d_operator_set_mass_structure has not been called");
+        if (op->mass_nc != op->nc) throw Fail(M3D_ERR_STATE, "mass structure and sensitivity disagree on cells");
+        if (nsys <= 0 || !omega || !d_ebg || !d_dsigma || !d_rhs) throw Fail(M3D_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        const std::vector<double> om(omega, omega + nsys);
+        op->s_omega.upload(om, s);
+        launch_perturbation_rhs(s, op->n, nsys, op->nc, op->s_edge_ptr.p, op->s_edge_cells.p, op->m_adj_ptr.p,
+                                op->m_adj_cell.p, op->m_len.p, op->s_sigma.p, d_dsigma, op->s_omega.p,
+                                reinterpret_cast<const double2*>(d_ebg), reinterpret_cast<double2*>(d_rhs));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
     });
 }
 

@@ -20,10 +20,9 @@ Mult belongs to the inversion driver, which is not part of this path.
 
 Instead of the reference's loop over frequencies and polarisations, every
 perturbation system runs in one device batch and every adjoint system in a
-second; the fields stay on the device and only the station cells' edges
-cross to the host (for delta Z and the adjoint right-hand sides).  The
-perturbation right-hand sides are formed on the host from a host copy of the
-background fields (one download when the operator is built).
+second.  The fields stay on the device (the perturbation right-hand sides
+are formed there, m3d_perturbation_rhs_device); only the station cells'
+edges cross to the host, for delta Z and the adjoint right-hand sides.
 """
 from __future__ import annotations
 
@@ -59,19 +58,19 @@ class GaussNewton:
         torch.cuda.synchronize()
         self.background = model.device.solve_device(MODE_FORWARD, self.omegas, d_rhs.data_ptr(),
                                                     d_aux.data_ptr(), self.d_ebg.data_ptr(), 2 * nf, fwd)
-        self.ebg = self.d_ebg.cpu().numpy()
         self.idx = torch.from_numpy(model.station_edges().astype(np.int64)).to(self.dev)
-        self.ebg_c = self.ebg[:, model.station_edges()]
+        self.ebg_c = self.d_ebg.index_select(1, self.idx).cpu().numpy()
         self.sigma = model.array("sigma").copy()
 
     def _perturbation_fields(self, x):
         """delta E of every (frequency, polarisation) system, on the device."""
         torch = self.torch
-        ds = self.sigma * np.asarray(x, dtype=np.float64)
-        prhs = np.stack([self.model.perturbation_rhs(w, self.ebg[k], ds) for k, w in enumerate(self.omegas)])
-        d_prhs = torch.from_numpy(prhs).to(self.dev)
+        d_ds = torch.from_numpy(self.sigma * np.asarray(x, dtype=np.float64)).to(self.dev)
+        d_prhs = torch.empty_like(self.d_ebg)
         d_de = torch.zeros_like(d_prhs)
         torch.cuda.synchronize()
+        self.model.device.perturbation_rhs_device(self.omegas, self.d_ebg.data_ptr(), d_ds.data_ptr(),
+                                                  d_prhs.data_ptr())
         res = self.model.device.solve_device(MODE_INTERLEAVED, self.omegas, d_prhs.data_ptr(), 0,
                                              d_de.data_ptr(), len(self.omegas), self.inner)
         return d_de, res

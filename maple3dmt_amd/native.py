@@ -134,7 +134,7 @@ DEVICE_SYMBOLS = [
     "m3d_operator_permutation", "m3d_operator_info", "m3d_context_stream", "m3d_bench_kernel",
     "m3d_operator_set_sensitivity", "m3d_sensitivity_device", "m3d_operator_get_shape",
     "m3d_operator_set_shape", "m3d_operator_autotune", "m3d_operator_set_mass_structure",
-    "m3d_operator_update_sigma", "m3d_operator_update_sigma_device",
+    "m3d_operator_update_sigma", "m3d_operator_update_sigma_device", "m3d_perturbation_rhs_device",
 ]
 HOST_SYMBOLS = [
     "m3dh_last_error", "m3dh_model_create", "m3dh_model_destroy", "m3dh_get_array",
@@ -183,6 +183,7 @@ def lib() -> C.CDLL:
     L.m3d_operator_info.argtypes = [vp, p(i), p(i), p(C.c_longlong), p(C.c_longlong)]
     L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d)]
     L.m3d_sensitivity_device.argtypes = [vp, i, vp, vp, vp, vp]
+    L.m3d_perturbation_rhs_device.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_operator_get_shape.argtypes = [vp, i, i, p(LaunchShapeC)]
     L.m3d_operator_set_shape.argtypes = [vp, i, i, p(LaunchShapeC)]
     L.m3d_operator_autotune.argtypes = [vp, i, vp, i, vp]
@@ -416,6 +417,13 @@ class DeviceOperator:
                 slot = int(w).bit_length() - 1
                 out[(prec, int(w))] = (float(best[prec * 6 + slot]), self.get_shape(prec, int(w)))
         return out
+
+    def perturbation_rhs_device(self, omegas, d_ebg: int, d_dsigma: int, d_rhs: int) -> None:
+        """m3d_perturbation_rhs_device: device pointers, E_bg and rhs complex128
+        [nsys][n], dsigma float64 [cells]."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        _check_dev(lib().m3d_perturbation_rhs_device(self.handle, om.shape[0], _ptr(om), C.c_void_p(d_ebg),
+                                                     C.c_void_p(d_dsigma), C.c_void_p(d_rhs)))
 
     def sensitivity_device(self, omegas, d_e: int, d_lambda: int, d_grad: int) -> None:
         """d_grad[c] = sum over systems of Re(i w sigma_c sum_e lambda E w_e),

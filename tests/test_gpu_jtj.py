@@ -68,6 +68,28 @@ def test_perturbation_field_matches_central_difference(ctx):
         assert err < 1e-5
 
 
+def test_device_perturbation_rhs_matches_host(ctx):
+    """m3d_perturbation_rhs_device equals the host build_perturbation_rhs
+    restatement bit for bit (heterogeneous model, random fields and dsigma
+    with zeros)."""
+    import torch
+    dev = torch.device("cuda:0")
+    wl = configs.random3d(8, min_level=3, max_level=5, n_freq=2, half_width=50e3, station_radius=5e3)
+    m = configs.build_model(wl, ctx)
+    rng = np.random.default_rng(9)
+    e = rng.standard_normal((3, m.n)) + 1j * rng.standard_normal((3, m.n))
+    ds = rng.standard_normal(m.nc) * m.array("sigma")
+    ds[rng.uniform(size=m.nc) < 0.3] = 0.0
+    om = np.array([2 * np.pi * 0.01, 2 * np.pi * 1.0, 2 * np.pi * 30.0])
+    ref = np.stack([m.perturbation_rhs(w, e[k], ds) for k, w in enumerate(om)])
+    d_e = torch.from_numpy(e).to(dev)
+    d_ds = torch.from_numpy(ds).to(dev)
+    d_r = torch.empty_like(d_e)
+    torch.cuda.synchronize()
+    m.device.perturbation_rhs_device(om, d_e.data_ptr(), d_ds.data_ptr(), d_r.data_ptr())
+    assert np.array_equal(d_r.cpu().numpy(), ref)
+
+
 def test_batched_product_matches_reference_order(ctx):
     wl = configs.random3d(7, min_level=3, max_level=5, n_freq=2, half_width=50e3, station_radius=5e3)
     m = configs.build_model(wl, ctx)
