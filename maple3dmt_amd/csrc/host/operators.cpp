@@ -1,5 +1,6 @@
 // Discrete operators; see operators.h for the reference mapping.
 #include "operators.h"
+#include "flat_map.h"
 
 #include <algorithm>
 #include <cmath>
@@ -255,24 +256,17 @@ void DiscreteOperators::build_node_gradient() {
         bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
     };
     struct KeyHash {
-        size_t operator()(const Key& k) const {
-            uint64_t h = uint64_t(k.x) * 0x9E3779B97F4A7C15ull;
-            h ^= uint64_t(k.y) * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
-            h ^= uint64_t(k.z) * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
-            return static_cast<size_t>(h);
+        uint64_t operator()(const Key& k) const {
+            return mix64(uint64_t(k.x) ^ mix64(uint64_t(k.y) ^ mix64(uint64_t(k.z))));
         }
     };
-    std::unordered_map<Key, int, KeyHash> nodes;
-    nodes.reserve(size_t(ne_) / 2 + 16);
+    FlatMap<Key, KeyHash> nodes(size_t(ne_) / 2 + 16);
     std::vector<Key> keys;
     auto node_of = [&](double x, double y, double z) {
         const Key k{q(x), q(y), q(z)};
-        auto it = nodes.find(k);
-        if (it != nodes.end()) return it->second;
-        const int id = static_cast<int>(keys.size());
-        nodes.emplace(k, id);
-        keys.push_back(k);
-        return id;
+        const auto r = nodes.insert(k, static_cast<int>(keys.size()));
+        if (r.second) keys.push_back(k);
+        return r.first;
     };
     std::vector<int> n1(ne_), n2(ne_);
     for (int e = 0; e < ne_; ++e) {

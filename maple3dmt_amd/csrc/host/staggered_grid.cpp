@@ -4,7 +4,8 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
-#include <unordered_map>
+
+#include "flat_map.h"
 
 namespace m3d {
 
@@ -59,22 +60,17 @@ void StaggeredGrid::build(const OctreeMesh& mesh) {
 
     edge_x.clear(); edge_y.clear(); edge_z.clear(); edge_len.clear();
     edge_dir.clear(); edge_adj.clear();
-    std::unordered_map<uint64_t, int> edge_map;
-    edge_map.reserve(size_t(num_cells_) * 4);
+    FlatMap<uint64_t> edge_map(size_t(num_cells_) * 4);
 
     auto add_edge = [&](double x, double y, double z, int dir, double len, int c) {
-        const uint64_t k = qz.key(x, y, z, dir);
-        auto it = edge_map.find(k);
-        int id;
-        if (it == edge_map.end()) {
-            id = num_edges();
-            edge_map.emplace(k, id);
+        const auto r = edge_map.insert(qz.key(x, y, z, dir), num_edges());
+        const int id = r.first;
+        if (r.second) {
             edge_x.push_back(x); edge_y.push_back(y); edge_z.push_back(z);
             edge_dir.push_back(static_cast<int8_t>(dir));
             edge_len.push_back(len);
             edge_adj.emplace_back();
-        } else {
-            id = it->second;
+            edge_adj.back().reserve(4);
         }
         edge_adj[id].push_back(c);
     };
@@ -97,8 +93,7 @@ void StaggeredGrid::build(const OctreeMesh& mesh) {
     // edges, staggered_grid.cpp:128-224.
     face_x.clear(); face_y.clear(); face_z.clear(); face_area.clear();
     face_normal.clear(); face_minus.clear(); face_plus.clear(); face_edges.clear();
-    std::unordered_map<uint64_t, int> face_map;
-    face_map.reserve(size_t(num_cells_) * 4);
+    FlatMap<uint64_t> face_map(size_t(num_cells_) * 4);
 
     struct ERef { double x, y, z; int dir; };
     for (int c = 0; c < num_cells_; ++c) {
@@ -127,25 +122,20 @@ void StaggeredGrid::build(const OctreeMesh& mesh) {
         };
         for (int f = 0; f < 6; ++f) {
             const FDef& d = fd[f];
-            const uint64_t k = qz.key(d.x, d.y, d.z, d.normal);
-            auto it = face_map.find(k);
-            int fid;
-            if (it == face_map.end()) {
-                fid = num_faces();
-                face_map.emplace(k, fid);
+            const auto r = face_map.insert(qz.key(d.x, d.y, d.z, d.normal), num_faces());
+            const int fid = r.first;
+            if (r.second) {
                 std::vector<int> eids;
                 eids.reserve(4);
                 for (int e = 0; e < 4; ++e) {
-                    auto ei = edge_map.find(qz.key(d.e[e].x, d.e[e].y, d.e[e].z, d.e[e].dir));
-                    if (ei != edge_map.end()) eids.push_back(ei->second);
+                    const int ei = edge_map.find(qz.key(d.e[e].x, d.e[e].y, d.e[e].z, d.e[e].dir));
+                    if (ei >= 0) eids.push_back(ei);
                 }
                 face_x.push_back(d.x); face_y.push_back(d.y); face_z.push_back(d.z);
                 face_normal.push_back(static_cast<int8_t>(d.normal));
                 face_area.push_back(d.area);
                 face_minus.push_back(-1); face_plus.push_back(-1);
                 face_edges.push_back(std::move(eids));
-            } else {
-                fid = it->second;
             }
             if (f == 0 || f == 2 || f == 4) face_minus[fid] = c;
             else face_plus[fid] = c;
@@ -168,13 +158,11 @@ void StaggeredGrid::detect_hanging(const OctreeMesh& mesh) {
                                    qz.dz / (1 << par.max_level)});
     const double btol = min_h * 0.01;
 
-    std::unordered_map<uint64_t, int> face_lk, edge_lk;
-    face_lk.reserve(size_t(num_faces()) * 2);
-    edge_lk.reserve(size_t(num_edges()) * 2);
-    for (int f = 0; f < num_faces(); ++f)
-        face_lk[qz.key(face_x[f], face_y[f], face_z[f], face_normal[f])] = f;
-    for (int e = 0; e < num_edges(); ++e)
-        edge_lk[qz.key(edge_x[e], edge_y[e], edge_z[e], edge_dir[e])] = e;
+    // Faces and edges have distinct keys (they were enumerated by them).
+    FlatMap<uint64_t> face_lk(static_cast<size_t>(num_faces()));
+    FlatMap<uint64_t> edge_lk(static_cast<size_t>(num_edges()));
+    for (int f = 0; f < num_faces(); ++f) face_lk.insert(qz.key(face_x[f], face_y[f], face_z[f], face_normal[f]), f);
+    for (int e = 0; e < num_edges(); ++e) edge_lk.insert(qz.key(edge_x[e], edge_y[e], edge_z[e], edge_dir[e]), e);
 
     std::vector<char> face_done(num_faces(), 0);
     std::vector<char> edge_hanging(num_edges(), 0);
@@ -203,9 +191,9 @@ void StaggeredGrid::detect_hanging(const OctreeMesh& mesh) {
                 if (n == 0) { sy += s1 * fhy; sz += s2 * fhz; }
                 else if (n == 1) { sx += s1 * fhx; sz += s2 * fhz; }
                 else { sx += s1 * fhx; sy += s2 * fhy; }
-                auto it = face_lk.find(qz.key(sx, sy, sz, n));
-                if (it == face_lk.end() || face_done[it->second]) { found = false; break; }
-                subs[idx++] = it->second;
+                const int ff = face_lk.find(qz.key(sx, sy, sz, n));
+                if (ff < 0 || face_done[ff]) { found = false; break; }
+                subs[idx++] = ff;
             }
         }
         if (!found) continue;
@@ -237,10 +225,10 @@ void StaggeredGrid::detect_hanging(const OctreeMesh& mesh) {
             double b[3] = {edge_x[eid], edge_y[eid], edge_z[eid]};
             a[d] -= ql;
             b[d] += ql;
-            auto i1 = edge_lk.find(qz.key(a[0], a[1], a[2], d));
-            auto i2 = edge_lk.find(qz.key(b[0], b[1], b[2], d));
-            if (i1 != edge_lk.end() && i2 != edge_lk.end()) {
-                hanging_edges.push_back({eid, {i1->second, i2->second}});
+            const int i1 = edge_lk.find(qz.key(a[0], a[1], a[2], d));
+            const int i2 = edge_lk.find(qz.key(b[0], b[1], b[2], d));
+            if (i1 >= 0 && i2 >= 0) {
+                hanging_edges.push_back({eid, {i1, i2}});
                 edge_hanging[eid] = 1;
             }
         }
