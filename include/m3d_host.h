@@ -131,6 +131,31 @@ int m3dh_adjoint_solve(m3dh_model* m, const double* omegas, int nsys, const doub
 /* Device operator of the model (NULL for host-only models). */
 m3d_operator* m3dh_device_operator(m3dh_model* m);
 
+/* EDI files (io::read_edi / write_edi / load_edi_directory /
+ * save_edi_directory, src/io/edi_io.cpp:156-619): one station per file,
+ * impedances in ohm (EDI mV/km/nT times 4 pi 1e-4), errors from the EDI
+ * variances, weight 0 for a component the file lacks.  Arrays, for ns
+ * stations and nf frequencies (index f * ns + s for the directory calls):
+ *   names [ns][name_len] chars, geo [ns][4] = {lat, lon, elev, has_geo},
+ *   freqs [nf], z [..][4] complex (Zxx, Zxy, Zyx, Zyy), z_err [..][4],
+ *   z_w [..][4], t [..][2] complex (Tx, Ty), t_err [..][2], t_w [..][2].
+ * Reading calls fill the counts always and the arrays that are not NULL, so
+ * a first call with NULL arrays sizes the second.  Directory loading takes
+ * the union of the stations' frequencies, highest first, with weight 0 where
+ * a station lacks a frequency; t, t_err, t_w may be NULL on writes (no
+ * tipper). */
+int m3dh_edi_read(const char* path, int* nfreq, char* name, int name_len, double* geo, double* freqs,
+                  double* z, double* z_err, double* z_w, double* t, double* t_err, double* t_w);
+int m3dh_edi_write(const char* path, const char* name, const double* geo, int nfreq, const double* freqs,
+                   const double* z, const double* z_err, const double* z_w, const double* t,
+                   const double* t_err, const double* t_w);
+int m3dh_edi_load_directory(const char* dir, int* ns, int* nf, char* names, int name_len, double* geo,
+                            double* freqs, double* z, double* z_err, double* z_w, double* t,
+                            double* t_err, double* t_w);
+int m3dh_edi_save_directory(const char* dir, int ns, int nf, const char* names, int name_len,
+                            const double* geo, const double* freqs, const double* z, const double* z_err,
+                            const double* z_w, const double* t, const double* t_err, const double* t_w);
+
 #ifdef __cplusplus
 }
 #endif

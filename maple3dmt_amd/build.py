@@ -27,6 +27,7 @@ HOST_SOURCES = [
     "host/fv_problem.cpp",
     "host/forward_solver_gpu.cpp",
     "host/host_capi.cpp",
+    "host/edi_io.cpp",
 ]
 DEVICE_SOURCES = [
     "device/kernels.hip",

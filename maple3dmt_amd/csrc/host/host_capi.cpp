@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "edi_io.h"
 #include "forward_solver_gpu.h"
 #include "m3d_host.h"
 
@@ -501,6 +502,143 @@ int m3dh_adjoint_solve(m3dh_model* m, const double* omegas, int nsys, const doub
         std::memset(x_out, 0, sizeof(double) * 2 * size_t(nsys) * n);
         if (m3d_solve_batch(op, nsys, M3D_MODE_INTERLEAVED, omegas, rhs, nullptr, x_out, prm, results) != M3D_OK)
             throw std::runtime_error(m3d_last_error());
+    });
+}
+
+
+// EDI files ----------------------------------------------------------------
+
+namespace {
+
+using m3d::data::Datum;
+using m3d::data::MTResponse;
+
+Datum* component(MTResponse& r, int k) {
+    Datum* d[6] = {&r.Zxx, &r.Zxy, &r.Zyx, &r.Zyy, &r.Tx, &r.Ty};
+    return d[k];
+}
+
+// Responses of one station at one frequency to / from the flat arrays:
+// z [4] complex, z_err [4], z_w [4], t [2] complex, t_err [2], t_w [2].
+void put_response(const MTResponse& r0, size_t i, double* z, double* ze, double* zw, double* t, double* te,
+                  double* tw) {
+    MTResponse r = r0;
+    for (int k = 0; k < 4; ++k) {
+        const Datum* d = component(r, k);
+        if (z) { z[2 * (4 * i + k)] = d->value.real(); z[2 * (4 * i + k) + 1] = d->value.imag(); }
+        if (ze) ze[4 * i + k] = d->error;
+        if (zw) zw[4 * i + k] = d->weight;
+    }
+    for (int k = 0; k < 2; ++k) {
+        const Datum* d = component(r, 4 + k);
+        if (t) { t[2 * (2 * i + k)] = d->value.real(); t[2 * (2 * i + k) + 1] = d->value.imag(); }
+        if (te) te[2 * i + k] = d->error;
+        if (tw) tw[2 * i + k] = d->weight;
+    }
+}
+
+MTResponse get_response(size_t i, const double* z, const double* ze, const double* zw, const double* t,
+                        const double* te, const double* tw) {
+    MTResponse r;
+    for (int k = 0; k < 4; ++k) {
+        Datum* d = component(r, k);
+        d->value = {z[2 * (4 * i + k)], z[2 * (4 * i + k) + 1]};
+        d->error = ze[4 * i + k];
+        d->weight = zw[4 * i + k];
+    }
+    for (int k = 0; k < 2; ++k) {
+        Datum* d = component(r, 4 + k);
+        if (t) d->value = {t[2 * (2 * i + k)], t[2 * (2 * i + k) + 1]};
+        if (te) d->error = te[2 * i + k];
+        d->weight = tw ? tw[2 * i + k] : 0.0;
+    }
+    return r;
+}
+
+void put_station(const m3d::data::Station& st, size_t s, char* names, int name_len, double* geo) {
+    if (names && name_len > 0) {
+        char* dst = names + s * size_t(name_len);
+        std::memset(dst, 0, size_t(name_len));
+        std::strncpy(dst, st.name.c_str(), size_t(name_len) - 1);
+    }
+    if (geo) {
+        geo[4 * s] = st.lat; geo[4 * s + 1] = st.lon; geo[4 * s + 2] = st.z;
+        geo[4 * s + 3] = st.has_geo ? 1.0 : 0.0;
+    }
+}
+
+m3d::data::Station get_station(size_t s, const char* names, int name_len, const double* geo) {
+    m3d::data::Station st;
+    if (names && name_len > 0) {
+        const char* src = names + s * size_t(name_len);
+        st.name.assign(src, strnlen(src, size_t(name_len)));
+    }
+    if (geo) {
+        st.lat = geo[4 * s]; st.lon = geo[4 * s + 1]; st.z = geo[4 * s + 2]; st.has_geo = geo[4 * s + 3] != 0.0;
+    }
+    return st;
+}
+
+}  // namespace
+
+int m3dh_edi_read(const char* path, int* nfreq, char* name, int name_len, double* geo, double* freqs,
+                  double* z, double* z_err, double* z_w, double* t, double* t_err, double* t_w) {
+    return guarded([&] {
+        if (!path || !nfreq) throw std::invalid_argument("null argument");
+        const m3d::io::EDIRecord rec = m3d::io::read_edi(path);
+        *nfreq = static_cast<int>(rec.frequencies.size());
+        put_station(rec.station, 0, name, name_len, geo);
+        for (size_t f = 0; f < rec.frequencies.size(); ++f) {
+            if (freqs) freqs[f] = rec.frequencies[f];
+            put_response(rec.responses[f], f, z, z_err, z_w, t, t_err, t_w);
+        }
+    });
+}
+
+int m3dh_edi_write(const char* path, const char* name, const double* geo, int nfreq, const double* freqs,
+                   const double* z, const double* z_err, const double* z_w, const double* t, const double* t_err,
+                   const double* t_w) {
+    return guarded([&] {
+        if (!path || nfreq < 0 || (nfreq > 0 && (!freqs || !z || !z_err || !z_w)))
+            throw std::invalid_argument("null argument");
+        m3d::data::Station st = get_station(0, name, name ? int(std::strlen(name)) + 1 : 0, geo);
+        std::vector<MTResponse> r(nfreq);
+        for (int f = 0; f < nfreq; ++f) r[f] = get_response(f, z, z_err, z_w, t, t_err, t_w);
+        m3d::io::write_edi(path, st, std::vector<double>(freqs, freqs + nfreq), r);
+    });
+}
+
+int m3dh_edi_load_directory(const char* dir, int* ns, int* nf, char* names, int name_len, double* geo,
+                            double* freqs, double* z, double* z_err, double* z_w, double* t, double* t_err,
+                            double* t_w) {
+    return guarded([&] {
+        if (!dir || !ns || !nf) throw std::invalid_argument("null argument");
+        m3d::data::MTData d;
+        m3d::io::load_edi_directory(dir, d);
+        *ns = d.num_stations();
+        *nf = d.num_frequencies();
+        for (int s = 0; s < d.num_stations(); ++s) put_station(d.station(s), size_t(s), names, name_len, geo);
+        for (int f = 0; f < d.num_frequencies(); ++f) {
+            if (freqs) freqs[f] = d.frequencies()[f];
+            for (int s = 0; s < d.num_stations(); ++s)
+                put_response(d.observed(s, f), size_t(f) * d.num_stations() + s, z, z_err, z_w, t, t_err, t_w);
+        }
+    });
+}
+
+int m3dh_edi_save_directory(const char* dir, int ns, int nf, const char* names, int name_len, const double* geo,
+                            const double* freqs, const double* z, const double* z_err, const double* z_w,
+                            const double* t, const double* t_err, const double* t_w) {
+    return guarded([&] {
+        if (!dir || ns < 0 || nf < 0 || (nf > 0 && (!freqs || !z || !z_err || !z_w)))
+            throw std::invalid_argument("null argument");
+        m3d::data::MTData d;
+        d.set_frequencies(std::vector<double>(freqs, freqs + nf));
+        for (int s = 0; s < ns; ++s) d.add_station(get_station(size_t(s), names, name_len, geo));
+        for (int f = 0; f < nf; ++f)
+            for (int s = 0; s < ns; ++s)
+                d.set_observed(s, f, get_response(size_t(f) * ns + s, z, z_err, z_w, t, t_err, t_w));
+        m3d::io::save_edi_directory(dir, d);
     });
 }
 

@@ -28,6 +28,7 @@ struct MTResponse {
 struct Station {
     std::string name;
     double x = 0, y = 0, z = 0, lat = 0, lon = 0;
+    bool has_geo = false;  // lat / lon given (EDI HEAD)
 };
 
 class MTData {

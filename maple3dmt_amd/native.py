@@ -143,6 +143,7 @@ HOST_SYMBOLS = [
     "m3dh_sensitivity", "m3dh_solve_frequencies", "m3dh_adjoint_solve", "m3dh_device_operator",
     "m3dh_update_sigma_device", "m3dh_station_edges", "m3dh_extract_impedance_compact",
     "m3dh_adjoint_rhs_compact", "m3dh_perturbation_rhs", "m3dh_delta_impedance_compact",
+    "m3dh_edi_read", "m3dh_edi_write", "m3dh_edi_load_directory", "m3dh_edi_save_directory",
 ]
 
 
