@@ -223,6 +223,16 @@ int m3d_operator_permutation(m3d_operator* op, int* perm, int* colour_start);
  * off-diagonal nonzeros. */
 int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long* stored_entries,
                       long long* nnz);
+/* Storage of the off-diagonal blocks (no reference counterpart: the
+ * reference keeps CSR values and columns).  value_table > 0: besides the
+ * separate value and column arrays the operator holds packed blocks, one
+ * 32-bit word per entry with the column in the low column_bits bits and the
+ * index of the entry's value in a table of value_table doubles, used at the
+ * widths whose launch shape has packed = 1; value_table = 0: no packed blocks.
+ * They are built at m3d_operator_create when the table fits (at most 4096
+ * values, index and column within 32 bits); the environment variable
+ * M3D_PACKED_SELL=0 switches them off. */
+int m3d_operator_format(m3d_operator* op, int* value_table, int* column_bits);
 
 /* Measurement (no reference counterpart): average duration in ms of one
  * launch of a kernel class over `reps` launches, timed with HIP events on the
@@ -288,12 +298,16 @@ typedef struct m3d_launch_shape {
     int sweep_chunk; /* 1: a sweep's workgroups grid-stride over its row blocks (sweep_cap
                         applies); 2, 4, 8 or 16: each workgroup takes that many consecutive
                         row blocks (sweep_cap is then unused) */
+    int packed;      /* 1: gather couplings from the packed blocks (see
+                        m3d_operator_format) when the operator has them; 0: from
+                        the separate value and column blocks.  Same values, same
+                        order of summation: results are identical either way */
 } m3d_launch_shape;
 
 /* width is a batch width 1, 2, 4, 8, 16 or 32; precision M3D_PREC_FP64
  * (complex128 kernels) or M3D_PREC_MIXED (the complex64 inner kernels).  The
  * default shape of a new operator: sweep_sub 1, sweep_cap 4096, row_cap 2048,
- * sweep_chunk 1. */
+ * sweep_chunk 1, packed 1 at widths up to 4 and 0 above. */
 int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launch_shape* out);
 int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d_launch_shape* shape);
 
@@ -301,7 +315,7 @@ int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d
  * every width whose bit is set in widths_mask (bit i = width 1 << i) and both
  * precisions, times one BiCGStab iteration (as m3d_bench_kernel kinds 2 and 3,
  * reps iterations per candidate, frequencies omega[0..width-1]) under candidate
- * shapes by coordinate descent (sweep_sub, sweep_cap, row_cap, sweep_chunk) and
+ * shapes by coordinate descent (sweep_sub, sweep_cap, row_cap, sweep_chunk, packed) and
  * keeps a candidate only when it is at least 5 % faster than the incumbent.  best_ms, if not NULL, receives 2 * 6 values:
  * [precision * 6 + log2(width)] = ms per iteration of the kept shape, 0 where
  * not tuned.  Blocking; run it outside timed regions.  Choices stay with the

@@ -92,6 +92,7 @@ struct LaunchShape {
     int row_cap;    // grid cap of the row kernels that reduce (a multiple of kXcds, <= kMaxBlocks)
     int sweep_chunk;  // 1: sweeps grid-stride over row blocks; k > 1: each workgroup
                       // takes k consecutive row blocks (neighbouring rows on one CU)
+    int packed;       // 1: gather from the packed blocks when the operator has them
 };
 constexpr int kWidths = 6;  // batch widths 1, 2, 4, 8, 16, 32
 __host__ __device__ constexpr int width_index(int ns) {
@@ -104,6 +105,14 @@ struct DevOp {
     int n;                 // padded rows
     LaunchShape shape[kWidths];
     const int4* slc;       // per slice: {lower offset, lower width, upper offset, upper width}
+    // Packed blocks (vtab != null): one word per entry, column in the low
+    // cbits bits and the index of its value in vtab above them; vtab[0] = 0
+    // for padding.  Used at the widths whose shape asks for them; the
+    // separate value and column blocks otherwise.
+    const unsigned* l_pk;
+    const unsigned* u_pk;
+    const V* vtab;
+    int cbits;
     const V* l_val;
     const int* l_col;
     const V* u_val;

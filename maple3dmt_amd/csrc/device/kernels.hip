@@ -54,11 +54,58 @@ __device__ inline void gather_sell(const V* __restrict__ val, const int* __restr
     }
 }
 
-template <typename V>
-__device__ inline void row_gather(const V* __restrict__ val, const int* __restrict__ col, int off,
-                                  int w, int lane, const cvec<V>* __restrict__ x, int NS, int sys,
-                                  V& ar, V& ai) {
-    gather_sell<8, 1, V>(val, col, off, w, lane, 0, x, NS, sys, ar, ai);
+// gather_sell over a packed block: the column and the value's table index
+// come from one word, so an entry costs 4 bytes instead of 8 + 4 (or 4 + 4).
+// Entries past w keep the column of entry w - 1 and take index 0 (value 0).
+// The additions are those of gather_sell in the same order.
+template <int U, int STEP, typename V>
+__device__ inline void gather_packed(const unsigned* __restrict__ pk, const V* __restrict__ tab, int cbits,
+                                     int off, int w, int lane, int k0, const cvec<V>* __restrict__ x,
+                                     int NS, int sys, V& ar, V& ai) {
+    off = __builtin_amdgcn_readfirstlane(off);
+    w = __builtin_amdgcn_readfirstlane(w);
+    const unsigned cm = (1u << cbits) - 1u;
+    for (int k = k0; k < w; k += U * STEP) {
+        unsigned wd[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + u * STEP;
+            wd[u] = pk[off + min(kk, w - 1) * kSlice + lane];
+            if (kk >= w) wd[u] &= cm;
+        }
+        cvec<V> xv[U];
+        V a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xv[u] = x[size_t(wd[u] & cm) * NS + sys];
+            a[u] = tab[wd[u] >> cbits];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ar += a[u] * xv[u].x;
+            ai += a[u] * xv[u].y;
+        }
+    }
+}
+
+// Entries k0, k0 + STEP, ... of the lower (UP = false) or upper block of a
+// row, from the packed blocks when the operator has them and the launch
+// shape of this batch width asks for them (a uniform branch).
+template <int U, int STEP, bool UP, typename V>
+__device__ inline void gather_block(const DevOp<V>& op, int off, int w, int lane, int k0,
+                                    const cvec<V>* __restrict__ x, int NS, int sys, V& ar, V& ai) {
+    if (op.vtab && op.shape[width_index(NS)].packed)
+        gather_packed<U, STEP, V>(UP ? op.u_pk : op.l_pk, op.vtab, op.cbits, off, w, lane, k0, x, NS, sys, ar,
+                                  ai);
+    else
+        gather_sell<U, STEP, V>(UP ? op.u_val : op.l_val, UP ? op.u_col : op.l_col, off, w, lane, k0, x, NS,
+                                sys, ar, ai);
+}
+
+template <bool UP, typename V>
+__device__ inline void row_gather(const DevOp<V>& op, int off, int w, int lane, const cvec<V>* __restrict__ x,
+                                  int NS, int sys, V& ar, V& ai) {
+    gather_block<8, 1, UP, V>(op, off, w, lane, 0, x, NS, sys, ar, ai);
 }
 
 // Sweeps may split each row over SUB lanes (entries k = sub, sub + SUB, ...;
@@ -68,11 +115,11 @@ __device__ inline void row_gather(const V* __restrict__ val, const int* __restri
 template <int NS, int SUB>
 __host__ __device__ constexpr int sweep_rpb() { return kBlock / (NS * SUB); }
 
-template <int SUB, typename V>
-__device__ inline void row_gather_split(const V* __restrict__ val, const int* __restrict__ col, int off,
-                                        int w, int lane, const cvec<V>* __restrict__ x, int NS, int sys,
-                                        int sub, V& ar, V& ai) {
-    gather_sell<(SUB == 1 ? 8 : SUB >= 8 ? 2 : 4), SUB, V>(val, col, off, w, lane, sub, x, NS, sys, ar, ai);
+template <int SUB, bool UP, typename V>
+__device__ inline void row_gather_split(const DevOp<V>& op, int off, int w, int lane,
+                                        const cvec<V>* __restrict__ x, int NS, int sys, int sub, V& ar,
+                                        V& ai) {
+    gather_block<(SUB == 1 ? 8 : SUB >= 8 ? 2 : 4), SUB, UP, V>(op, off, w, lane, sub, x, NS, sys, ar, ai);
 #pragma unroll
     for (int m = 1; m < SUB; m <<= 1) {
         ar += __shfl_xor(ar, NS * m);
@@ -88,8 +135,8 @@ __device__ inline cvec<V> apply_row(const DevOp<V>& op, int row, int NS, int sys
     const int4 sc = op.slc[row >> 6];
     const int lane = row & 63;
     V ar = 0, ai = 0;
-    row_gather(op.l_val, op.l_col, sc.x, sc.y, lane, x, NS, sys, ar, ai);
-    row_gather(op.u_val, op.u_col, sc.z, sc.w, lane, x, NS, sys, ar, ai);
+    row_gather<false>(op, sc.x, sc.y, lane, x, NS, sys, ar, ai);
+    row_gather<true>(op, sc.z, sc.w, lane, x, NS, sys, ar, ai);
     const V dr = op.diag_s[row], di = w * op.me[row];
     const cvec<V> xi = x[size_t(row) * NS + sys];
     cvec<V> y;
@@ -107,7 +154,7 @@ __device__ inline cvec<V> apply_row_eis(const DevOp<V>& op, int row, int NS, int
     const int4 sc = op.slc[row >> 6];
     const int lane = row & 63;
     V ar = 0, ai = 0;
-    row_gather(op.l_val, op.l_col, sc.x, sc.y, lane, z, NS, sys, ar, ai);
+    row_gather<false>(op, sc.x, sc.y, lane, z, NS, sys, ar, ai);
     const V dr = op.diag_s[row], di = w * op.me[row];
     const cvec<V> yi = y[size_t(row) * NS + sys];
     cvec<V> out;
@@ -290,7 +337,7 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
         }
         const int lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather_split<SUB>(op.l_val, op.l_col, sc.x, sc.y, lane, y, NS, sys, sub, ar, ai);
+        row_gather_split<SUB, false>(op, sc.x, sc.y, lane, y, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(ds, w * me, dr, di);
         const V er = src.x - ar, ei = src.y - ai;
@@ -324,7 +371,7 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
         const cvec<V> yi = y[i];
         const int lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather_split<SUB>(op.u_val, op.u_col, sc.z, sc.w, lane, z, NS, sys, sub, ar, ai);
+        row_gather_split<SUB, true>(op, sc.z, sc.w, lane, z, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(ds, w * me, dr, di);
         cvec<V> out;

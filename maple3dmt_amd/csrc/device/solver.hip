@@ -157,6 +157,55 @@ bool use_eisenstat() {
     return on;
 }
 
+// Largest value table of packed SELL blocks (32 KB of doubles).
+constexpr int kMaxValueTable = 4096;
+
+uint64_t bits_of(double a) {
+    uint64_t b;
+    std::memcpy(&b, &a, sizeof b);
+    return b;
+}
+
+// Bit pattern of a double -> its index in the value table: open addressing
+// over twice kMaxValueTable slots (the table never holds more keys).
+class FlatIndex {
+  public:
+    FlatIndex() : key_(kSlots, 0), val_(kSlots, -1) {}
+    // Adds key -> v unless present; true when added.
+    bool insert(uint64_t key, int v) {
+        size_t h = slot(key);
+        while (val_[h] >= 0) {
+            if (key_[h] == key) return false;
+            h = (h + 1) & (kSlots - 1);
+        }
+        key_[h] = key;
+        val_[h] = v;
+        return true;
+    }
+    int find(uint64_t key) const {
+        for (size_t h = slot(key); val_[h] >= 0; h = (h + 1) & (kSlots - 1))
+            if (key_[h] == key) return val_[h];
+        return -1;
+    }
+
+  private:
+    static constexpr size_t kSlots = 4 * kMaxValueTable;
+    static size_t slot(uint64_t k) {
+        k ^= k >> 33;
+        k *= 0xff51afd7ed558ccdULL;
+        k ^= k >> 33;
+        return size_t(k) & (kSlots - 1);
+    }
+    std::vector<uint64_t> key_;
+    std::vector<int> val_;
+};
+
+// Packed SELL blocks unless M3D_PACKED_SELL=0 (read at each operator build).
+bool packed_sell() {
+    const char* v = std::getenv("M3D_PACKED_SELL");
+    return !(v && v[0] == '0');
+}
+
 bool row_sort() {
     static const bool on = [] {
         const char* v = std::getenv("M3D_ROW_SORT");
@@ -206,6 +255,12 @@ struct m3d_operator {
     DBuf<int4> slc;
     DBuf<double> l_val, u_val, diag_s, me;
     DBuf<float> l_val32, u_val32, diag_s32, me32;
+    // Packed SELL blocks and their value table (empty when the table does not
+    // fit beside the column bits, or packing is switched off).
+    DBuf<unsigned> l_pk, u_pk;
+    DBuf<double> vtab;
+    DBuf<float> vtab32;
+    int cbits = 0, ntab = 0;
     DBuf<unsigned char> bc, constrained;
     std::vector<unsigned char> h_constrained;
     bool mass_set = false;
@@ -248,6 +303,7 @@ DevOp<double> m3d_operator::view<double>() const {
     for (int i = 0; i < kWidths; ++i) o.shape[i] = shape[0][i];
     o.slc = slc.p;
     o.l_val = l_val.p; o.l_col = l_col.p; o.u_val = u_val.p; o.u_col = u_col.p;
+    o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab.p : nullptr; o.cbits = cbits;
     o.diag_s = diag_s.p; o.me = me.p;
     return o;
 }
@@ -258,6 +314,7 @@ DevOp<float> m3d_operator::view<float>() const {
     for (int i = 0; i < kWidths; ++i) o.shape[i] = shape[1][i];
     o.slc = slc.p;
     o.l_val = l_val32.p; o.l_col = l_col.p; o.u_val = u_val32.p; o.u_col = u_col.p;
+    o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab32.p : nullptr; o.cbits = cbits;
     o.diag_s = diag_s32.p; o.me = me32.p;
     return o;
 }
@@ -1074,11 +1131,15 @@ std::vector<int> colour_rows(int n, const int* ptr, const int* col, const unsign
 
 LaunchShape default_shape(int width) {
     LaunchShape sh;
-    (void)width;
     sh.sweep_sub = 1;  // with grouped gathers one lane per row keeps enough loads in flight
     sh.sweep_cap = 4096;
     sh.row_cap = kMaxBlocks;
     sh.sweep_chunk = 1;
+    // Packed blocks pay at narrow widths, where a row's few gathers wait on
+    // memory latency and fewer bytes per entry shorten every sweep (one
+    // BiCGStab iteration -12 % at 1 system, -5 % at 2 on a 1.45 M-edge mesh);
+    // at 16 systems the separate blocks were 3 % faster.
+    sh.packed = width <= 4 ? 1 : 0;
     return sh;
 }
 
@@ -1194,12 +1255,61 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
     std::vector<int4> slc(nsl);
     for (int sl = 0; sl < nsl; ++sl) slc[sl] = make_int4(loff[sl], lw[sl], uoff[sl], uw[sl]);
     op.slc.upload(slc, s);
+    // The couplings of an octree mesh take few distinct values (cell sizes
+    // are powers of two: 82 values among 15 M entries on a five-level mesh),
+    // so the blocks are stored as one word per entry, column | value index <<
+    // cbits, with a value table, when the table index fits in the bits the
+    // columns leave and the table stays small enough to sit in cache.  The
+    // values and their order of summation are unchanged.
+    std::vector<double> tab{0.0};
+    std::vector<float> tab32, lval32, uval32;
+    std::vector<unsigned> lpk, upk;
+    bool pack = packed_sell();
+    int cbits = 1;
+    while ((1LL << cbits) < op.npad) ++cbits;
+    if (pack) {
+        FlatIndex tix;
+        tix.insert(bits_of(0.0), 0);
+        auto scan = [&](const std::vector<double>& v) {
+            for (double a : v) {
+                if (!tix.insert(bits_of(a), int(tab.size()))) continue;
+                tab.push_back(a);
+                if (tab.size() > size_t(kMaxValueTable)) return false;
+            }
+            return true;
+        };
+        pack = scan(lval) && scan(uval);
+        int vbits = 1;
+        while ((1LL << vbits) < (long long)tab.size()) ++vbits;
+        pack = pack && cbits + vbits <= 32;
+        if (pack) {
+            auto words = [&](const std::vector<double>& v, const std::vector<int>& c, std::vector<unsigned>& out) {
+                out.resize(v.size());
+                for (size_t i = 0; i < v.size(); ++i)
+                    out[i] = unsigned(c[i]) | (unsigned(tix.find(bits_of(v[i]))) << cbits);
+            };
+            words(lval, lcol, lpk);
+            words(uval, ucol, upk);
+            tab32.assign(tab.begin(), tab.end());
+            op.l_pk.upload(lpk, s);
+            op.u_pk.upload(upk, s);
+            op.vtab.upload(tab, s);
+            op.vtab32.upload(tab32, s);
+            op.cbits = cbits;
+            op.ntab = int(tab.size());
+        }
+    }
+    // The separate blocks stay resident beside the packed ones: the launch
+    // shape picks one layout per batch width.
     op.l_val.upload(lval, s); op.u_val.upload(uval, s); op.l_col.upload(lcol, s); op.u_col.upload(ucol, s);
+    lval32.assign(lval.begin(), lval.end());
+    uval32.assign(uval.begin(), uval.end());
+    op.l_val32.upload(lval32, s); op.u_val32.upload(uval32, s);
     op.diag_s.upload(ds, s);
     op.bc.upload(bc, s);
     op.constrained.upload(cons, s);
-    std::vector<float> lval32(lval.begin(), lval.end()), uval32(uval.begin(), uval.end()), ds32(ds.begin(), ds.end());
-    op.l_val32.upload(lval32, s); op.u_val32.upload(uval32, s); op.diag_s32.upload(ds32, s);
+    std::vector<float> ds32(ds.begin(), ds.end());
+    op.diag_s32.upload(ds32, s);
 
     // Hanging pairs in dependency levels that keep the sequential semantics
     // of the caller's list order.
@@ -1593,6 +1703,14 @@ int m3d_operator_permutation(m3d_operator* op, int* perm, int* colour_start) {
     });
 }
 
+int m3d_operator_format(m3d_operator* op, int* value_table, int* column_bits) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        if (value_table) *value_table = op->ntab;
+        if (column_bits) *column_bits = op->ntab ? op->cbits : 0;
+    });
+}
+
 int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long* stored_entries,
                       long long* nnz) {
     return guarded([&] {
@@ -1707,8 +1825,10 @@ void divcorr_host_impl(m3d_operator& op, int nsys, const double* omega, double2*
 // Average duration of one launch of a kernel class, timed with HIP events on
 // the context stream, plus its algorithmic HBM bytes (free rows n, stored
 // off-diagonal nonzeros m, k systems):
-//   SpMV        m*(8+4) + n*2*8 + 2*k*n*16
-//   SSOR apply  m*(8+4) + n*4*8 + 4*k*n*16   (lower half forward, upper half backward)
+//   SpMV        m*e + n*2*8 + 2*k*n*16
+//   SSOR apply  m*e + n*4*8 + 4*k*n*16   (lower half forward, upper half backward)
+// with e = 4 bytes per entry when the width gathers from packed blocks,
+// 8 + 4 otherwise.
 template <int NS>
 void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int reps, double* avg_ms,
                 double* bytes) {
@@ -1809,8 +1929,10 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     const double m = double(op.nnz), n = double(nfree), k = double(nsys);
     // Algorithmic bytes: values + columns of the m off-diagonal nonzeros, the
     // (S_ii, Me_i) diagonal pair, and each vector touched once.
-    auto spmv_b = [&](double wv, double wc) { return m * (wv + 4.0) + 2.0 * n * wv + 2.0 * k * n * wc; };
-    auto ssor_b = [&](double wv, double wc) { return m * (wv + 4.0) + 4.0 * n * wv + 4.0 * k * n * wc; };
+    const bool pk = op.ntab && op.shape[kind == 3 ? 1 : 0][width_index(NS)].packed;
+    auto ent = [&](double wv) { return pk ? 4.0 : wv + 4.0; };
+    auto spmv_b = [&](double wv, double wc) { return m * ent(wv) + 2.0 * n * wv + 2.0 * k * n * wc; };
+    auto ssor_b = [&](double wv, double wc) { return m * ent(wv) + 4.0 * n * wv + 4.0 * k * n * wc; };
     // One right-preconditioned BiCGStab iteration: 2 SpMV, 2 SSOR and 17
     // vector passes of the AXPY / dot work.
     auto iter_b = [&](double wv, double wc) {
@@ -1819,7 +1941,7 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     // Backward sweep: the upper half of the couplings, the diagonal pair, y
     // in and z out, per launch (one colour on average).
     auto bwd_b = [&](double wv, double wc) {
-        return (0.5 * m * (wv + 4.0) + 2.0 * n * wv + 2.0 * k * n * wc) / op.ncolours;
+        return (0.5 * m * ent(wv) + 2.0 * n * wv + 2.0 * k * n * wc) / op.ncolours;
     };
     *bytes = kind == 0   ? spmv_b(8, 16)
              : kind == 1 ? ssor_b(8, 16)
@@ -1834,7 +1956,7 @@ bool valid_shape(int width, const LaunchShape& sh) {
     const int ch = sh.sweep_chunk;
     return sub_ok && sh.sweep_cap >= kXcds && sh.sweep_cap <= 65536 && sh.sweep_cap % kXcds == 0 &&
            sh.row_cap >= kXcds && sh.row_cap <= kMaxBlocks && sh.row_cap % kXcds == 0 &&
-           (ch == 1 || ch == 2 || ch == 4 || ch == 8 || ch == 16);
+           (ch == 1 || ch == 2 || ch == 4 || ch == 8 || ch == 16) && (sh.packed == 0 || sh.packed == 1);
 }
 
 int width_slot(int width) {
@@ -1908,6 +2030,11 @@ double tune_one(m3d_operator& op, int p, int slot, const double* omega, int reps
         if (ch == after_row.sweep_chunk) continue;
         LaunchShape c = best;
         c.sweep_chunk = ch;
+        consider(c);
+    }
+    if (op.ntab) {
+        LaunchShape c = best;
+        c.packed = 1 - c.packed;
         consider(c);
     }
     op.shape[p][slot] = best;
@@ -2000,6 +2127,7 @@ int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launc
         out->sweep_cap = sh.sweep_cap;
         out->row_cap = sh.row_cap;
         out->sweep_chunk = sh.sweep_chunk;
+        out->packed = sh.packed;
     });
 }
 
@@ -2013,6 +2141,7 @@ int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d
         sh.sweep_cap = shape->sweep_cap;
         sh.row_cap = shape->row_cap;
         sh.sweep_chunk = shape->sweep_chunk;
+        sh.packed = shape->packed;
         if (!valid_shape(width, sh)) throw Fail(M3D_ERR_ARG, "invalid launch shape for this width");
         HIPCHK(hipSetDevice(op->ctx->device));
         HIPCHK(hipStreamSynchronize(op->ctx->stream));

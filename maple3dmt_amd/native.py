@@ -123,6 +123,7 @@ class LaunchShapeC(C.Structure):
         ("sweep_cap", C.c_int),
         ("row_cap", C.c_int),
         ("sweep_chunk", C.c_int),
+        ("packed", C.c_int),
     ]
 
 
@@ -131,7 +132,7 @@ DEVICE_SYMBOLS = [
     "m3d_context_create", "m3d_context_destroy", "m3d_last_error", "m3d_operator_create",
     "m3d_operator_destroy", "m3d_operator_set_mass", "m3d_operator_set_divcorr", "m3d_solve_batch",
     "m3d_solve_batch_device", "m3d_divcorr", "m3d_spmv", "m3d_ssor_apply",
-    "m3d_operator_permutation", "m3d_operator_info", "m3d_context_stream", "m3d_bench_kernel",
+    "m3d_operator_permutation", "m3d_operator_info", "m3d_operator_format", "m3d_context_stream", "m3d_bench_kernel",
     "m3d_operator_set_sensitivity", "m3d_sensitivity_device", "m3d_operator_get_shape",
     "m3d_operator_set_shape", "m3d_operator_autotune", "m3d_operator_set_mass_structure",
     "m3d_operator_update_sigma", "m3d_operator_update_sigma_device", "m3d_perturbation_rhs_device",
@@ -182,6 +183,7 @@ def lib() -> C.CDLL:
     L.m3d_ssor_apply.argtypes = [vp, i, vp, vp, vp]
     L.m3d_operator_permutation.argtypes = [vp, vp, vp]
     L.m3d_operator_info.argtypes = [vp, p(i), p(i), p(C.c_longlong), p(C.c_longlong)]
+    L.m3d_operator_format.argtypes = [vp, p(i), p(i)]
     L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d)]
     L.m3d_sensitivity_device.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_perturbation_rhs_device.argtypes = [vp, i, vp, vp, vp, vp]
@@ -366,7 +368,10 @@ class DeviceOperator:
         st, nz = C.c_longlong(), C.c_longlong()
         _check_dev(lib().m3d_operator_info(self.handle, C.byref(npad), C.byref(nc), C.byref(st),
                                            C.byref(nz)))
-        return dict(n_padded=npad.value, ncolours=nc.value, stored=st.value, nnz=nz.value)
+        tab, cb = C.c_int(), C.c_int()
+        _check_dev(lib().m3d_operator_format(self.handle, C.byref(tab), C.byref(cb)))
+        return dict(n_padded=npad.value, ncolours=nc.value, stored=st.value, nnz=nz.value,
+                    value_table=tab.value, column_bits=cb.value)
 
     def permutation(self):
         inf = self.info()
@@ -395,11 +400,14 @@ class DeviceOperator:
         sh = LaunchShapeC()
         _check_dev(lib().m3d_operator_get_shape(self.handle, precision, width, C.byref(sh)))
         return {"sweep_sub": sh.sweep_sub, "sweep_cap": sh.sweep_cap, "row_cap": sh.row_cap,
-                "sweep_chunk": sh.sweep_chunk}
+                "sweep_chunk": sh.sweep_chunk, "packed": sh.packed}
 
     def set_shape(self, precision: int, width: int, sweep_sub: int, sweep_cap: int, row_cap: int,
-                  sweep_chunk: int = 1) -> None:
-        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap, sweep_chunk)
+                  sweep_chunk: int = 1, packed: Optional[int] = None) -> None:
+        """packed=None keeps the width's current layout choice."""
+        if packed is None:
+            packed = self.get_shape(precision, width)["packed"]
+        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap, sweep_chunk, packed)
         _check_dev(lib().m3d_operator_set_shape(self.handle, precision, width, C.byref(sh)))
 
     def autotune(self, omegas, widths=(1, 2, 4, 8, 16), reps: int = 16) -> dict:

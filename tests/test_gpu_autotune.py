@@ -28,10 +28,14 @@ def block(ctx):
     return wl, m, o
 
 
+def _default_packed(w):
+    return 1 if w <= 4 else 0
+
+
 def _reset(dev):
     for prec in (PREC_FP64, PREC_MIXED):
         for w in (1, 2, 4, 8, 16, 32):
-            dev.set_shape(prec, w, 1, 4096, 2048)
+            dev.set_shape(prec, w, 1, 4096, 2048, 1, _default_packed(w))
 
 
 def test_shape_defaults_and_validation(block):
@@ -40,7 +44,7 @@ def test_shape_defaults_and_validation(block):
     for prec in (PREC_FP64, PREC_MIXED):
         for w in (1, 2, 4, 8, 16, 32):
             assert dev.get_shape(prec, w) == {"sweep_sub": 1, "sweep_cap": 4096, "row_cap": 2048,
-                                              "sweep_chunk": 1}
+                                              "sweep_chunk": 1, "packed": _default_packed(w)}
     for bad in ((3, 4096, 2048), (8, 4096, 2048), (1, 100, 2048), (1, 4096, 4096), (1, 0, 2048)):
         with pytest.raises(NativeError):
             dev.set_shape(PREC_FP64, 8, *bad)   # width 8: sub 8 gives 64 lanes per row
@@ -48,9 +52,13 @@ def test_shape_defaults_and_validation(block):
         dev.get_shape(PREC_FP64, 3)
     with pytest.raises(NativeError):
         dev.set_shape(PREC_FP64, 8, 1, 4096, 2048, 3)   # chunk 3
+    with pytest.raises(NativeError):
+        dev.set_shape(PREC_FP64, 8, 1, 4096, 2048, 1, 2)   # packed 2
     dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4)
     assert dev.get_shape(PREC_FP64, 8) == {"sweep_sub": 4, "sweep_cap": 1024, "row_cap": 512,
-                                           "sweep_chunk": 4}
+                                           "sweep_chunk": 4, "packed": 0}
+    dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4, 1)
+    assert dev.get_shape(PREC_FP64, 8)["packed"] == 1
     _reset(dev)
 
 
@@ -71,9 +79,10 @@ def test_every_sweep_split_matches_oracle_ssor(block, width):
         for sub in (1, 2, 4, 8):
             if width * sub > 32:
                 continue
-            for cap, chunk in ((8, 1), (4096, 1), (4096, 4), (4096, 16)):
-                dev.set_shape(PREC_FP64, width, sub, cap, 2048, chunk)
-                print(f"width {width} sweep_sub {sub} sweep_cap {cap} sweep_chunk {chunk}", flush=True)
+            for cap, chunk, packed in ((8, 1, 1), (4096, 1, 0), (4096, 1, 1), (4096, 4, 1), (4096, 16, 0)):
+                dev.set_shape(PREC_FP64, width, sub, cap, 2048, chunk, packed)
+                print(f"width {width} sweep_sub {sub} sweep_cap {cap} sweep_chunk {chunk} packed {packed}",
+                      flush=True)
                 z = dev.ssor_apply(np.full(width, w), r)
                 for i in range(width):
                     assert np.linalg.norm(z[i] - ref[i]) <= 1e-12 * np.linalg.norm(ref[i]), (sub, cap, i)

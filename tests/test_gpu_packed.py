@@ -1,0 +1,104 @@
+"""Packed SELL blocks (one 32-bit word per coupling: column and value-table
+index) against the separate value / column blocks.
+
+The packed layout changes only how a coupling's value is fetched (from a
+table of the operator's distinct values), not the value nor the order of the
+additions, so every result must be identical bit for bit: SpMV, SSOR apply,
+and whole FP64 and mixed-precision solves (same iteration counts, same
+fields).  Compared between two operators, one built without packed blocks
+(M3D_PACKED_SELL=0), and within one operator by switching the launch shape's
+layout choice.  The other GPU tests run the default choice (packed up to 4
+systems) against the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from maple3dmt_amd import MODE_FORWARD, PREC_MIXED, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(ctx, wl, packed):
+    old = os.environ.get("M3D_PACKED_SELL")
+    os.environ["M3D_PACKED_SELL"] = "1" if packed else "0"
+    try:
+        return configs.build_model(wl, ctx)
+    finally:
+        if old is None:
+            del os.environ["M3D_PACKED_SELL"]
+        else:
+            os.environ["M3D_PACKED_SELL"] = old
+
+
+@pytest.fixture(scope="module")
+def pair(ctx):
+    wl = configs.random3d(5, min_level=3, max_level=6, n_freq=2, half_width=60e3, station_radius=8e3)
+    mp = _build(ctx, wl, True)
+    mu = _build(ctx, wl, False)
+    yield wl, mp, mu
+    mp.close()
+    mu.close()
+
+
+def test_format(pair):
+    _, mp, mu = pair
+    ip, iu = mp.device.info(), mu.device.info()
+    print("packed", ip, "unpacked", iu, flush=True)
+    assert 1 < ip["value_table"] <= 4096
+    assert ip["column_bits"] >= int(np.ceil(np.log2(ip["n_padded"])))
+    assert iu["value_table"] == 0 and iu["column_bits"] == 0
+    assert {k: ip[k] for k in ("n_padded", "ncolours", "stored", "nnz")} == \
+        {k: iu[k] for k in ("n_padded", "ncolours", "stored", "nnz")}
+
+
+def _set_packed(dev, packed):
+    for prec in (0, PREC_MIXED):
+        for w in (1, 2, 4, 8, 16, 32):
+            sh = dev.get_shape(prec, w)
+            sh["packed"] = packed
+            dev.set_shape(prec, w, **sh)
+
+
+@pytest.fixture
+def all_packed(pair):
+    _, mp, _ = pair
+    _set_packed(mp.device, 1)
+    yield
+    for prec in (0, PREC_MIXED):
+        for w in (1, 2, 4, 8, 16, 32):
+            sh = mp.device.get_shape(prec, w)
+            sh["packed"] = 1 if w <= 4 else 0
+            mp.device.set_shape(prec, w, **sh)
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 16])
+def test_spmv_and_ssor_bitwise(pair, all_packed, k):
+    _, mp, mu = pair
+    rng = np.random.default_rng(k)
+    om = 2 * np.pi * 10.0 ** rng.uniform(-3, 2, k)
+    x = rng.standard_normal((k, mp.n)) + 1j * rng.standard_normal((k, mp.n))
+    assert np.array_equal(mp.device.spmv(om, x), mu.device.spmv(om, x))
+    assert np.array_equal(mp.device.ssor_apply(om, x), mu.device.ssor_apply(om, x))
+    # The same operator with the layout switched by the launch shape.
+    _set_packed(mp.device, 0)
+    assert np.array_equal(mp.device.spmv(om, x), mu.device.spmv(om, x))
+    assert np.array_equal(mp.device.ssor_apply(om, x), mu.device.ssor_apply(om, x))
+
+
+@pytest.mark.parametrize("precision", [0, PREC_MIXED])
+def test_solve_bitwise(pair, all_packed, precision):
+    wl, mp, mu = pair
+    prm = mp.fwd_params.solve_params(precision=precision, inner_tol=1e-3 if precision else 0.0)
+    omegas = np.repeat(2 * np.pi * wl.freqs, 2)
+    out = []
+    for m in (mp, mu):
+        rhs = np.stack([m.rhs_total(m.bc_values(w, p % 2)) for p, w in enumerate(omegas)])
+        aux = np.stack([m.primary_field(w, p % 2) for p, w in enumerate(omegas)])
+        x, res = m.device.solve(MODE_FORWARD, omegas, rhs, aux, params=prm)
+        out.append((x, [r["iterations"] for r in res], [r["info"] for r in res]))
+    print("iterations", out[0][1], out[1][1], flush=True)
+    assert out[0][1] == out[1][1]
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0])
