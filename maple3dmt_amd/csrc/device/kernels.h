@@ -21,8 +21,11 @@ struct Launch {
     static void bicg_init(hipStream_t s, const DevOp<V>& op, const double* omega, const C* b,
                           const C* x, C* r, C* r0, SysState* st, double* part,
                           int x_zero);
+    // zl != nullptr (the last colour only): also write the backward result
+    // z = y of these rows, which have no later colours.
     static void sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
-                          bool pupd, const C* in, C* p, const C* v, C* y, const SysState* st);
+                          bool pupd, const C* in, C* p, const C* v, C* y, const SysState* st,
+                          C* zl = nullptr);
     static void sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
                           const C* y, C* z, const SysState* st);
     // ys != nullptr: z is SSOR-preconditioned and ys its forward-sweep output

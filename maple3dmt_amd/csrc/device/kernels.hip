@@ -298,11 +298,14 @@ __global__ __launch_bounds__(kFinBlock) void k_bicg_init_fin(SysState* st, const
 //   y_i = (in_i - sum_{j in earlier colours} a_ij y_j) / d_i   (ssor.h:58-65)
 // With PUPD the input is the search direction, updated first:
 //   p = r (first iteration) or p = r + beta (p - omega v)   (bicgstab.cpp:60-67)
+// With zl (the last colour, which has no later colours) the backward step of
+// these rows is done here too: z_i = y_i - (0 / d_i), the backward sweep's
+// arithmetic with an empty upper block.
 template <int NS, typename V, int SUB, bool PUPD>
 __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
                             const cvec<V>* __restrict__ in, cvec<V>* __restrict__ p,
                             const cvec<V>* __restrict__ v, cvec<V>* __restrict__ y,
-                            const SysState* __restrict__ st) {
+                            const SysState* __restrict__ st, cvec<V>* __restrict__ zl) {
     const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
     constexpr int rpb = sweep_rpb<NS, SUB>();
     if (!((status_mask<NS>(st, kRunning) >> sys) & 1u)) return;
@@ -344,7 +347,16 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
         cvec<V> out;
         out.x = er * dr - ei * di;
         out.y = er * di + ei * dr;
-        if (sub == 0) y[i] = out;
+        if (sub == 0) {
+            y[i] = out;
+            if (zl) {
+                const V zr = 0, zi = 0;
+                cvec<V> zo;
+                zo.x = out.x - (zr * dr - zi * di);
+                zo.y = out.y - (zr * di + zi * dr);
+                zl[i] = zo;
+            }
+        }
     }
 }
 
@@ -1381,14 +1393,15 @@ int sweep_grid(const DevOp<V>& op, int rows) {
 
 template <int NS, typename V, int SUB>
 void sweep_fwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1, bool pupd,
-                  const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, const SysState* st) {
+                  const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, const SysState* st,
+                  cvec<V>* zl) {
     const dim3 g(sweep_grid<NS, SUB>(op, r1 - r0));
     if (pupd)
         hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, true>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
-                           in, p, v, y, st);
+                           in, p, v, y, st, zl);
     else
         hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, false>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
-                           in, p, v, y, st);
+                           in, p, v, y, st, zl);
 }
 template <int NS, typename V, int SUB>
 void sweep_bwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
@@ -1400,12 +1413,12 @@ void sweep_bwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0
 template <int NS, typename V>
 void Launch<NS, V>::sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
                               int r1, bool pupd, const cvec<V>* in, cvec<V>* p, const cvec<V>* v,
-                              cvec<V>* y, const SysState* st) {
+                              cvec<V>* y, const SysState* st, cvec<V>* zl) {
     const int sub = op.shape[width_index(NS)].sweep_sub;
-    if (sub == 8 && sub_ok<NS, 8>()) sweep_fwd_at<NS, V, (sub_ok<NS, 8>() ? 8 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
-    else if (sub == 4 && sub_ok<NS, 4>()) sweep_fwd_at<NS, V, (sub_ok<NS, 4>() ? 4 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
-    else if (sub == 2 && sub_ok<NS, 2>()) sweep_fwd_at<NS, V, (sub_ok<NS, 2>() ? 2 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
-    else sweep_fwd_at<NS, V, 1>(s, op, omega, r0, r1, pupd, in, p, v, y, st);
+    if (sub == 8 && sub_ok<NS, 8>()) sweep_fwd_at<NS, V, (sub_ok<NS, 8>() ? 8 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
+    else if (sub == 4 && sub_ok<NS, 4>()) sweep_fwd_at<NS, V, (sub_ok<NS, 4>() ? 4 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
+    else if (sub == 2 && sub_ok<NS, 2>()) sweep_fwd_at<NS, V, (sub_ok<NS, 2>() ? 2 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
+    else sweep_fwd_at<NS, V, 1>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
 }
 template <int NS, typename V>
 void Launch<NS, V>::sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,

@@ -200,6 +200,13 @@ class FlatIndex {
     std::vector<int> val_;
 };
 
+// The last colour's backward step folded into its forward launch unless
+// M3D_FUSE_LAST_COLOUR=0 (read at each operator build).
+bool fuse_last_colour() {
+    const char* v = std::getenv("M3D_FUSE_LAST_COLOUR");
+    return !(v && v[0] == '0');
+}
+
 // Packed SELL blocks unless M3D_PACKED_SELL=0 (read at each operator build).
 bool packed_sell() {
     const char* v = std::getenv("M3D_PACKED_SELL");
@@ -261,6 +268,8 @@ struct m3d_operator {
     DBuf<double> vtab;
     DBuf<float> vtab32;
     int cbits = 0, ntab = 0;
+    // The last colour has no upper couplings (its backward step is z = y).
+    bool last_upper_empty = false;
     DBuf<unsigned char> bc, constrained;
     std::vector<unsigned char> h_constrained;
     bool mass_set = false;
@@ -393,26 +402,37 @@ std::vector<SysState> read_states(m3d_operator& op, const DBuf<SysState>& st) {
 }
 
 // One BiCGStab iteration (bicgstab.cpp:50-160) for every running system.
+// One multicolour SSOR application z = (D+U)^-1 D (D+L)^-1 in: forward
+// sweeps over the colours in order (with pupd the input is the search
+// direction, updated in the first sweep), then backward sweeps in reverse
+// order.  The last colour couples only to earlier colours, so when its
+// upper block is empty its backward step z = y is written by its forward
+// launch and the backward sweeps start one colour earlier.
+template <int NS, typename V>
+void ssor_sweeps(hipStream_t s, const m3d_operator& op, const DevOp<V>& A, const double* omega, bool pupd,
+                 const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, cvec<V>* z,
+                 const SysState* st) {
+    using L = Launch<NS, V>;
+    const int K = op.ncolours;
+    const bool fuse = op.last_upper_empty;
+    for (int c = 0; c < K; ++c)
+        L::sweep_fwd(s, A, omega, op.colour_start[c], op.colour_start[c + 1], pupd, in, p, v, y, st,
+                     fuse && c == K - 1 ? z : nullptr);
+    for (int c = K - (fuse ? 2 : 1); c >= 0; --c)
+        L::sweep_bwd(s, A, omega, op.colour_start[c], op.colour_start[c + 1], y, z, st);
+}
+
 template <int NS, typename V>
 void bicg_iteration(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x) {
     hipStream_t s = op.ctx->stream;
     const DevOp<V> A = op.view<V>();
     using L = Launch<NS, V>;
-    const int K = op.ncolours;
-    for (int c = 0; c < K; ++c)
-        L::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], true, w.r.p, w.p.p,
-                     w.v.p, w.y.p, W.st.p);
-    for (int c = K - 1; c >= 0; --c)
-        L::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p, w.ph.p, W.st.p);
+    ssor_sweeps<NS, V>(s, op, A, W.omega.p, true, w.r.p, w.p.p, w.v.p, w.y.p, w.ph.p, W.st.p);
     // v = A p_hat; in Eisenstat form from y, the forward-sweep output of p.
     const bool eis = use_eisenstat();
     L::spmv_v(s, A, W.omega.p, w.ph.p, eis ? w.y.p : nullptr, w.r0.p, w.v.p, W.st.p, W.part.p);
     L::bicg_s(s, op.npad, w.r.p, w.v.p, w.s.p, W.st.p, W.part.p);
-    for (int c = 0; c < K; ++c)
-        L::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], false, w.s.p,
-                     nullptr, nullptr, w.y.p, W.st.p);
-    for (int c = K - 1; c >= 0; --c)
-        L::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p, w.sh.p, W.st.p);
+    ssor_sweeps<NS, V>(s, op, A, W.omega.p, false, w.s.p, nullptr, nullptr, w.y.p, w.sh.p, W.st.p);
     L::spmv_t(s, A, W.omega.p, w.sh.p, eis ? w.y.p : nullptr, w.s.p, w.t.p, W.st.p, W.part.p);
     L::bicg_xr(s, op.npad, x, w.ph.p, w.sh.p, w.s.p, w.t.p, w.r0.p, w.r.p, W.st.p, W.part.p);
 }
@@ -1241,6 +1261,9 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
         }
     }
     op.stored = ltot + utot;
+    op.last_upper_empty = K > 0 && fuse_last_colour();
+    for (int sl = K > 0 ? op.colour_start[K - 1] / kSlice : nsl; sl < nsl; ++sl)
+        if (uw[sl] != 0) op.last_upper_empty = false;
     op.nnz = nz;
     std::vector<double> ds(op.npad, 1.0);
     std::vector<unsigned char> bc(op.npad, 1), cons(op.npad, 1);
@@ -1794,12 +1817,7 @@ void ssor_impl(m3d_operator& op, int nsys, const double* omega, const double2* d
     launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
     Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_r, W.x.p);
     const DevOp<double> A = op.view<double>();
-    for (int c = 0; c < op.ncolours; ++c)
-        Launch<NS, double>::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], false,
-                                      W.x.p, nullptr, nullptr, w.y.p, W.st.p);
-    for (int c = op.ncolours - 1; c >= 0; --c)
-        Launch<NS, double>::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1], w.y.p,
-                                      W.tmp.p, W.st.p);
+    ssor_sweeps<NS, double>(s, op, A, W.omega.p, false, W.x.p, nullptr, nullptr, w.y.p, W.tmp.p, W.st.p);
     Launch<NS, double>::unpack(s, op.npad, op.n, nsys, op.d_perm.p, W.tmp.p, d_z);
     launch_bicg_begin(s, W.st.p, 0u, 1, 1.0);
     HIPCHK(hipGetLastError());
@@ -1865,6 +1883,7 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
         launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
     }
     Launch<NS, double>::fill_zero(s, nv, W.x.p);
+    const int bwd_last = op.ncolours - (op.last_upper_empty ? 2 : 1);
     auto one = [&]() {
         if (kind == 0) {
             Launch<NS, double>::spmv(s, A, W.omega.p, W.x.p, nullptr, W.tmp.p);
@@ -1874,16 +1893,13 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
             auto& w32 = vec32<NS>(op, W);
             bicg_iteration<NS, float>(op, W, w32, w32.x.p);
         } else if (kind == 4) {
-            for (int c = op.ncolours - 1; c >= 0; --c)
+            // The backward launches an SSOR application makes.
+            for (int c = bwd_last; c >= 0; --c)
                 Launch<NS, double>::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
                                               W.x.p, W.tmp.p, W.st.p);
         } else {
-            for (int c = 0; c < op.ncolours; ++c)
-                Launch<NS, double>::sweep_fwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
-                                              false, W.x.p, nullptr, nullptr, w.y.p, W.st.p);
-            for (int c = op.ncolours - 1; c >= 0; --c)
-                Launch<NS, double>::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
-                                              w.y.p, W.tmp.p, W.st.p);
+            ssor_sweeps<NS, double>(s, op, A, W.omega.p, false, W.x.p, nullptr, nullptr, w.y.p, W.tmp.p,
+                                    W.st.p);
         }
     };
     one();  // warm
@@ -1923,9 +1939,11 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     launch_bicg_begin(s, W.st.p, 0u, 1, 1.0);
     HIPCHK(hipStreamSynchronize(s));
     // Kind 4 reports one launch: the mean over the colours of a sweep.
-    *avg_ms = double(ms) / (units * per) / (kind == 4 ? op.ncolours : 1);
-    long long nfree = 0;
+    *avg_ms = double(ms) / (units * per) / (kind == 4 ? bwd_last + 1 : 1);
+    long long nfree = 0, nfree_bwd = 0;
     for (int e = 0; e < op.n; ++e) nfree += op.h_constrained[e] ? 0 : 1;
+    for (int r = 0; r < op.colour_start[bwd_last + 1]; ++r)
+        nfree_bwd += (op.perm[r] >= 0 && !op.h_constrained[op.perm[r]]) ? 1 : 0;
     const double m = double(op.nnz), n = double(nfree), k = double(nsys);
     // Algorithmic bytes: values + columns of the m off-diagonal nonzeros, the
     // (S_ii, Me_i) diagonal pair, and each vector touched once.
@@ -1939,9 +1957,10 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
         return 2.0 * spmv_b(wv, wc) + 2.0 * ssor_b(wv, wc) + 17.0 * k * n * wc;
     };
     // Backward sweep: the upper half of the couplings, the diagonal pair, y
-    // in and z out, per launch (one colour on average).
+    // in and z out of the rows it visits, per launch (one colour on average).
+    const double nb = double(nfree_bwd);
     auto bwd_b = [&](double wv, double wc) {
-        return (0.5 * m * ent(wv) + 2.0 * n * wv + 2.0 * k * n * wc) / op.ncolours;
+        return (0.5 * m * ent(wv) + 2.0 * nb * wv + 2.0 * k * nb * wc) / (bwd_last + 1);
     };
     *bytes = kind == 0   ? spmv_b(8, 16)
              : kind == 1 ? ssor_b(8, 16)

@@ -102,3 +102,43 @@ def test_solve_bitwise(pair, all_packed, precision):
     assert out[0][1] == out[1][1]
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.fixture(scope="module")
+def unfused(ctx, pair):
+    """The packed operator's twin with the last colour's backward step in a
+    launch of its own (M3D_FUSE_LAST_COLOUR=0)."""
+    wl = pair[0]
+    old = os.environ.get("M3D_FUSE_LAST_COLOUR")
+    os.environ["M3D_FUSE_LAST_COLOUR"] = "0"
+    try:
+        m = configs.build_model(wl, ctx)
+    finally:
+        if old is None:
+            del os.environ["M3D_FUSE_LAST_COLOUR"]
+        else:
+            os.environ["M3D_FUSE_LAST_COLOUR"] = old
+    yield m
+    m.close()
+
+
+@pytest.mark.parametrize("k", [1, 2, 16])
+def test_fused_last_colour_bitwise(pair, unfused, k):
+    """Writing z = y of the last colour from its forward launch gives the
+    separate backward launch's values bit for bit, in the SSOR apply and in
+    whole solves."""
+    wl, mp, _ = pair
+    rng = np.random.default_rng(10 + k)
+    om = 2 * np.pi * 10.0 ** rng.uniform(-3, 2, k)
+    x = rng.standard_normal((k, mp.n)) + 1j * rng.standard_normal((k, mp.n))
+    assert np.array_equal(mp.device.ssor_apply(om, x), unfused.device.ssor_apply(om, x))
+    if k == 2:
+        omegas = np.repeat(2 * np.pi * wl.freqs[:1], 2)
+        out = []
+        for m in (mp, unfused):
+            rhs = np.stack([m.rhs_total(m.bc_values(w, p % 2)) for p, w in enumerate(omegas)])
+            aux = np.stack([m.primary_field(w, p % 2) for p, w in enumerate(omegas)])
+            xs, res = m.device.solve(MODE_FORWARD, omegas, rhs, aux, params=m.fwd_params.solve_params())
+            out.append((xs, [r["iterations"] for r in res]))
+        assert out[0][1] == out[1][1]
+        assert np.array_equal(out[0][0], out[1][0])
