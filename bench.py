@@ -31,6 +31,10 @@ against it; and
   spmv           the batched SpMV of all 16 systems (the metric's SpMV TB/s)
   multi_model    misfit gradients (forward + adjoint systems) of several models
                  on meshes of uneven size, sequential and concurrent
+  network        the e0 network's BF16 MFMA kernels: inference and one training
+                 step on a batch of 8 samples, achieved TFLOP/s against the
+                 dense BF16 peak, plus the rocprofv3 MfmaUtil / MfmaFlopsBF16
+                 counters of the committed profile
   cpu_baseline   the CPU oracle timed on a bounded sample of the same workload
 """
 from __future__ import annotations
@@ -47,10 +51,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16
 WEIGHTS = os.path.join(ROOT, "maple3dmt_amd", "data", "e0_net.npz")
 # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
 # calibrated on a 1 GiB device copy (see the summary beside it).
 TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+# Per-kernel MFMA counters of the network (rocprofv3 --pmc MfmaUtil MfmaFlopsBF16).
+MFMA = os.path.join(ROOT, "profiles", "mfma_e0_network.json")
 
 
 def measured_traffic(kernel: str):
@@ -91,6 +98,8 @@ def parse():
                     help="profiling mode: build the workload, then only run the benchmark "
                          "kernels (SpMV, SSOR, BiCGStab iterations, backward sweep) and four "
                          "1 GiB device copies (counter calibration)")
+    ap.add_argument("--network-only", action="store_true",
+                    help="profiling mode: only the e0 network's inference and training steps")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of tuning them on the bench "
@@ -150,6 +159,12 @@ def main():
 
     if args.kernel_only:
         kernel_only(args)
+        return
+    if args.network_only:
+        from maple3dmt_amd import Context
+        ctx = Context(0)
+        print(json.dumps(network_leg(args, ctx)), flush=True)
+        ctx.close()
         return
 
     import torch
@@ -322,6 +337,8 @@ def main():
     if args.models > 0:
         models = multi_model_leg(args, model, ctx)
 
+    network = network_leg(args, ctx)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(model, wl, args.cpu_budget, iters)
@@ -360,6 +377,7 @@ def main():
         "gradient": gradient,
         "variants": legs,
         "multi_model": models,
+        "network": network,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -411,6 +429,70 @@ def multi_model_leg(args, bench_model, ctx):
         j.model.close()
     for c in contexts:
         c.close()
+    return out
+
+
+# Layer shapes of the e0 network: (input channels, output channels) of the
+# four 3x3x3 convolutions on a 32^3 grid.
+NET_LAYERS = ((2, 32), (32, 32), (32, 32), (32, 12))
+
+
+def network_leg(args, ctx):
+    """BF16 MFMA throughput of the e0 network on a batch of 8 random samples:
+    inference (forward) and one training step (forward, input and weight
+    gradients, Adam at lr 0), timed over repeated blocking calls.  Useful
+    flops per sample and layer 2 * 27 * Cin * Cout * 32^3; a training step
+    adds the input gradient of layers 2-4 and the weight gradient of all
+    four."""
+    import torch
+
+    from maple3dmt_amd.learned import CoarseNet
+    dil = (1, 4, 8, 16)
+    if os.path.exists(args.weights):
+        with np.load(args.weights) as wts:
+            dil = tuple(int(v) for v in wts["dilations"])
+    batch, vox = 8, 32 ** 3
+    net = CoarseNet(ctx, max_batch=batch, dilations=dil)
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(1)
+    d_in = torch.randn(batch * vox * 2, device=dev, dtype=torch.float32, generator=g)
+    d_tgt = torch.randn(batch * vox * 12, device=dev, dtype=torch.float32, generator=g)
+    d_out = torch.empty(batch * vox * 12, device=dev, dtype=torch.float32)
+    torch.cuda.synchronize()
+    per = 2 * 27 * vox
+    f_fwd = batch * per * sum(ci * co for ci, co in NET_LAYERS)
+    f_train = f_fwd + batch * per * (sum(ci * co for ci, co in NET_LAYERS[1:]) +
+                                     sum(ci * co for ci, co in NET_LAYERS))
+    reps = max(4, args.reps)
+    for _ in range(3):
+        net.forward(d_in.data_ptr(), d_out.data_ptr(), batch)
+        net.train_step(d_in.data_ptr(), d_tgt.data_ptr(), batch, 0.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        net.forward(d_in.data_ptr(), d_out.data_ptr(), batch)
+    torch.cuda.synchronize()
+    ms_fwd = (time.perf_counter() - t) * 1e3 / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(d_in.data_ptr(), d_tgt.data_ptr(), batch, 0.0)
+    torch.cuda.synchronize()
+    ms_train = (time.perf_counter() - t) * 1e3 / reps
+    net.close()
+    tf_fwd = f_fwd / (ms_fwd * 1e-3) / 1e12
+    tf_train = f_train / (ms_train * 1e-3) / 1e12
+    out = {"batch": batch, "grid": 32, "layers": [list(x) for x in NET_LAYERS], "dilations": list(dil),
+           "forward_ms": ms_fwd, "forward_gflop": f_fwd / 1e9, "forward_TFLOPs": tf_fwd,
+           "train_step_ms": ms_train, "train_step_gflop": f_train / 1e9, "train_step_TFLOPs": tf_train,
+           "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS, "forward_frac": tf_fwd / MFMA_BF16_PEAK_TFLOPS,
+           "train_step_frac": tf_train / MFMA_BF16_PEAK_TFLOPS, "counters": None}
+    try:
+        with open(MFMA) as f:
+            out["counters"] = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        pass
+    log(f"[bench] network: forward {ms_fwd:.3f} ms ({tf_fwd:.1f} TFLOP/s), train step {ms_train:.3f} ms "
+        f"({tf_train:.1f} TFLOP/s)")
     return out
 
 
