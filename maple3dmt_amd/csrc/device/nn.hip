@@ -1,6 +1,8 @@
 // Coarse-grid neural operator for the initial field e0: four 3x3x3 dilated
 // conv layers on a 32^3 grid, BF16 storage, FP32 accumulation on the CDNA
-// matrix cores (v_mfma_f32_32x32x8_bf16), trained with Adam on the device.
+// matrix cores (v_mfma_f32_32x32x16_bf16 over 32-channel inputs,
+// v_mfma_f32_32x32x8_bf16 over the padded 8-channel input and in the weight
+// gradient), trained with Adam on the device.
 //
 // Every conv pass is an implicit GEMM whose 32x32 MFMA tile is one x-row of
 // the grid (32 voxels) against 32 channels:
@@ -32,7 +34,20 @@ constexpr int COUT = 12;              // output channels (2 pol x 3 comp x re/im
 constexpr int CIN_USED = 2;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f16 __attribute__((ext_vector_type(16)));
+
+// 32x32x16 BF16 MFMA: lane l holds row (A) / column (B) l % 32 and the eight
+// k values 8 (l / 32) .. 8 (l / 32) + 7 of the 16-wide k step; the
+// accumulator layout is that of the 32x32x8 form.
+__device__ inline f16 mfma16(s8 a, s8 b, f16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0,
+                                                   0);
+}
+
+// Taps whose operands are in flight ahead of the MFMAs of the current tap.
+constexpr int kTapStages = 3;
 
 __device__ inline unsigned short f2bf(float f) {
     unsigned u = __float_as_uint(f);
@@ -49,8 +64,11 @@ __device__ inline void tap_offsets(int tap, int& dz, int& dy, int& dx) {
 
 // Forward conv of one layer.  One wavefront = one x-row (b, z, y) x 32 output
 // channels.  Lane l: voxel x = l % 32 for the A operand, output channel
-// l % 32 for B; k-quad q = l / 32 selects channels 4q..4q+3 of each 8-wide
-// k step.  Accumulator r of lane l is D[x = 8(r/4) + 4q + r%4][co = l % 32].
+// l % 32 for B; q = l / 32 selects channels 4q..4q+3 of each 8-wide k step
+// (CIN = 8) or 8q..8q+7 of each 16-wide one (CIN a multiple of 16).
+// Accumulator r of lane l is D[x = 8(r/4) + 4q + r%4][co = l % 32].  With
+// 16-wide steps the 27 taps are unrolled and the operands of the next
+// kTapStages - 1 taps are loaded while the current tap's MFMAs run.
 template <int CIN, bool RELU, bool OUTF32>
 __global__ __launch_bounds__(256) void k_conv_fwd(int batch, int dil,
                                                   const unsigned short* __restrict__ in,
@@ -65,20 +83,57 @@ __global__ __launch_bounds__(256) void k_conv_fwd(int batch, int dil,
     const int m = lane & 31, q = lane >> 5;
     f16 acc;
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-    for (int tap = 0; tap < 27; ++tap) {
-        int dz, dy, dx;
-        tap_offsets(tap, dz, dy, dx);
-        const int zz = z + dz * dil, yy = y + dy * dil, xx = m + dx * dil;
-        const bool ok = zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G;
-        const size_t vin = ((size_t(b) * G + (ok ? zz : 0)) * G + (ok ? yy : 0)) * G + (ok ? xx : 0);
-        const unsigned short* src = in + vin * CIN;
-        const unsigned short* wsrc = wt + (size_t(tap) * HID + m) * CIN;
+    if constexpr (CIN % 16 == 0) {
+        constexpr int NK = CIN / 16;
+        s8 a[kTapStages][NK], w[kTapStages][NK];
+        auto fetch = [&](int tap, s8 (&A)[NK], s8 (&W)[NK]) {
+            int dz, dy, dx;
+            tap_offsets(tap, dz, dy, dx);
+            const int zz = z + dz * dil, yy = y + dy * dil, xx = m + dx * dil;
+            const bool ok = zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G;
+            const size_t vin = ((size_t(b) * G + (ok ? zz : 0)) * G + (ok ? yy : 0)) * G + (ok ? xx : 0);
+            const unsigned short* src = in + vin * CIN;
+            const unsigned short* wsrc = wt + (size_t(tap) * HID + m) * CIN;
 #pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 8) {
-            s4 a = {0, 0, 0, 0};
-            if (ok) a = *reinterpret_cast<const s4*>(src + c0 + 4 * q);
-            const s4 w = *reinterpret_cast<const s4*>(wsrc + c0 + 4 * q);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w, acc, 0, 0, 0);
+            for (int k = 0; k < NK; ++k) {
+                s8 t = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (ok) t = *reinterpret_cast<const s8*>(src + 16 * k + 8 * q);
+                A[k] = t;
+                W[k] = *reinterpret_cast<const s8*>(wsrc + 16 * k + 8 * q);
+            }
+        };
+#pragma unroll
+        for (int t = 0; t < kTapStages - 1; ++t) fetch(t, a[t], w[t]);
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            if (tap + kTapStages - 1 < 27)
+                fetch(tap + kTapStages - 1, a[(tap + kTapStages - 1) % kTapStages],
+                      w[(tap + kTapStages - 1) % kTapStages]);
+#pragma unroll
+            for (int k = 0; k < NK; ++k) acc = mfma16(a[tap % kTapStages][k], w[tap % kTapStages][k], acc);
+        }
+    } else {
+        static_assert(CIN == 8, "8-wide k steps serve the padded input layer");
+        s4 a[kTapStages], w[kTapStages];
+        auto fetch = [&](int tap, s4& A, s4& W) {
+            int dz, dy, dx;
+            tap_offsets(tap, dz, dy, dx);
+            const int zz = z + dz * dil, yy = y + dy * dil, xx = m + dx * dil;
+            const bool ok = zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G;
+            const size_t vin = ((size_t(b) * G + (ok ? zz : 0)) * G + (ok ? yy : 0)) * G + (ok ? xx : 0);
+            s4 t = {0, 0, 0, 0};
+            if (ok) t = *reinterpret_cast<const s4*>(in + vin * CIN + 4 * q);
+            A = t;
+            W = *reinterpret_cast<const s4*>(wt + (size_t(tap) * HID + m) * CIN + 4 * q);
+        };
+#pragma unroll
+        for (int t = 0; t < kTapStages - 1; ++t) fetch(t, a[t], w[t]);
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            if (tap + kTapStages - 1 < 27)
+                fetch(tap + kTapStages - 1, a[(tap + kTapStages - 1) % kTapStages],
+                      w[(tap + kTapStages - 1) % kTapStages]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[tap % kTapStages], w[tap % kTapStages], acc, 0, 0, 0);
         }
     }
     const int co = m;
@@ -98,7 +153,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(int batch, int dil,
 }
 
 // Input gradient of a 32-channel layer: dz = dA * 1[act > 0] (when act is
-// given) gathered at v - d*tap; B operand lane l reads W[tap][co-quad][ci = l%32].
+// given) gathered at v - d*tap; B operand lane l reads W[tap][ci = l%32][co
+// k-octet].  16-wide MFMA steps, the taps pipelined as in the forward pass.
 __global__ __launch_bounds__(256) void k_conv_bwd_data(int batch, int dil,
                                                        const unsigned short* __restrict__ dout,
                                                        const unsigned short* __restrict__ act,
@@ -111,7 +167,9 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(int batch, int dil,
     const int m = lane & 31, q = lane >> 5;
     f16 acc;
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-    for (int tap = 0; tap < 27; ++tap) {
+    constexpr int NK = HID / 16;
+    s8 a[kTapStages][NK], g[kTapStages][NK], w[kTapStages][NK];
+    auto fetch = [&](int tap, s8 (&A)[NK], s8 (&M)[NK], s8 (&W)[NK]) {
         int dz, dy, dx;
         tap_offsets(tap, dz, dy, dx);
         const int zz = z - dz * dil, yy = y - dy * dil, xx = m - dx * dil;
@@ -121,18 +179,34 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(int batch, int dil,
         const unsigned short* msk = act ? act + vsrc * HID : nullptr;
         const unsigned short* wsrc = wb + (size_t(tap) * HID + m) * HID;  // [tap][ci][co]
 #pragma unroll
-        for (int c0 = 0; c0 < HID; c0 += 8) {
-            s4 a = {0, 0, 0, 0};
+        for (int k = 0; k < NK; ++k) {
+            s8 t = {0, 0, 0, 0, 0, 0, 0, 0};
+            s8 u = {1, 1, 1, 1, 1, 1, 1, 1};  // no mask: every entry passes
             if (ok) {
-                a = *reinterpret_cast<const s4*>(src + c0 + 4 * q);
-                if (msk) {
-                    const s4 g = *reinterpret_cast<const s4*>(msk + c0 + 4 * q);
-                    for (int j = 0; j < 4; ++j)
-                        if (bf2f(static_cast<unsigned short>(g[j])) <= 0.f) a[j] = 0;
-                }
+                t = *reinterpret_cast<const s8*>(src + 16 * k + 8 * q);
+                if (msk) u = *reinterpret_cast<const s8*>(msk + 16 * k + 8 * q);
             }
-            const s4 w = *reinterpret_cast<const s4*>(wsrc + c0 + 4 * q);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w, acc, 0, 0, 0);
+            A[k] = t;
+            M[k] = u;
+            W[k] = *reinterpret_cast<const s8*>(wsrc + 16 * k + 8 * q);
+        }
+    };
+#pragma unroll
+    for (int t = 0; t < kTapStages - 1; ++t) fetch(t, a[t], g[t], w[t]);
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap) {
+        if (tap + kTapStages - 1 < 27) {
+            const int nx = (tap + kTapStages - 1) % kTapStages;
+            fetch(tap + kTapStages - 1, a[nx], g[nx], w[nx]);
+        }
+        const int cur = tap % kTapStages;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            s8 av = a[cur][k];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (bf2f(static_cast<unsigned short>(g[cur][k][j])) <= 0.f) av[j] = 0;
+            acc = mfma16(av, w[cur][k], acc);
         }
     }
     const size_t vrow = ((size_t(b) * G + z) * G + y) * G;
