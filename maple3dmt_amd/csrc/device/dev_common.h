@@ -168,6 +168,9 @@ __device__ inline unsigned status_mask(const SysState* __restrict__ st, int want
 // fence (on CDNA that writes back the XCD's L2) or a same-address atomic.
 
 constexpr int kFinBlock = 1024;  // threads of a finishing kernel
+// Offset of the ||s||^2 partials in a workspace's partials buffer (after
+// three quantities x kMaxSys systems x kMaxBlocks blocks).
+constexpr size_t kPartSNorm = size_t(3) * kMaxSys * kMaxBlocks;
 
 // Block stage: sums NQ per-thread values over the threads of each system in
 // the block and stores them block-major, part[blockIdx * P + p] with

@@ -11,6 +11,10 @@ void launch_sensitivity(hipStream_t s, int nc, int n, int nsys, const int* cptr,
                         const double* wgt, const double* sigma, const double* omega, const double2* E,
                         const double2* lam, double* g);
 
+// Input of a forward sweep: in as given, the updated search direction
+// p = r + beta (p - omega v), or s = r - alpha v (both written out).
+enum SweepInput : int { kInPlain = 0, kInPUpdate = 1, kInSUpdate = 2 };
+
 template <int NS, typename V>
 struct Launch {
     using C = cvec<V>;
@@ -21,10 +25,11 @@ struct Launch {
     static void bicg_init(hipStream_t s, const DevOp<V>& op, const double* omega, const C* b,
                           const C* x, C* r, C* r0, SysState* st, double* part,
                           int x_zero);
-    // zl != nullptr (the last colour only): also write the backward result
-    // z = y of these rows, which have no later colours.
+    // mode: SweepInput (in = r and p the output for the updates).  zl !=
+    // nullptr (the last colour only): also write the backward result z = y
+    // of these rows, which have no later colours.
     static void sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
-                          bool pupd, const C* in, C* p, const C* v, C* y, const SysState* st,
+                          int mode, const C* in, C* p, const C* v, C* y, const SysState* st,
                           C* zl = nullptr);
     static void sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
                           const C* y, C* z, const SysState* st);
@@ -32,8 +37,6 @@ struct Launch {
     // (Eisenstat form, lower couplings only); nullptr: plain A z.
     static void spmv_v(hipStream_t s, const DevOp<V>& op, const double* omega, const C* z,
                        const C* ys, const C* r0, C* v, SysState* st, double* part);
-    static void bicg_s(hipStream_t s, int n, const C* r, const C* v, C* sv, SysState* st,
-                       double* part);
     static void spmv_t(hipStream_t s, const DevOp<V>& op, const double* omega, const C* z,
                        const C* ys, const C* sv, C* t, SysState* st, double* part);
     static void bicg_xr(hipStream_t s, int n, C* x, const C* ph, const C* sh, const C* sv,

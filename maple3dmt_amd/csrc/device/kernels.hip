@@ -163,6 +163,9 @@ __device__ inline cvec<V> apply_row_eis(const DevOp<V>& op, int row, int NS, int
     return out;
 }
 
+__device__ inline double fmav(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fmav(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename V>
 __device__ inline cvec<V> to_v(double2 a) {
     cvec<V> r;
@@ -296,12 +299,18 @@ __global__ __launch_bounds__(kFinBlock) void k_bicg_init_fin(SysState* st, const
 
 // Forward multicolour Gauss-Seidel sweep over one colour [r0, r1):
 //   y_i = (in_i - sum_{j in earlier colours} a_ij y_j) / d_i   (ssor.h:58-65)
-// With PUPD the input is the search direction, updated first:
-//   p = r (first iteration) or p = r + beta (p - omega v)   (bicgstab.cpp:60-67)
+// The input, by MODE:
+//   kInPlain   in
+//   kInPUpdate the search direction, updated first (in = r, out p):
+//              p = r (first iteration) or p = r + beta (p - omega v)
+//              (bicgstab.cpp:60-67)
+//   kInSUpdate s = r - alpha v (in = r, out s; bicgstab.cpp:88-92), which
+//              the sweep needs row by row, so it is formed here instead of
+//              in a pass of its own; its norm is reduced by k_spmv_t.
 // With zl (the last colour, which has no later colours) the backward step of
 // these rows is done here too: z_i = y_i - (0 / d_i), the backward sweep's
 // arithmetic with an empty upper block.
-template <int NS, typename V, int SUB, bool PUPD>
+template <int NS, typename V, int SUB, int MODE>
 __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
                             const cvec<V>* __restrict__ in, cvec<V>* __restrict__ p,
                             const cvec<V>* __restrict__ v, cvec<V>* __restrict__ y,
@@ -327,7 +336,7 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
         const int4 sc = op.slc[row >> 6];
         const V ds = op.diag_s[row], me = op.me[row];
         cvec<V> src = in[i];
-        if (PUPD) {
+        if (MODE == kInPUpdate) {
             if (!first) {
                 const cvec<V> pi = p[i], vi = v[i];
                 // p - omega v
@@ -336,6 +345,17 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
                 src.x = src.x + (br * tr - bi * ti);
                 src.y = src.y + (br * ti + bi * tr);
             }
+            if (sub == 0) p[i] = src;
+        } else if (MODE == kInSUpdate) {
+            // alpha v with explicit fused multiply-adds: the rounding of s
+            // is fixed by the source, not by what contraction the compiler
+            // picks for this kernel.
+            const V ar = V(S.alpha.x), ai = V(S.alpha.y);
+            const cvec<V> vi = v[i];
+            const V avr = fmav(ar, vi.x, -(ai * vi.y));
+            const V avi = fmav(ai, vi.x, ar * vi.y);
+            src.x = src.x - avr;
+            src.y = src.y - avi;
             if (sub == 0) p[i] = src;
         }
         const int lane = row & 63;
@@ -438,50 +458,12 @@ __global__ __launch_bounds__(kFinBlock) void k_spmv_v_fin(SysState* st, const do
     S.alpha = cdiv(S.rho, r0v);
 }
 
-// s = r - alpha v, ||s||; early exit when ||s|| / ||b|| < tol
-// (bicgstab.cpp:89-103); x += alpha p_hat is applied by k_bicg_xr.
-template <int NS, typename V>
-__global__ void k_bicg_s(int n, const cvec<V>* __restrict__ r, const cvec<V>* __restrict__ v,
-                         cvec<V>* __restrict__ s, const SysState* __restrict__ st, double* part) {
-    const int tid = threadIdx.x, sys = tid % NS;
-    const int rpb = rows_per_block<NS>();
-    const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
-    const V ar = V(st[sys].alpha.x), ai = V(st[sys].alpha.y);
-    double acc[1] = {0.0};
-    const int nrb = (n + rpb - 1) / rpb;
-    const RowRange R = row_range(nrb);
-    for (int rb = R.first; rb < R.end; rb += R.step) {
-        const int row = rb * rpb + tid / NS;
-        if (row >= n || !run) continue;
-        const size_t i = size_t(row) * NS + sys;
-        const cvec<V> ri = r[i], vi = v[i];
-        cvec<V> si;
-        si.x = ri.x - (ar * vi.x - ai * vi.y);
-        si.y = ri.y - (ar * vi.y + ai * vi.x);
-        s[i] = si;
-        acc[0] += cabs2(to_d<V>(si));
-    }
-    block_partials<NS, 1>(acc, part);
-}
 
-template <int NS>
-__global__ __launch_bounds__(kFinBlock) void k_bicg_s_fin(SysState* st, const double* __restrict__ part, int nb) {
-    double out[1];
-    if (!finish_partials<NS, 1>(part, nb, out)) return;
-    const int tid = threadIdx.x;
-    SysState& S = st[tid];
-    if (S.status != kRunning) return;
-    S.snorm = sqrt(out[0]);
-    if (S.snorm / S.bnorm < S.tol) {
-        S.status = kPendConvEarly;
-        S.residual = S.snorm / S.bnorm;
-        S.iter = S.iter + 1;
-        S.info = 1;
-    }
-}
 
-// t = A s_hat, (t,t), (t,s); omega = (t,s)/(t,t) (bicgstab.cpp:112-125).
-// With ys (the forward-sweep output of s_hat) t = L s_hat + D ys.
+// t = A s_hat, (t,t), (t,s) and ||s||^2 (s was formed by the forward sweeps
+// of s_hat); the finish tests ||s|| (bicgstab.cpp:94-103), then
+// omega = (t,s)/(t,t) (:112-125).  With ys (the forward-sweep output of
+// s_hat) t = L s_hat + D ys.
 template <int NS, typename V>
 __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
                          const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys,
@@ -491,7 +473,7 @@ __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
     const int rpb = rows_per_block<NS>();
     const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
     const V w = V(omega[sys]);
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int nrb = (op.n + rpb - 1) / rpb;
     const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
@@ -502,20 +484,42 @@ __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
         t[i] = y;
         const double2 td = to_d<V>(y);
         acc[0] += cabs2(td);
-        const double2 ts = cconjmul(td, to_d<V>(s[i]));
+        const double2 sd = to_d<V>(s[i]);
+        const double2 ts = cconjmul(td, sd);
         acc[1] += ts.x;
         acc[2] += ts.y;
+        acc[3] += cabs2(sd);
     }
-    block_partials<NS, 3>(acc, part);
+    // (t,t) and (t,s) in the three-quantity layout of the other reductions
+    // (the finishing tree groups blocks by the number of quantities, so the
+    // layout fixes their rounding); ||s||^2 in a region of its own.
+    double tq[3] = {acc[0], acc[1], acc[2]};
+    double sq[1] = {acc[3]};
+    block_partials<NS, 3>(tq, part);
+    block_partials<NS, 1>(sq, part + kPartSNorm);
 }
 
 template <int NS>
+// ||s|| first (bicgstab.cpp:94-103): a system whose s has converged stops
+// with x += alpha p_hat pending, whatever t is; then the (t, t) breakdown
+// guard and omega = (t, s) / (t, t).
 __global__ __launch_bounds__(kFinBlock) void k_spmv_t_fin(SysState* st, const double* __restrict__ part, int nb) {
-    double out[3];
-    if (!finish_partials<NS, 3>(part, nb, out)) return;
+    double out[3], sn[1];
+    const bool mine = finish_partials<NS, 3>(part, nb, out);
+    __syncthreads();
+    finish_partials<NS, 1>(part + kPartSNorm, nb, sn);
+    if (!mine) return;
     const int tid = threadIdx.x;
     SysState& S = st[tid];
     if (S.status != kRunning) return;
+    S.snorm = sqrt(sn[0]);
+    if (S.snorm / S.bnorm < S.tol) {
+        S.status = kPendConvEarly;
+        S.residual = S.snorm / S.bnorm;
+        S.iter = S.iter + 1;
+        S.info = 1;
+        return;
+    }
     const double tt = out[0];
     if (fabs(tt) < 1e-30) {
         S.status = kPendBreakTT;
@@ -1392,15 +1396,18 @@ int sweep_grid(const DevOp<V>& op, int rows) {
 }
 
 template <int NS, typename V, int SUB>
-void sweep_fwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1, bool pupd,
+void sweep_fwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1, int mode,
                   const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, const SysState* st,
                   cvec<V>* zl) {
     const dim3 g(sweep_grid<NS, SUB>(op, r1 - r0));
-    if (pupd)
-        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, true>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
+    if (mode == kInPUpdate)
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInPUpdate>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
+                           in, p, v, y, st, zl);
+    else if (mode == kInSUpdate)
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInSUpdate>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
                            in, p, v, y, st, zl);
     else
-        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, false>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInPlain>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
                            in, p, v, y, st, zl);
 }
 template <int NS, typename V, int SUB>
@@ -1412,13 +1419,13 @@ void sweep_bwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0
 
 template <int NS, typename V>
 void Launch<NS, V>::sweep_fwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
-                              int r1, bool pupd, const cvec<V>* in, cvec<V>* p, const cvec<V>* v,
+                              int r1, int mode, const cvec<V>* in, cvec<V>* p, const cvec<V>* v,
                               cvec<V>* y, const SysState* st, cvec<V>* zl) {
     const int sub = op.shape[width_index(NS)].sweep_sub;
-    if (sub == 8 && sub_ok<NS, 8>()) sweep_fwd_at<NS, V, (sub_ok<NS, 8>() ? 8 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
-    else if (sub == 4 && sub_ok<NS, 4>()) sweep_fwd_at<NS, V, (sub_ok<NS, 4>() ? 4 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
-    else if (sub == 2 && sub_ok<NS, 2>()) sweep_fwd_at<NS, V, (sub_ok<NS, 2>() ? 2 : 1)>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
-    else sweep_fwd_at<NS, V, 1>(s, op, omega, r0, r1, pupd, in, p, v, y, st, zl);
+    if (sub == 8 && sub_ok<NS, 8>()) sweep_fwd_at<NS, V, (sub_ok<NS, 8>() ? 8 : 1)>(s, op, omega, r0, r1, mode, in, p, v, y, st, zl);
+    else if (sub == 4 && sub_ok<NS, 4>()) sweep_fwd_at<NS, V, (sub_ok<NS, 4>() ? 4 : 1)>(s, op, omega, r0, r1, mode, in, p, v, y, st, zl);
+    else if (sub == 2 && sub_ok<NS, 2>()) sweep_fwd_at<NS, V, (sub_ok<NS, 2>() ? 2 : 1)>(s, op, omega, r0, r1, mode, in, p, v, y, st, zl);
+    else sweep_fwd_at<NS, V, 1>(s, op, omega, r0, r1, mode, in, p, v, y, st, zl);
 }
 template <int NS, typename V>
 void Launch<NS, V>::sweep_bwd(hipStream_t s, const DevOp<V>& op, const double* omega, int r0,
@@ -1436,13 +1443,6 @@ void Launch<NS, V>::spmv_v(hipStream_t s, const DevOp<V>& op, const double* omeg
     const int g = row_grid<NS>(op.n, op.shape[width_index(NS)].row_cap);
     hipLaunchKernelGGL((k_spmv_v<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, r0, v, st, part);
     hipLaunchKernelGGL((k_spmv_v_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
-}
-template <int NS, typename V>
-void Launch<NS, V>::bicg_s(hipStream_t s, int n, const cvec<V>* r, const cvec<V>* v, cvec<V>* sv,
-                           SysState* st, double* part) {
-    const int g = row_grid<NS>(n);
-    hipLaunchKernelGGL((k_bicg_s<NS, V>), dim3(g), dim3(kBlock), 0, s, n, r, v, sv, st, part);
-    hipLaunchKernelGGL((k_bicg_s_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
 void Launch<NS, V>::spmv_t(hipStream_t s, const DevOp<V>& op, const double* omega,

@@ -347,7 +347,7 @@ Work64<NS>& ready_workspace(m3d_operator& op) {
     W.b.alloc(nv); W.x.alloc(nv); W.aux.alloc(nv); W.tmp.alloc(nv); W.rres.alloc(nv);
     W.omega.alloc(NS); W.scale.alloc(NS);
     W.st.alloc(kMaxSys);
-    W.part.alloc(size_t(3) * kMaxSys * kMaxBlocks);
+    W.part.alloc(size_t(4) * kMaxSys * kMaxBlocks);
     W.flags.alloc(NS);
     HIPCHK(hipMemsetAsync(W.st.p, 0, kMaxSys * sizeof(SysState), op.ctx->stream));
     W.ready = true;
@@ -403,20 +403,20 @@ std::vector<SysState> read_states(m3d_operator& op, const DBuf<SysState>& st) {
 
 // One BiCGStab iteration (bicgstab.cpp:50-160) for every running system.
 // One multicolour SSOR application z = (D+U)^-1 D (D+L)^-1 in: forward
-// sweeps over the colours in order (with pupd the input is the search
-// direction, updated in the first sweep), then backward sweeps in reverse
-// order.  The last colour couples only to earlier colours, so when its
+// sweeps over the colours in order (mode: the input as given, or formed from
+// in = r and written to p by the search-direction or s update), then
+// backward sweeps in reverse order.  The last colour couples only to earlier colours, so when its
 // upper block is empty its backward step z = y is written by its forward
 // launch and the backward sweeps start one colour earlier.
 template <int NS, typename V>
-void ssor_sweeps(hipStream_t s, const m3d_operator& op, const DevOp<V>& A, const double* omega, bool pupd,
+void ssor_sweeps(hipStream_t s, const m3d_operator& op, const DevOp<V>& A, const double* omega, int mode,
                  const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, cvec<V>* z,
                  const SysState* st) {
     using L = Launch<NS, V>;
     const int K = op.ncolours;
     const bool fuse = op.last_upper_empty;
     for (int c = 0; c < K; ++c)
-        L::sweep_fwd(s, A, omega, op.colour_start[c], op.colour_start[c + 1], pupd, in, p, v, y, st,
+        L::sweep_fwd(s, A, omega, op.colour_start[c], op.colour_start[c + 1], mode, in, p, v, y, st,
                      fuse && c == K - 1 ? z : nullptr);
     for (int c = K - (fuse ? 2 : 1); c >= 0; --c)
         L::sweep_bwd(s, A, omega, op.colour_start[c], op.colour_start[c + 1], y, z, st);
@@ -427,12 +427,13 @@ void bicg_iteration(m3d_operator& op, Work64<NS>& W, Work<NS, V>& w, cvec<V>* x)
     hipStream_t s = op.ctx->stream;
     const DevOp<V> A = op.view<V>();
     using L = Launch<NS, V>;
-    ssor_sweeps<NS, V>(s, op, A, W.omega.p, true, w.r.p, w.p.p, w.v.p, w.y.p, w.ph.p, W.st.p);
+    ssor_sweeps<NS, V>(s, op, A, W.omega.p, kInPUpdate, w.r.p, w.p.p, w.v.p, w.y.p, w.ph.p, W.st.p);
     // v = A p_hat; in Eisenstat form from y, the forward-sweep output of p.
     const bool eis = use_eisenstat();
     L::spmv_v(s, A, W.omega.p, w.ph.p, eis ? w.y.p : nullptr, w.r0.p, w.v.p, W.st.p, W.part.p);
-    L::bicg_s(s, op.npad, w.r.p, w.v.p, w.s.p, W.st.p, W.part.p);
-    ssor_sweeps<NS, V>(s, op, A, W.omega.p, false, w.s.p, nullptr, nullptr, w.y.p, w.sh.p, W.st.p);
+    // s = r - alpha v is formed by the forward sweeps of s_hat = M^-1 s; its
+    // norm test follows in the (t, t), (t, s) reduction.
+    ssor_sweeps<NS, V>(s, op, A, W.omega.p, kInSUpdate, w.r.p, w.s.p, w.v.p, w.y.p, w.sh.p, W.st.p);
     L::spmv_t(s, A, W.omega.p, w.sh.p, eis ? w.y.p : nullptr, w.s.p, w.t.p, W.st.p, W.part.p);
     L::bicg_xr(s, op.npad, x, w.ph.p, w.sh.p, w.s.p, w.t.p, w.r0.p, w.r.p, W.st.p, W.part.p);
 }
@@ -1817,7 +1818,7 @@ void ssor_impl(m3d_operator& op, int nsys, const double* omega, const double2* d
     launch_bicg_begin(s, W.st.p, mask, 1, 1.0);
     Launch<NS, double>::pack(s, op.npad, op.n, nsys, op.d_perm.p, d_r, W.x.p);
     const DevOp<double> A = op.view<double>();
-    ssor_sweeps<NS, double>(s, op, A, W.omega.p, false, W.x.p, nullptr, nullptr, w.y.p, W.tmp.p, W.st.p);
+    ssor_sweeps<NS, double>(s, op, A, W.omega.p, kInPlain, W.x.p, nullptr, nullptr, w.y.p, W.tmp.p, W.st.p);
     Launch<NS, double>::unpack(s, op.npad, op.n, nsys, op.d_perm.p, W.tmp.p, d_z);
     launch_bicg_begin(s, W.st.p, 0u, 1, 1.0);
     HIPCHK(hipGetLastError());
@@ -1898,7 +1899,7 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
                 Launch<NS, double>::sweep_bwd(s, A, W.omega.p, op.colour_start[c], op.colour_start[c + 1],
                                               W.x.p, W.tmp.p, W.st.p);
         } else {
-            ssor_sweeps<NS, double>(s, op, A, W.omega.p, false, W.x.p, nullptr, nullptr, w.y.p, W.tmp.p,
+            ssor_sweeps<NS, double>(s, op, A, W.omega.p, kInPlain, W.x.p, nullptr, nullptr, w.y.p, W.tmp.p,
                                     W.st.p);
         }
     };
