@@ -15,8 +15,10 @@ frequency runs the per-frequency loop of InversionFV::compute_gradient_
 """
 from __future__ import annotations
 
+import os
 import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -38,12 +40,34 @@ class ModelJob:
     worker: int = -1
 
 
-def prepare(jobs: Sequence[ModelJob], contexts: Sequence[Context]) -> None:
+def build_models(workloads: Sequence[configs.Workload], contexts: Sequence[Optional[Context]],
+                 n_threads: Optional[int] = None) -> List:
+    """Models of several workloads built at once, one host thread each (the
+    host builder holds no shared state and its calls release the GIL);
+    workload i goes to context i % len(contexts) (None: host-only).  The
+    models are identical to ones built one after another."""
+    n = len(workloads)
+    if n == 0:
+        return []
+    n_threads = n_threads or min(n, os.cpu_count() or 1)
+    out: List = [None] * n
+    with ThreadPoolExecutor(max_workers=n_threads) as ex:
+        futs = {ex.submit(configs.build_model, wl, contexts[i % len(contexts)]): i
+                for i, wl in enumerate(workloads)}
+        for f, i in futs.items():
+            out[i] = f.result()
+    return out
+
+
+def prepare(jobs: Sequence[ModelJob], contexts: Sequence[Context], n_threads: Optional[int] = None) -> None:
     """Build every model's mesh, operators and device operator (host work,
-    outside any timed region); job i is assigned to context i % len(contexts)."""
+    outside any timed region), several at once; job i is assigned to context
+    i % len(contexts)."""
     for i, job in enumerate(jobs):
         job.worker = i % len(contexts)
-        job.model = configs.build_model(job.workload, contexts[job.worker])
+    models = build_models([j.workload for j in jobs], contexts, n_threads)
+    for job, m in zip(jobs, models):
+        job.model = m
 
 
 def run(jobs: Sequence[ModelJob], n_workers: int, params: Optional[SolveParams] = None,

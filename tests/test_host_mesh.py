@@ -193,3 +193,19 @@ def test_compact_station_steps_equal_full_field_forms():
     rest = np.ones(m.n, bool)
     rest[idx] = False
     assert not np.any(r1[rest]) and not np.any(r2[rest])
+
+
+def test_concurrent_builds_equal_sequential():
+    """Host models built on several threads at once (multimodel.build_models)
+    hold exactly the arrays of the same models built one after another."""
+    from maple3dmt_amd import multimodel
+    wls = [configs.random3d(20 + i, min_level=3, max_level=5, n_freq=2, half_width=50e3,
+                            station_radius=5e3) for i in range(4)]
+    seq = [configs.build_model(wl, None) for wl in wls]
+    par = multimodel.build_models(wls, [None], n_threads=4)
+    names = ("offdiag_ptr", "offdiag_col", "offdiag_val", "diag_s", "me", "sigma", "cell_cx", "L_val",
+             "G_col", "is_bc")
+    for a, b in zip(seq, par):
+        assert a.n == b.n and a.nc == b.nc
+        for name in names:
+            assert np.array_equal(a.array(name), b.array(name)), name
