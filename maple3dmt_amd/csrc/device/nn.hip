@@ -1,8 +1,8 @@
 // Coarse-grid neural operator for the initial field e0: four 3x3x3 dilated
 // conv layers on a 32^3 grid, BF16 storage, FP32 accumulation on the CDNA
-// matrix cores (v_mfma_f32_32x32x16_bf16 over 32-channel inputs,
-// v_mfma_f32_32x32x8_bf16 over the padded 8-channel input and in the weight
-// gradient), trained with Adam on the device.
+// matrix cores (v_mfma_f32_32x32x16_bf16; v_mfma_f32_32x32x8_bf16 over the
+// padded 8-channel input of the forward pass), trained with Adam on the
+// device.
 //
 // Every conv pass is an implicit GEMM whose 32x32 MFMA tile is one x-row of
 // the grid (32 voxels) against 32 channels:
@@ -219,7 +219,16 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(int batch, int dil,
 
 // Weight gradient partials: block (chunk, tap) sums dz[v, co] * in[v + d*tap, ci]
 // over the rows of its chunk; M = co (32), N = ci (CIN, zero-padded to 32),
-// K = voxels, 8 per MFMA step (x = 8s + 4q + j for lane quad q).
+// K = voxels.  The contraction runs over voxels, the strided dimension of
+// channels-last activations, so each wavefront stages one x-row of dz
+// (masked by act > 0) and of the shifted input through LDS: 16-byte global
+// loads, written transposed into wave-private tiles [channel][x] (row stride
+// kTileStride keeps every 8-voxel operand 16-byte aligned), then read back
+// as the operands of two 16-deep MFMAs.  A tile belongs to one wavefront and
+// LDS serves a wavefront's accesses in order, so no block barrier is needed
+// between rows (the wavefronts of a block may take different row counts).
+constexpr int kTileStride = 40;
+
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows_per_chunk,
                                                     const unsigned short* __restrict__ in,
@@ -227,11 +236,17 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows
                                                     const unsigned short* __restrict__ act,
                                                     float* __restrict__ part) {
     __shared__ float red[4][16][64];
+    __shared__ __attribute__((aligned(16))) unsigned short tdz[4][HID * kTileStride];
+    __shared__ __attribute__((aligned(16))) unsigned short tin[4][HID * kTileStride];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int chunk = blockIdx.x, tap = blockIdx.y;
     const int m = lane & 31, q = lane >> 5;
     int dz, dy, dx;
     tap_offsets(tap, dz, dy, dx);
+    unsigned short* Tdz = tdz[wave];
+    unsigned short* Tin = tin[wave];
+    // Input channels CIN..31 of the B tile stay zero.
+    for (int k = lane; k < HID * kTileStride; k += 64) Tin[k] = 0;
     f16 acc;
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
     const int rows = batch * G * G;
@@ -242,22 +257,61 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows
         const int zz = z + dz * dil, yy = y + dy * dil;
         const bool okzy = zz >= 0 && zz < G && yy >= 0 && yy < G;
         const size_t vsrc_row = ((size_t(b) * G + (okzy ? zz : 0)) * G + (okzy ? yy : 0)) * G;
+        __builtin_amdgcn_wave_barrier();
+        // dz tile: lane l takes voxels x, x + 1 (x = 2 (l / 4)) of channels
+        // c0 .. c0 + 7 (c0 = 8 (l % 4)) and writes each channel's pair as one
+        // 4-byte word of the transposed tile.
+        {
+            const int x = 2 * (lane >> 2), c0 = (lane & 3) * 8;
+            s8 g[2];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            s4 a, w;
-            for (int j = 0; j < 4; ++j) {
-                const int x = 8 * s + 4 * q + j;
-                const size_t v = vrow + x;
-                unsigned short g = dout[v * HID + m];
-                if (act && bf2f(act[v * HID + m]) <= 0.f) g = 0;
-                a[j] = static_cast<short>(g);
-                const int xx = x + dx * dil;
-                unsigned short iv = 0;
-                if (okzy && xx >= 0 && xx < G && m < CIN) iv = in[(vsrc_row + xx) * CIN + m];
-                w[j] = static_cast<short>(iv);
+            for (int h = 0; h < 2; ++h) {
+                g[h] = *reinterpret_cast<const s8*>(dout + (vrow + x + h) * HID + c0);
+                if (act) {
+                    const s8 a = *reinterpret_cast<const s8*>(act + (vrow + x + h) * HID + c0);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if (bf2f(static_cast<unsigned short>(a[j])) <= 0.f) g[h][j] = 0;
+                }
             }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, w, acc, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                *reinterpret_cast<unsigned*>(Tdz + (c0 + j) * kTileStride + x) =
+                    unsigned(static_cast<unsigned short>(g[0][j])) |
+                    (unsigned(static_cast<unsigned short>(g[1][j])) << 16);
         }
+        // Input tile: Tin[ci][x] = in[x + dx * dil][ci], zero outside the
+        // grid; the same pairing over the CIN / 8 channel octets.
+        {
+            constexpr int NO = CIN / 8;        // channel octets per voxel
+            constexpr int NP = 16 * NO;        // (voxel pair, octet) items
+            if (lane < NP) {
+                const int x = 2 * (lane / NO), c0 = (lane % NO) * 8;
+                s8 t[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int xs = x + h + dx * dil;
+                    s8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+                    if (okzy && xs >= 0 && xs < G) v = *reinterpret_cast<const s8*>(in + (vsrc_row + xs) * CIN + c0);
+                    t[h] = v;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    *reinterpret_cast<unsigned*>(Tin + (c0 + j) * kTileStride + x) =
+                        unsigned(static_cast<unsigned short>(t[0][j])) |
+                        (unsigned(static_cast<unsigned short>(t[1][j])) << 16);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int s16 = 0; s16 < 2; ++s16) {
+            const s8 a = *reinterpret_cast<const s8*>(Tdz + m * kTileStride + 16 * s16 + 8 * q);
+            const s8 w = *reinterpret_cast<const s8*>(Tin + m * kTileStride + 16 * s16 + 8 * q);
+            acc = mfma16(a, w, acc);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     }
     for (int r = 0; r < 16; ++r) red[wave][r][lane] = acc[r];
     __syncthreads();
