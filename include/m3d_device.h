@@ -240,7 +240,9 @@ int m3d_operator_format(m3d_operator* op, int* value_table, int* column_bits);
  * kind 0: batched SpMV y = A x; kind 1: one multicolour SSOR apply;
  * kind 2: one FP64 BiCGStab iteration (b = 1, tolerance 0, so every system
  * keeps running); kind 3: the same iteration in complex64; kind 4: one
- * backward-sweep launch (the mean over the colours of a backward sweep). */
+ * backward-sweep launch (the mean over the backward launches of an SSOR
+ * apply: every colour but the last, whose backward step is written by its
+ * forward launch when it has no upper couplings). */
 int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
                      double* avg_ms, double* alg_bytes);
 
