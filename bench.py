@@ -37,6 +37,9 @@ against it; and
                  counters of the committed profile
   cpu_baseline   the CPU oracle timed on a bounded sample of the same workload,
                  on one core and on P cores at once (value: the P-core rate)
+
+python bench.py --c3 instead prints one line for the 16-model batch (1024
+forward and adjoint systems on meshes of 0.24-2.4 M edges, four streams).
 """
 from __future__ import annotations
 
@@ -99,6 +102,9 @@ def parse():
                     help="profiling mode: build the workload, then only run the benchmark "
                          "kernels (SpMV, SSOR, BiCGStab iterations, backward sweep) and four "
                          "1 GiB device copies (counter calibration)")
+    ap.add_argument("--c3", action="store_true",
+                    help="separate mode: the 16-model batch (1024 forward + adjoint systems on "
+                         "meshes of 0.24-2.4 M edges), one JSON line of its own")
     ap.add_argument("--network-only", action="store_true",
                     help="profiling mode: only the e0 network's inference and training steps")
     ap.add_argument("--reps", type=int, default=20)
@@ -160,6 +166,9 @@ def main():
 
     if args.kernel_only:
         kernel_only(args)
+        return
+    if args.c3:
+        print(json.dumps(c3_leg(args)), flush=True)
         return
     if args.network_only:
         from maple3dmt_amd import Context
@@ -428,6 +437,64 @@ def multi_model_leg(args, bench_model, ctx):
     log(f"[bench] multi-model: {out}")
     for j in js:
         j.model.close()
+    for c in contexts:
+        c.close()
+    return out
+
+
+def c3_leg(args):
+    """Misfit gradients of 16 seeded random models on their own meshes of
+    uneven size (octree levels 6-7, 0.24-2.4 M edges), 16 frequencies
+    log-spaced 1e-3..1e2 Hz x 2 polarisations, forward and adjoint: 1024
+    complex128 systems, solved by BiCGStab with interleaved DivCorr (the
+    reference's adjoint method, bursts of 100 iterations, DivCorr PCG to
+    1e-4 inside the bursts) in batches of 32 per model and phase, four
+    models at a time on four streams.  Each model
+    is released once its gradient is done, so at most four models' solve
+    workspaces are resident."""
+    from maple3dmt_amd import Context, configs, multimodel
+    sizes = [(6, 80e3), (7, 40e3), (7, 60e3), (7, 80e3), (7, 100e3)]
+    wls = []
+    for i in range(16):
+        lev, rad = sizes[i % len(sizes)]
+        wls.append(configs.random3d(300 + i, max_level=lev, n_freq=16, half_width=100e3,
+                                    station_radius=rad, maxiter=args.maxiter))
+    n_streams = 4
+    contexts = [Context(0) for _ in range(n_streams)]
+    t = time.time()
+    jobs = []
+    for wl in wls:
+        z, err = multimodel.halfspace_data(wl)
+        jobs.append(multimodel.ModelJob(wl, z, err))
+    multimodel.prepare(jobs, contexts, n_threads=16)
+    t_build = time.time() - t
+    edges = [j.model.n for j in jobs]
+    log(f"[c3] 16 models built in {t_build:.1f}s, edges {edges}")
+    prm = configs.ForwardParams(bicgstab_tol=1e-7, bicgstab_maxiter=args.maxiter).fast_params(
+        precision=0, dc_interval=args.dc_interval)
+    wall = multimodel.run(jobs, n_streams, prm, prm, release=True)
+    n_sys = multimodel.systems(jobs)
+    iters = [int(sum(r["iterations"] for r in j.result.forward + j.result.adjoint)) for j in jobs]
+    # A system is solved when it converged, or broke down (info 2) with its
+    # true residual already below the tolerance.
+    def solved(r):
+        return bool(r["converged"]) or (r["info"] == 2 and r["true_residual"] < prm.tol)
+    # Systems that stopped without meeting the tolerance: (model, phase,
+    # system, info, true residual, iterations).
+    unconverged = [(k, ph, i, r["info"], r["true_residual"], r["iterations"])
+                   for k, j in enumerate(jobs)
+                   for ph, rs in (("forward", j.result.forward), ("adjoint", j.result.adjoint))
+                   for i, r in enumerate(rs) if not solved(r)]
+    n_solved = n_sys - len(unconverged)
+    conv = not unconverged
+    out = {"metric": "solved systems/s, 16-model batch", "value": n_solved / wall, "unit": "systems/s",
+           "systems": n_sys, "solved": n_solved, "models": 16, "frequencies": 16, "edges": edges, "total_edges": int(sum(edges)),
+           "wall_s": wall, "build_s": t_build, "streams": n_streams, "converged": conv,
+           "iterations_per_model": iters, "per_model_s": [j.seconds for j in jobs],
+           "unconverged": unconverged,
+           "method": "complex128 BiCGStab + SSOR with DivCorr every 100 iterations (forward and adjoint)",
+           "dtype": "complex128 (f64)", "data": "synthetic (seeded random 3-D models, halfspace observations)"}
+    log(f"[c3] {n_sys} systems in {wall:.1f}s, {n_solved} solved = {n_solved / wall:.1f} solved systems/s")
     for c in contexts:
         c.close()
     return out

@@ -71,10 +71,11 @@ def prepare(jobs: Sequence[ModelJob], contexts: Sequence[Context], n_threads: Op
 
 
 def run(jobs: Sequence[ModelJob], n_workers: int, params: Optional[SolveParams] = None,
-        adjoint_params: Optional[SolveParams] = None) -> float:
+        adjoint_params: Optional[SolveParams] = None, release: bool = False) -> float:
     """Gradient of every job; with one worker all jobs run in order on the
     calling thread's schedule, else worker w runs the jobs assigned to
-    context w in order.  Returns the wall time in seconds."""
+    context w in order.  With release, each model (and its device memory) is
+    closed once its gradient is done.  Returns the wall time in seconds."""
     errors: List[BaseException] = []
 
     def work(w: int):
@@ -86,6 +87,8 @@ def run(jobs: Sequence[ModelJob], n_workers: int, params: Optional[SolveParams] 
                 job.result = inversion.gradient(job.model, job.workload.freqs, job.z_obs, job.z_err,
                                                 params, adjoint_params)
                 job.seconds = time.perf_counter() - t
+                if release:
+                    job.model.close()
         except BaseException as e:  # re-raised in the caller
             errors.append(e)
 
