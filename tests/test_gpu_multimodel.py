@@ -46,3 +46,24 @@ def test_concurrent_models_match_sequential():
         j.model.close()
     for c in contexts:
         c.close()
+
+
+def test_release_after_gradient_keeps_results():
+    """run(..., release=True) closes each model once its gradient is done
+    (the device memory goes back) and leaves the gradients those of a run
+    that keeps the models; the host threads build identical models."""
+    contexts = [Context(0) for _ in range(2)]
+    keep = _jobs()
+    multimodel.prepare(keep, contexts, n_threads=1)
+    multimodel.run(keep, 2)
+    rel = _jobs()
+    multimodel.prepare(rel, contexts, n_threads=3)
+    multimodel.run(rel, 2, release=True)
+    for a, b in zip(keep, rel):
+        assert b.model.handle is None
+        assert a.result.phi == b.result.phi
+        assert np.array_equal(a.result.grad, b.result.grad)
+    for j in keep:
+        j.model.close()
+    for c in contexts:
+        c.close()
