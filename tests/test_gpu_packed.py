@@ -142,3 +142,67 @@ def test_fused_last_colour_bitwise(pair, unfused, k):
             out.append((xs, [r["iterations"] for r in res]))
         assert out[0][1] == out[1][1]
         assert np.array_equal(out[0][0], out[1][0])
+
+
+def _raw_operator(ctx, n, values, seed):
+    """A device operator from a random sparse CSR (a symmetric pattern of
+    about 12 couplings per row, no constrained rows) whose couplings take
+    values(rng, nnz)."""
+    import ctypes as C
+    import scipy.sparse as sp
+    from maple3dmt_amd.native import DeviceOperator, OperatorDescC, _check_dev, lib
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n), 6)
+    cols = rng.integers(0, n, rows.size)
+    keep = rows != cols
+    A = sp.csr_matrix((np.ones(keep.sum()), (rows[keep], cols[keep])), shape=(n, n))
+    A = (A + A.T).tocsr()                 # a symmetric pattern, as the colouring expects
+    A.sum_duplicates()
+    A.sort_indices()
+    A.data = values(rng, A.nnz)
+    ptr = A.indptr.astype(np.int32)
+    col = A.indices.astype(np.int32)
+    val = np.ascontiguousarray(A.data, np.float64)
+    diag = rng.uniform(4.0, 8.0, n)
+    me = rng.uniform(0.1, 1.0, n)
+    zero = np.zeros(n, np.uint8)
+    keep_alive = (ptr, col, val, diag, me, zero)
+    d = OperatorDescC(n, ptr.ctypes.data_as(C.POINTER(C.c_int)), col.ctypes.data_as(C.POINTER(C.c_int)),
+                      val.ctypes.data_as(C.POINTER(C.c_double)), diag.ctypes.data_as(C.POINTER(C.c_double)),
+                      zero.ctypes.data_as(C.POINTER(C.c_ubyte)), zero.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                      0, None)
+    h = C.c_void_p()
+    _check_dev(lib().m3d_operator_create(ctx.handle, C.byref(d), C.byref(h)))
+    _check_dev(lib().m3d_operator_set_mass(h, me.ctypes.data_as(C.c_void_p)))
+    return DeviceOperator(h, n=n), A, diag, me, keep_alive
+
+
+@pytest.mark.parametrize("distinct", [50, None])
+def test_packing_on_general_matrices(ctx, distinct):
+    """Couplings drawn from 50 values are packed (a table of 51 with the
+    padding zero); random couplings (over 4096 distinct values) are not.
+    Either way the SpMV at 1 and 4 systems matches SciPy's product."""
+    from maple3dmt_amd.native import _check_dev, lib
+    if distinct:
+        def values(rng, k):
+            return rng.choice(rng.standard_normal(distinct), k)
+    else:
+        def values(rng, k):
+            return rng.standard_normal(k)
+    dev, A, diag, me, _keep = _raw_operator(ctx, 20000, values, 5)
+    try:
+        info = dev.info()
+        if distinct:
+            assert info["value_table"] == distinct + 1
+        else:
+            assert info["value_table"] == 0
+        rng = np.random.default_rng(6)
+        for k in (1, 4):
+            om = 2 * np.pi * rng.uniform(0.1, 10.0, k)
+            x = rng.standard_normal((k, A.shape[0])) + 1j * rng.standard_normal((k, A.shape[0]))
+            y = dev.spmv(om, x)
+            for i in range(k):
+                ref = A @ x[i] + (diag + 1j * om[i] * me) * x[i]
+                assert np.linalg.norm(y[i] - ref) <= 1e-13 * np.linalg.norm(ref)
+    finally:
+        _check_dev(lib().m3d_operator_destroy(dev.handle))
