@@ -304,6 +304,7 @@ def main():
     ach_spmv = bytes_spmv / (ms_spmv * 1e-3) / 1e9
     ms_ssor, bytes_ssor = op.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
     info = op.info()
+    info["device_bytes"] = op.device_bytes()
     log(f"[bench] sweep_bwd x{tw} {ms_sw * 1e3:.1f} us/launch, {ach:.0f} GB/s; spmv x{ns} {ms_spmv:.3f} ms, "
         f"{ach_spmv:.0f} GB/s; ssor x{ns} {ms_ssor:.3f} ms; {info}")
 

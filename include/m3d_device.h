@@ -234,6 +234,18 @@ int m3d_operator_info(m3d_operator* op, int* n_padded, int* ncolours, long long*
  * M3D_PACKED_SELL=0 switches them off. */
 int m3d_operator_format(m3d_operator* op, int* value_table, int* column_bits);
 
+/* Device memory of the operator (no reference counterpart; for callers that
+ * keep several meshes resident): operator_bytes = the operator, DivCorr,
+ * sigma-update and sensitivity arrays; workspace_bytes = the solve
+ * workspaces of every batch width used so far, which stay allocated between
+ * solves (about 20 GB per million edges once every width up to 32 systems
+ * has run in FP64 with DivCorr).  Either output may be NULL. */
+int m3d_operator_device_bytes(m3d_operator* op, long long* operator_bytes, long long* workspace_bytes);
+/* Frees the solve workspaces and their captured graphs after the context
+ * stream drains; the operator, its shapes and data stay, and the next solve
+ * allocates what it needs again (with the same results). */
+int m3d_operator_trim(m3d_operator* op);
+
 /* Measurement (no reference counterpart): average duration in ms of one
  * launch of a kernel class over `reps` launches, timed with HIP events on the
  * context stream, and its algorithmic HBM bytes per launch.

@@ -1368,6 +1368,43 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
     HIPCHK(hipStreamSynchronize(s));
 }
 
+// Device bytes held by a set of buffers.
+template <typename T>
+size_t nbytes(const DBuf<T>& b) {
+    return b.p ? b.n * sizeof(T) : 0;
+}
+template <typename T, typename... R>
+size_t nbytes(const DBuf<T>& b, const R&... rest) {
+    return nbytes(b) + nbytes(rest...);
+}
+
+template <int NS, typename V>
+size_t work_bytes(const Work<NS, V>* w) {
+    return w ? nbytes(w->b, w->x, w->r, w->r0, w->p, w->v, w->s, w->t, w->y, w->ph, w->sh) : 0;
+}
+template <int NS>
+size_t work64_bytes(const Work64<NS>* W) {
+    if (!W) return 0;
+    return nbytes(W->b, W->x, W->aux, W->tmp, W->rres, W->phi0, W->dr, W->dz, W->dp, W->dphi, W->dap,
+                  W->omega, W->scale, W->st, W->part, W->flags) +
+           work_bytes<NS, double>(W->w64.get()) + work_bytes<NS, float>(W->w32.get());
+}
+
+size_t operator_bytes(const m3d_operator& op) {
+    return nbytes(op.d_perm, op.l_col, op.u_col, op.hang, op.slc, op.l_val, op.u_val, op.diag_s, op.me,
+                  op.l_val32, op.u_val32, op.diag_s32, op.me32, op.l_pk, op.u_pk, op.vtab, op.vtab32, op.bc,
+                  op.constrained, op.nt_ptr, op.nt_edge, op.g_col, op.l_ptr, op.l_colidx, op.nt_val,
+                  op.g_val, op.ms, op.l_vals, op.l_diag, op.dc_bc_edge, op.dc_dead, op.dc_bc_node,
+                  op.m_adj_ptr, op.m_adj_cell, op.l_edge, op.d_err, op.m_area, op.m_len, op.m_massless,
+                  op.m_dead, op.dc_in_pattern, op.s_cell_ptr, op.s_cell_edges, op.s_edge_ptr,
+                  op.s_edge_cells, op.s_weight, op.s_sigma, op.s_omega);
+}
+
+size_t workspace_bytes(const m3d_operator& op) {
+    return work64_bytes<1>(op.w1.get()) + work64_bytes<2>(op.w2.get()) + work64_bytes<4>(op.w4.get()) +
+           work64_bytes<8>(op.w8.get()) + work64_bytes<16>(op.w16.get()) + work64_bytes<32>(op.w32.get());
+}
+
 void clear_graphs(m3d_operator& op) {
     if (op.w1) op.w1->graphs.clear();
     if (op.w2) op.w2->graphs.clear();
@@ -1724,6 +1761,28 @@ int m3d_operator_permutation(m3d_operator* op, int* perm, int* colour_start) {
         if (!op) throw Fail(M3D_ERR_ARG, "null operator");
         if (perm) std::copy(op->perm.begin(), op->perm.end(), perm);
         if (colour_start) std::copy(op->colour_start.begin(), op->colour_start.end(), colour_start);
+    });
+}
+
+int m3d_operator_device_bytes(m3d_operator* op, long long* operator_bytes_out, long long* workspace_bytes_out) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        if (operator_bytes_out) *operator_bytes_out = static_cast<long long>(operator_bytes(*op));
+        if (workspace_bytes_out) *workspace_bytes_out = static_cast<long long>(workspace_bytes(*op));
+    });
+}
+
+int m3d_operator_trim(m3d_operator* op) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        HIPCHK(hipSetDevice(op->ctx->device));
+        HIPCHK(hipStreamSynchronize(op->ctx->stream));
+        op->w1.reset();
+        op->w2.reset();
+        op->w4.reset();
+        op->w8.reset();
+        op->w16.reset();
+        op->w32.reset();
     });
 }
 

@@ -132,7 +132,7 @@ DEVICE_SYMBOLS = [
     "m3d_context_create", "m3d_context_destroy", "m3d_last_error", "m3d_operator_create",
     "m3d_operator_destroy", "m3d_operator_set_mass", "m3d_operator_set_divcorr", "m3d_solve_batch",
     "m3d_solve_batch_device", "m3d_divcorr", "m3d_spmv", "m3d_ssor_apply",
-    "m3d_operator_permutation", "m3d_operator_info", "m3d_operator_format", "m3d_context_stream", "m3d_bench_kernel",
+    "m3d_operator_permutation", "m3d_operator_info", "m3d_operator_format", "m3d_operator_device_bytes", "m3d_operator_trim", "m3d_context_stream", "m3d_bench_kernel",
     "m3d_operator_set_sensitivity", "m3d_sensitivity_device", "m3d_operator_get_shape",
     "m3d_operator_set_shape", "m3d_operator_autotune", "m3d_operator_set_mass_structure",
     "m3d_operator_update_sigma", "m3d_operator_update_sigma_device", "m3d_perturbation_rhs_device",
@@ -184,6 +184,8 @@ def lib() -> C.CDLL:
     L.m3d_operator_permutation.argtypes = [vp, vp, vp]
     L.m3d_operator_info.argtypes = [vp, p(i), p(i), p(C.c_longlong), p(C.c_longlong)]
     L.m3d_operator_format.argtypes = [vp, p(i), p(i)]
+    L.m3d_operator_device_bytes.argtypes = [vp, p(C.c_longlong), p(C.c_longlong)]
+    L.m3d_operator_trim.argtypes = [vp]
     L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d)]
     L.m3d_sensitivity_device.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_perturbation_rhs_device.argtypes = [vp, i, vp, vp, vp, vp]
@@ -372,6 +374,16 @@ class DeviceOperator:
         _check_dev(lib().m3d_operator_format(self.handle, C.byref(tab), C.byref(cb)))
         return dict(n_padded=npad.value, ncolours=nc.value, stored=st.value, nnz=nz.value,
                     value_table=tab.value, column_bits=cb.value)
+
+    def device_bytes(self) -> dict:
+        """Device memory of the operator's arrays and of its solve workspaces."""
+        ob, wb = C.c_longlong(), C.c_longlong()
+        _check_dev(lib().m3d_operator_device_bytes(self.handle, C.byref(ob), C.byref(wb)))
+        return {"operator": ob.value, "workspaces": wb.value}
+
+    def trim(self) -> None:
+        """Free the solve workspaces (the next solve allocates them again)."""
+        _check_dev(lib().m3d_operator_trim(self.handle))
 
     def permutation(self):
         inf = self.info()

@@ -67,3 +67,26 @@ def test_release_after_gradient_keeps_results():
         j.model.close()
     for c in contexts:
         c.close()
+
+
+def test_trim_frees_workspaces_and_keeps_results():
+    """m3d_operator_trim frees the solve workspaces (the device bytes they
+    held go back to zero) and the next solve, which allocates them again,
+    gives the same fields bit for bit."""
+    ctx = Context(0)
+    wl = configs.random3d(24, min_level=3, max_level=5, n_freq=2, half_width=50e3, station_radius=5e3)
+    m = configs.build_model(wl, ctx)
+    b0 = m.device.device_bytes()
+    E1, Z1, r1 = m.solve_frequencies(wl.freqs)
+    b1 = m.device.device_bytes()
+    print("device bytes", b0, b1, flush=True)
+    assert b1["operator"] == b0["operator"] > 0
+    assert b1["workspaces"] > 0
+    m.device.trim()
+    assert m.device.device_bytes() == {"operator": b0["operator"], "workspaces": 0}
+    E2, Z2, r2 = m.solve_frequencies(wl.freqs)
+    assert [r["iterations"] for r in r1] == [r["iterations"] for r in r2]
+    assert np.array_equal(E1, E2)
+    assert np.array_equal(Z1, Z2)
+    m.close()
+    ctx.close()
