@@ -78,9 +78,10 @@ struct Launch64 {
     static void dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
                       const double2* p, double2* ap, SysState* st, double* part);
     static void dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
-                          const double2* ap, double2* phi, double2* r, double2* z, SysState* st,
-                          double* part);
-    static void dc_p(hipStream_t s, int nn, const double2* z, double2* p, const SysState* st);
+                          const double2* ap, double2* phi, double2* r, SysState* st, double* part);
+    // p = D^-1 r + beta p (the Jacobi-preconditioned residual formed in place).
+    static void dc_p(hipStream_t s, int nn, const double* ldiag, const double2* r, double2* p,
+                     const SysState* st);
     static void dc_correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
                            const unsigned char* bc_edge, const double2* phi, double2* E,
                            const SysState* st);

@@ -1094,8 +1094,7 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_lp_fin(SysState* st, const dou
 template <int NS>
 __global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const double2* __restrict__ p,
                             const double2* __restrict__ ap, double2* __restrict__ phi,
-                            double2* __restrict__ r, double2* __restrict__ z,
-                            const SysState* __restrict__ st, double* part) {
+                            double2* __restrict__ r, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = (dc_mask<NS>(st, true) >> sys) & 1u;
@@ -1117,9 +1116,10 @@ __global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const doub
         ri.x -= aap.x;
         ri.y -= aap.y;
         r[i] = ri;
+        // z = D^-1 r enters the (r, z) product only: k_dc_p forms it again
+        // from r with the same division instead of a stored copy.
         const double d = ldiag[v];
         const double2 zi = (fabs(d) > 1e-30) ? mk(ri.x / d, ri.y / d) : ri;
-        z[i] = zi;
         acc[0] += cabs2(ri);
         const double2 rz = cconjmul(ri, zi);
         acc[1] += rz.x;
@@ -1148,8 +1148,10 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_update_fin(SysState* st, const
 
 // p = z + beta p.
 template <int NS>
-__global__ void k_dc_p(int nn, const double2* __restrict__ z, double2* __restrict__ p,
-                       const SysState* __restrict__ st) {
+// p = z + beta p with z = D^-1 r formed here (the Jacobi preconditioner of
+// k_dc_update, same division).
+__global__ void k_dc_p(int nn, const double* __restrict__ ldiag, const double2* __restrict__ r,
+                       double2* __restrict__ p, const SysState* __restrict__ st) {
     const size_t total = size_t(nn) * NS;
     const unsigned run = dc_mask<NS>(st, true);
     for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
@@ -1158,7 +1160,9 @@ __global__ void k_dc_p(int nn, const double2* __restrict__ z, double2* __restric
         if (!((run >> sys) & 1u)) continue;
         const double2 b = st[sys].pcg_beta;
         const double2 bp = cmul(b, p[t]);
-        const double2 zi = z[t];
+        const double2 ri = r[t];
+        const double d = ldiag[t / NS];
+        const double2 zi = (fabs(d) > 1e-30) ? mk(ri.x / d, ri.y / d) : ri;
         p[t] = mk(zi.x + bp.x, zi.y + bp.y);
     }
 }
@@ -1562,17 +1566,16 @@ void Launch64<NS>::dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol
 }
 template <int NS>
 void Launch64<NS>::dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
-                             const double2* ap, double2* phi, double2* r, double2* z, SysState* st,
-                             double* part) {
+                             const double2* ap, double2* phi, double2* r, SysState* st, double* part) {
     const int g = row_grid<NS>(nn);
-    hipLaunchKernelGGL((k_dc_update<NS>), dim3(g), dim3(kBlock), 0, s, nn, ldiag, p, ap, phi, r, z, st,
-                       part);
+    hipLaunchKernelGGL((k_dc_update<NS>), dim3(g), dim3(kBlock), 0, s, nn, ldiag, p, ap, phi, r, st, part);
     hipLaunchKernelGGL((k_dc_update_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS>
-void Launch64<NS>::dc_p(hipStream_t s, int nn, const double2* z, double2* p, const SysState* st) {
+void Launch64<NS>::dc_p(hipStream_t s, int nn, const double* ldiag, const double2* r, double2* p,
+                        const SysState* st) {
     hipLaunchKernelGGL((k_dc_p<NS>), dim3(grid_for(size_t(nn) * NS, kStreamCap)), dim3(kBlock), 0, s,
-                       nn, z, p, st);
+                       nn, ldiag, r, p, st);
 }
 template <int NS>
 void Launch64<NS>::dc_correct(hipStream_t s, int npad, const int* g_col, const double* g_val,

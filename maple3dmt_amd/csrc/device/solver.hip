@@ -655,9 +655,8 @@ void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, u
         auto pcg_iter = [&] {
             L::dc_lp(s, op.nn, op.l_ptr.p, op.l_colidx.p, op.l_vals.p, W.dp.p, W.dap.p, W.st.p,
                      W.part.p);
-            L::dc_update(s, op.nn, op.l_diag.p, W.dp.p, W.dap.p, W.dphi.p, W.dr.p, W.dz.p, W.st.p,
-                         W.part.p);
-            L::dc_p(s, op.nn, W.dz.p, W.dp.p, W.st.p);
+            L::dc_update(s, op.nn, op.l_diag.p, W.dp.p, W.dap.p, W.dphi.p, W.dr.p, W.st.p, W.part.p);
+            L::dc_p(s, op.nn, op.l_diag.p, W.dr.p, W.dp.p, W.st.p);
         };
         while (any_pcg) {
             if (use_graphs()) {
