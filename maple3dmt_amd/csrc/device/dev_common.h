@@ -119,6 +119,10 @@ struct DevOp {
     const int* u_col;
     const V* diag_s;       // per row; 1 on constrained / padding rows
     const V* me;           // per row; 0 on constrained / padding rows
+    // Per batch width: the row blocks (kBlock / width rows each) sorted by the
+    // smallest caller edge index among their rows, or null.  Used by the
+    // kernels without reductions, which may visit row blocks in any order.
+    const int* order[kWidths];
 };
 
 // Row-block schedule: a plain grid stride over the row blocks.  The
@@ -135,6 +139,20 @@ __device__ inline RowRange row_range(int nrb) {
     r.first = blockIdx.x;
     r.end = nrb;
     r.step = gridDim.x;
+    return r;
+}
+
+// Positions in a spatial block order for a kernel without reductions: the
+// order is cut into kXcds contiguous parts and the workgroups dispatched to
+// one XCD (workgroup i goes to XCD i % kXcds) walk one part, so rows that
+// gather the same neighbours run on one XCD close in time and share its L2.
+// The grid must be a multiple of kXcds.
+__device__ inline RowRange xcd_part(int nrb) {
+    const int x = blockIdx.x % kXcds;
+    RowRange r;
+    r.first = int(static_cast<long long>(nrb) * x / kXcds) + blockIdx.x / kXcds;
+    r.end = int(static_cast<long long>(nrb) * (x + 1) / kXcds);
+    r.step = gridDim.x / kXcds;
     return r;
 }
 

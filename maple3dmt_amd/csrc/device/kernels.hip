@@ -632,8 +632,10 @@ __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
     const int rpb = rows_per_block<NS>();
     const V w = V(omega[sys]);
     const int nrb = (op.n + rpb - 1) / rpb;
-    const RowRange R = row_range(nrb);
-    for (int rb = R.first; rb < R.end; rb += R.step) {
+    const int* __restrict__ order = op.order[width_index(NS)];
+    const RowRange R = order ? xcd_part(nrb) : row_range(nrb);
+    for (int pos = R.first; pos < R.end; pos += R.step) {
+        const int rb = order ? order[pos] : pos;
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
         const size_t i = size_t(row) * NS + sys;

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -213,6 +214,13 @@ bool packed_sell() {
     return !(v && v[0] == '0');
 }
 
+// Spatial row-block order for the plain SpMV (M3D_SPMV_ORDER=0: device
+// order).  The order changes which rows run together, never a row's values.
+bool spmv_order() {
+    const char* v = std::getenv("M3D_SPMV_ORDER");
+    return !(v && v[0] == '0');
+}
+
 bool row_sort() {
     static const bool on = [] {
         const char* v = std::getenv("M3D_ROW_SORT");
@@ -270,6 +278,8 @@ struct m3d_operator {
     int cbits = 0, ntab = 0;
     // The last colour has no upper couplings (its backward step is z = y).
     bool last_upper_empty = false;
+    // Row blocks per batch width in spatial order (empty: device order).
+    DBuf<int> order[kWidths];
     DBuf<unsigned char> bc, constrained;
     std::vector<unsigned char> h_constrained;
     bool mass_set = false;
@@ -314,6 +324,7 @@ DevOp<double> m3d_operator::view<double>() const {
     o.l_val = l_val.p; o.l_col = l_col.p; o.u_val = u_val.p; o.u_col = u_col.p;
     o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab.p : nullptr; o.cbits = cbits;
     o.diag_s = diag_s.p; o.me = me.p;
+    for (int i = 0; i < kWidths; ++i) o.order[i] = order[i].p;
     return o;
 }
 template <>
@@ -325,6 +336,7 @@ DevOp<float> m3d_operator::view<float>() const {
     o.l_val = l_val32.p; o.l_col = l_col.p; o.u_val = u_val32.p; o.u_col = u_col.p;
     o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab32.p : nullptr; o.cbits = cbits;
     o.diag_s = diag_s32.p; o.me = me32.p;
+    for (int i = 0; i < kWidths; ++i) o.order[i] = order[i].p;
     return o;
 }
 
@@ -1278,6 +1290,29 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
     std::vector<int4> slc(nsl);
     for (int sl = 0; sl < nsl; ++sl) slc[sl] = make_int4(loff[sl], lw[sl], uoff[sl], uw[sl]);
     op.slc.upload(slc, s);
+    // Device rows come in colour order, so the rows that gather one edge's
+    // values sit in as many distant row ranges as it has neighbour colours.
+    // The caller's edge numbering follows the mesh (Morton order of cells):
+    // row blocks sorted by their smallest caller edge bring those rows
+    // together for the kernels that may visit blocks in any order.
+    std::vector<int> order[kWidths];
+    if (spmv_order()) {
+        for (int wi = 0; wi < kWidths; ++wi) {
+            const int rpb = kBlock >> wi;  // widths 1, 2, 4, ..., 32
+            const int nrb = (op.npad + rpb - 1) / rpb;
+            std::vector<std::pair<int, int>> key(nrb);
+            for (int b = 0; b < nrb; ++b) {
+                int k = std::numeric_limits<int>::max();
+                for (int r = b * rpb; r < std::min(op.npad, (b + 1) * rpb); ++r)
+                    if (op.perm[r] >= 0) k = std::min(k, op.perm[r]);
+                key[b] = {k, b};
+            }
+            std::sort(key.begin(), key.end());
+            order[wi].resize(nrb);
+            for (int b = 0; b < nrb; ++b) order[wi][b] = key[b].second;
+            op.order[wi].upload(order[wi], s);
+        }
+    }
     // The couplings of an octree mesh take few distinct values (cell sizes
     // are powers of two: 82 values among 15 M entries on a five-level mesh),
     // so the blocks are stored as one word per entry, column | value index <<
@@ -1396,7 +1431,8 @@ size_t operator_bytes(const m3d_operator& op) {
                   op.g_val, op.ms, op.l_vals, op.l_diag, op.dc_bc_edge, op.dc_dead, op.dc_bc_node,
                   op.m_adj_ptr, op.m_adj_cell, op.l_edge, op.d_err, op.m_area, op.m_len, op.m_massless,
                   op.m_dead, op.dc_in_pattern, op.s_cell_ptr, op.s_cell_edges, op.s_edge_ptr,
-                  op.s_edge_cells, op.s_weight, op.s_sigma, op.s_omega);
+                  op.s_edge_cells, op.s_weight, op.s_sigma, op.s_omega, op.order[0], op.order[1],
+                  op.order[2], op.order[3], op.order[4], op.order[5]);
 }
 
 size_t workspace_bytes(const m3d_operator& op) {
