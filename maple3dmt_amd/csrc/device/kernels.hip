@@ -25,8 +25,9 @@ __device__ inline int rows_per_block() { return kBlock / NS; }
 // loaded before any of their x entries, so a row waits for two memory round
 // trips per U entries instead of two per entry.  Every lane of a wavefront
 // works on rows of one slice, so off and w are read from the first lane into
-// scalar registers and the loop runs without divergence; entries past w
-// repeat entry w - 1 with value 0 (a valid address, added as +0).
+// scalar registers and the loop runs without divergence; entries past w load
+// nothing and are added as 0 * 0 (a row sum is never -0, so adding +0 leaves
+// it unchanged, as the entries' absence would).
 template <int U, int STEP, typename V>
 __device__ inline void gather_sell(const V* __restrict__ val, const int* __restrict__ col, int off,
                                    int w, int lane, int k0, const cvec<V>* __restrict__ x, int NS,
@@ -39,13 +40,17 @@ __device__ inline void gather_sell(const V* __restrict__ val, const int* __restr
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kk = k + u * STEP;
-            const int idx = off + min(kk, w - 1) * kSlice + lane;
-            c[u] = col[idx];
+            const int idx = off + kk * kSlice + lane;
+            c[u] = kk < w ? col[idx] : 0;
             a[u] = kk < w ? val[idx] : V(0);
         }
         cvec<V> xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = x[size_t(c[u]) * NS + sys];
+        for (int u = 0; u < U; ++u) {
+            xv[u].x = V(0);
+            xv[u].y = V(0);
+            if (k + u * STEP < w) xv[u] = x[size_t(c[u]) * NS + sys];
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             ar += a[u] * xv[u].x;
@@ -56,7 +61,7 @@ __device__ inline void gather_sell(const V* __restrict__ val, const int* __restr
 
 // gather_sell over a packed block: the column and the value's table index
 // come from one word, so an entry costs 4 bytes instead of 8 + 4 (or 4 + 4).
-// Entries past w keep the column of entry w - 1 and take index 0 (value 0).
+// Entries past w load nothing and add 0 * 0, as in gather_sell.
 // The additions are those of gather_sell in the same order.
 template <int U, int STEP, typename V>
 __device__ inline void gather_packed(const unsigned* __restrict__ pk, const V* __restrict__ tab, int cbits,
@@ -70,15 +75,19 @@ __device__ inline void gather_packed(const unsigned* __restrict__ pk, const V* _
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kk = k + u * STEP;
-            wd[u] = pk[off + min(kk, w - 1) * kSlice + lane];
-            if (kk >= w) wd[u] &= cm;
+            wd[u] = kk < w ? pk[off + kk * kSlice + lane] : 0u;
         }
         cvec<V> xv[U];
         V a[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            xv[u] = x[size_t(wd[u] & cm) * NS + sys];
-            a[u] = tab[wd[u] >> cbits];
+            xv[u].x = V(0);
+            xv[u].y = V(0);
+            a[u] = V(0);
+            if (k + u * STEP < w) {
+                xv[u] = x[size_t(wd[u] & cm) * NS + sys];
+                a[u] = tab[wd[u] >> cbits];
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
