@@ -185,6 +185,9 @@ typedef struct {
     int guess_used;       /* FORWARD with use_initial_guess: 1 if the guess was kept */
     double guess_residual; /* FORWARD with use_initial_guess: ||b - A x0|| / ||b|| of the
                               (scaled) guess, whether kept or not; else 0 */
+    int stagnated;        /* INTERLEAVED: stopped by the stagnation rule (a burst that
+                             lowered the true residual by less than 5 %, accepted as is:
+                             forward_solver_fv.cpp:866-873) */
 } m3d_solve_result;
 
 /* Solve nsys systems that share this operator, one omega each.
@@ -340,6 +343,83 @@ int m3d_operator_autotune(m3d_operator* op, int widths_mask, const double* omega
 /* Stream the library launches on (hipStream_t), for callers that time or
  * order work around it. */
 void* m3d_context_stream(m3d_context* ctx);
+
+/* ---- Station side of the gradient, on the device ----------------------
+ * The steps either side of the adjoint solve of InversionFV::compute_gradient_
+ * (inversion_fv.cpp:250-280) on device-resident fields, so that the forward
+ * and adjoint batches follow each other with no station data crossing the
+ * bus: impedance extraction (extract_impedance_, forward_solver_fv.cpp:
+ * 1490-1727), the weighted residual and misfit (inversion_fv.cpp:258-271,
+ * :75-98) and the adjoint right-hand side (build_adjoint_rhs_from_residual,
+ * forward_solver_fv.cpp:1804-2013).  A station set belongs to one mesh and
+ * one list of stations; fields are complex128, system-major in caller edge
+ * order, system 2f the first polarisation of frequency f and 2f + 1 the
+ * second (ForwardSolverFV's E1_ / E2_). */
+typedef struct m3d_stations m3d_stations;
+
+/* Station table (the host producer's find_stations_ result, :1393-1485, and
+ * the edge grouping of :1534-1562).  Per station: valid = 1 when it has a
+ * station cell; the cell's edges in the cell-edge order of the staggered
+ * grid, each with its role (bits 0-1 direction 0 x, 1 y, 2 z; bit 2 set for
+ * an x / y edge above the cell centre or a z edge on the +x side; bit 3 set
+ * for a z edge on the +y side); the cell sizes dx, dy, dz.  Per station edge
+ * (sorted, unique): the contributions of the adjoint right-hand side in the
+ * order the reference adds them (stations in order; in a station the E part
+ * before the H part; for z edges the y part before the x part), 4 ints each:
+ * station, kind (0: s_x we, 1: s_y we, 2: aH_y wh, 3: aH_x wh), axis * 2 +
+ * (sign < 0), n, where wh = sign * i_axis / n, i_axis = 1 / (i omega mu0
+ * h_axis) and axis 0 = z, 1 = x, 2 = y (:1928-2008); the E kinds carry their
+ * real weight we in contrib_we.  max_nf bounds the frequencies of one call
+ * (the device buffers are sized for it at creation). */
+typedef struct {
+    int n;                        /* edges of the mesh (length of one field) */
+    int ns;                       /* stations */
+    int max_nf;                   /* frequencies per call, at most */
+    const int* valid;             /* [ns] */
+    const int* edge_ptr;          /* [ns + 1] */
+    const int* edge;              /* [edge_ptr[ns]] caller edge ids */
+    const unsigned char* role;    /* [edge_ptr[ns]] */
+    const double* cell_size;      /* [3 ns] */
+    int nu;                       /* station edges */
+    const int* uedge;             /* [nu] */
+    const int* contrib_ptr;       /* [nu + 1] */
+    const int* contrib;           /* [4 contrib_ptr[nu]] */
+    const double* contrib_we;     /* [contrib_ptr[nu]] */
+} m3d_station_desc;
+
+/* Uploads the table; the set keeps its context alive until it is destroyed. */
+int m3d_stations_create(m3d_context* ctx, const m3d_station_desc* desc, m3d_stations** out);
+int m3d_stations_destroy(m3d_stations* st);
+
+/* extract_impedance_ (:1490-1727) of nf frequencies: omega [nf] host,
+ * station_sigma [ns] host (conductivity of each station cell, used by the
+ * 1-D fallback when |det H| <= 1e-30), d_E device [2 nf][n] complex, d_Z
+ * device [nf][ns][4] complex (Zxx, Zxy, Zyx, Zyy; 0 for stations without a
+ * cell).  The set keeps each (frequency, station) H, det H and Z on the
+ * device, as ForwardSolverFV's station_H_cache_ (forward_solver_fv.h:
+ * 216-224), for the next m3d_adjoint_rhs_device.  Returns after the context
+ * stream drains. */
+int m3d_impedance_device(m3d_stations* st, int nf, const double* omega, const double* station_sigma,
+                         const double* d_E, double* d_Z);
+
+/* Weighted residual and misfit of the frequencies of the last
+ * m3d_impedance_device call, its Z as the prediction: per datum
+ *   wr = w * w * conj(obs - pred), w = 1 / err   (inversion_fv.cpp:258-271)
+ * and d_phi[0] = sum over frequencies, stations and the four components of
+ * 0.5 w w |obs - pred|^2 in that order (:75-98); data with err <= 0 or
+ * weight <= 0 give wr = 0 and add nothing.  d_obs device [nf][ns][4]
+ * complex, d_err and d_weight device [nf][ns][4] double (d_weight may be
+ * NULL: all 1), d_wr device [nf][ns][4] complex, d_phi device double [1].
+ * Returns after the context stream drains. */
+int m3d_weighted_residual_device(m3d_stations* st, int nf, const double* d_obs, const double* d_err,
+                                 const double* d_weight, double* d_wr, double* d_phi);
+
+/* build_adjoint_rhs_from_residual (:1804-2013) for every frequency of the
+ * last m3d_impedance_device call: d_wr device [nf][ns][4] complex in, d_rhs
+ * device [2 nf][n] complex out (overwritten: the right-hand side of the
+ * first polarisation in system 2f, of the second in 2f + 1, zero off the
+ * station edges).  Returns after the context stream drains. */
+int m3d_adjoint_rhs_device(m3d_stations* st, int nf, const double* d_wr, double* d_rhs);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,7 @@ DEVICE_SOURCES = [
     "device/kernels.hip",
     "device/solver.hip",
     "device/nn.hip",
+    "device/stations.hip",
 ]
 
 ARCH = os.environ.get("M3D_OFFLOAD_ARCH", "gfx950")

@@ -20,74 +20,31 @@
 #include <string>
 #include <vector>
 
+#include "abi_common.h"
 #include "kernels.h"
 #include "m3d_device.h"
 
 using namespace m3d::dev;
 
-namespace {
+namespace m3d {
+namespace abi {
 
-thread_local std::string g_last_error;
+std::string& last_error() {
+    thread_local std::string msg;
+    return msg;
+}
 
-struct Fail : std::runtime_error {
-    int code;
-    Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            throw Fail(M3D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
-    } while (0)
-
-template <typename T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        release();
-        if (count == 0) count = 1;
-        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            throw Fail(M3D_ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
-        }
-        n = count;
-    }
-    void upload(const std::vector<T>& h, hipStream_t s) {
-        if (!p || n != std::max<size_t>(h.size(), 1)) alloc(h.size());
-        if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-};
-
-}  // namespace
-
-// A context lives until it has been destroyed by the caller and its last
-// operator is gone, whichever comes later.
-struct m3d_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int operators = 0;
-    bool closed = false;
-};
-
-namespace {
 void context_release(m3d_context* ctx) {
     if (!ctx->closed || ctx->operators > 0) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
-}  // namespace
+
+}  // namespace abi
+}  // namespace m3d
+
+using namespace m3d::abi;
 
 namespace {
 
@@ -900,6 +857,7 @@ void interleaved(m3d_operator& op, Work64<NS>& W, const double2* b, double2* x, 
             } else if (outer > 0 && tr[i] > 0.95 * prev[i]) {
                 active &= ~(1u << i);  // stagnation: accept (:866-873)
                 left |= 1u << i;
+                if (!unfinished) res[i].stagnated = 1;
             } else {
                 prev[i] = tr[i];
             }
@@ -1078,20 +1036,6 @@ void check_params(const m3d_solve_params* p) {
         throw Fail(M3D_ERR_ARG, "interleaved_forward must be 0 or 1");
     if (!(p->divcorr_pcg_tol >= 0.0 && p->divcorr_pcg_tol < 1.0))
         throw Fail(M3D_ERR_ARG, "divcorr_pcg_tol must be in [0, 1)");
-}
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        return M3D_OK;
-    } catch (const Fail& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return M3D_ERR_ARG;
-    }
 }
 
 // Dispatch a batch of any size onto the fixed batch widths (<= 32 per launch).
@@ -1604,7 +1548,7 @@ struct Staging {
 
 extern "C" {
 
-const char* m3d_last_error(void) { return g_last_error.c_str(); }
+const char* m3d_last_error(void) { return last_error().c_str(); }
 
 int m3d_context_create(int device, m3d_context** out) {
     return guarded([&] {

@@ -438,6 +438,107 @@ void FVProblem::adjoint_rhs_compact(double omega, const std::vector<StationHCach
     });
 }
 
+void FVProblem::station_table(StationTable& t) const {
+    const OctreeMesh& m = *mesh_;
+    const int ns = static_cast<int>(station_cell_.size());
+    t = StationTable{};
+    t.valid.assign(ns, 0);
+    t.edge_ptr.assign(ns + 1, 0);
+    t.cell_size.assign(3 * size_t(ns), 0.0);
+    struct Contribution {
+        int edge, station, kind, axis_sign, n;
+        double we;
+    };
+    std::vector<Contribution> cs;
+    for (int s = 0; s < ns; ++s) {
+        const int cell = station_cell_[s];
+        if (cell >= 0) {
+            t.valid[s] = 1;
+            double cx, cy, cz, dx, dy, dz;
+            m.cell_center(cell, cx, cy, cz);
+            m.cell_size_xyz(cell, dx, dy, dz);
+            t.cell_size[3 * s] = dx;
+            t.cell_size[3 * s + 1] = dy;
+            t.cell_size[3 * s + 2] = dz;
+            CellGroups g;
+            for (const CellEdge& ce : grid_.cell_edges[cell]) {
+                const int e = ce.edge_id;
+                const int d = grid_.edge_dir[e];
+                unsigned char role = static_cast<unsigned char>(d);
+                if (d == 0 || d == 1) {
+                    const bool top = grid_.edge_z[e] > cz;
+                    if (top) role |= 4;
+                    if (d == 0) { if (top) ++g.nxt; else ++g.nxb; }
+                    else { if (top) ++g.nyt; else ++g.nyb; }
+                } else {
+                    const bool px = grid_.edge_x[e] > cx, py = grid_.edge_y[e] > cy;
+                    if (px) role |= 4;
+                    if (py) role |= 8;
+                    if (px) ++g.nzpx; else ++g.nzmx;
+                    if (py) ++g.nzpy; else ++g.nzmy;
+                }
+                t.edge.push_back(e);
+                t.role.push_back(role);
+            }
+            // Weights and H terms exactly as adjoint_rhs_core forms them.
+            double wxt = g.nxt > 0 ? 0.5 / g.nxt : 0.0, wxb = g.nxb > 0 ? 0.5 / g.nxb : 0.0;
+            double wyt = g.nyt > 0 ? 0.5 / g.nyt : 0.0, wyb = g.nyb > 0 ? 0.5 / g.nyb : 0.0;
+            if (g.nxt == 0) wxb = g.nxb > 0 ? 1.0 / g.nxb : 0.0;
+            if (g.nxb == 0) wxt = g.nxt > 0 ? 1.0 / g.nxt : 0.0;
+            if (g.nyt == 0) wyb = g.nyb > 0 ? 1.0 / g.nyb : 0.0;
+            if (g.nyb == 0) wyt = g.nyt > 0 ? 1.0 / g.nyt : 0.0;
+            for (const CellEdge& ce : grid_.cell_edges[cell]) {
+                const int e = ce.edge_id;
+                const int d = grid_.edge_dir[e];
+                if (d == 0) {
+                    const bool top = grid_.edge_z[e] > cz;
+                    cs.push_back({e, s, 0, 0, 0, top ? wxt : wxb});
+                    if (top && g.nxt > 0) cs.push_back({e, s, 2, 0, g.nxt, 0.0});
+                    else if (!top && g.nxb > 0) cs.push_back({e, s, 2, 1, g.nxb, 0.0});
+                } else if (d == 1) {
+                    const bool top = grid_.edge_z[e] > cz;
+                    cs.push_back({e, s, 1, 0, 0, top ? wyt : wyb});
+                    if (top && g.nyt > 0) cs.push_back({e, s, 3, 1, g.nyt, 0.0});
+                    else if (!top && g.nyb > 0) cs.push_back({e, s, 3, 0, g.nyb, 0.0});
+                } else {
+                    const bool py = grid_.edge_y[e] > cy;
+                    if (py && g.nzpy > 0) cs.push_back({e, s, 3, 4, g.nzpy, 0.0});
+                    else if (!py && g.nzmy > 0) cs.push_back({e, s, 3, 5, g.nzmy, 0.0});
+                    const bool px = grid_.edge_x[e] > cx;
+                    if (px && g.nzpx > 0) cs.push_back({e, s, 2, 3, g.nzpx, 0.0});
+                    else if (!px && g.nzmx > 0) cs.push_back({e, s, 2, 2, g.nzmx, 0.0});
+                }
+            }
+        }
+        t.edge_ptr[s + 1] = static_cast<int>(t.edge.size());
+    }
+    t.uedge = station_edges();
+    const int nu = static_cast<int>(t.uedge.size());
+    t.contrib_ptr.assign(nu + 1, 0);
+    auto pos = [&](int e) {
+        return static_cast<int>(std::lower_bound(t.uedge.begin(), t.uedge.end(), e) - t.uedge.begin());
+    };
+    for (const Contribution& c : cs) ++t.contrib_ptr[pos(c.edge) + 1];
+    for (int k = 0; k < nu; ++k) t.contrib_ptr[k + 1] += t.contrib_ptr[k];
+    t.contrib.assign(4 * cs.size(), 0);
+    t.contrib_we.assign(cs.size(), 0.0);
+    std::vector<int> fill(t.contrib_ptr.begin(), t.contrib_ptr.end() - 1);
+    for (const Contribution& c : cs) {  // stable: keeps the order adjoint_rhs adds in
+        const int j = fill[pos(c.edge)]++;
+        t.contrib[4 * j] = c.station;
+        t.contrib[4 * j + 1] = c.kind;
+        t.contrib[4 * j + 2] = c.axis_sign;
+        t.contrib[4 * j + 3] = c.n;
+        t.contrib_we[j] = c.we;
+    }
+}
+
+void FVProblem::station_sigma(std::vector<double>& out) const {
+    out.assign(station_cell_.size(), 0.0);
+    for (size_t s = 0; s < station_cell_.size(); ++s)
+        if (station_cell_[s] >= 0) out[s] = sigma[station_cell_[s]];
+}
+
 // g_c = Re(i omega sum_{e in c, unconstrained} lambda_e E_e sigma_c A_dual L / n_adj)
 // with unconjugated lambda (:989-1059).
 void FVProblem::sensitivity_weights(std::vector<double>& w) const {

@@ -47,6 +47,30 @@ struct StationHCache {
     double cdx = 0, cdy = 0, cdz = 0;
 };
 
+// Station data of the device impedance and adjoint right-hand-side kernels
+// (m3d_station_desc in include/m3d_device.h).  Per station: whether it has a
+// cell, the station cell's edges in cell-edge order with their group role
+// (bits 0-1 direction; bit 2 top for x / y edges, +x side for z edges; bit 3
+// +y side for z edges: the grouping of :1534-1562) and the cell sizes.  Per
+// station edge (sorted, unique): the contributions the adjoint right-hand
+// side adds to it, in the order adjoint_rhs adds them (stations in order, in
+// each station the E part before the H part, for z edges the y part before
+// the x part).  contrib holds 4 ints per contribution: station, kind (0 E_x
+// part s_x * we, 1 E_y part s_y * we, 2 H part aH_y * wh, 3 H part aH_x * wh),
+// axis * 2 + (sign < 0) with wh = sign * i_axis / n and i_axis =
+// 1 / (i omega mu0 h_axis) for axis 0 = z, 1 = x, 2 = y, and n.
+struct StationTable {
+    std::vector<int> valid;             // [ns]
+    std::vector<int> edge_ptr;          // [ns + 1]
+    std::vector<int> edge;              // caller edge ids
+    std::vector<unsigned char> role;    // per edge of edge
+    std::vector<double> cell_size;      // [3 ns] dx, dy, dz
+    std::vector<int> uedge;             // [nu] station edges, sorted
+    std::vector<int> contrib_ptr;       // [nu + 1]
+    std::vector<int> contrib;           // [4 * contributions]
+    std::vector<double> contrib_we;     // weight of the E kinds (0 for H kinds)
+};
+
 class FVProblem {
 public:
     void setup(const OctreeMesh& mesh, const ForwardParams& params);
@@ -111,6 +135,10 @@ public:
     void adjoint_rhs_compact(double omega, const std::vector<StationHCache>& cache,
                              const std::vector<std::array<cplx, 4>>& wr, const std::vector<int>& edges,
                              cplx* rhs1, cplx* rhs2) const;
+    void station_table(StationTable& t) const;
+    // Conductivity of each station's cell (the 1-D fallback of the
+    // impedance, :1704-1712), 0 for stations without a cell.
+    void station_sigma(std::vector<double>& s) const;
     // Per-edge weight A_dual * L / n_adj of the sensitivity sum (0 for the
     // edges it skips), the data the device's sensitivity kernel needs.
     void sensitivity_weights(std::vector<double>& w) const;

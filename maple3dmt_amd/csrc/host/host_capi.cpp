@@ -197,6 +197,24 @@ struct m3dh_model {
         if (name == "me") return f64(P.me);
         if (name == "me_s") return f64(P.me_s);
         if (name == "station_cell") return i32(P.station_cells());
+        if (name.rfind("st_", 0) == 0) {
+            if (name == "st_sigma") {
+                std::vector<double> v;
+                P.station_sigma(v);
+                return f64(v);
+            }
+            StationTable t;
+            P.station_table(t);
+            if (name == "st_valid") return i32(t.valid);
+            if (name == "st_edge_ptr") return i32(t.edge_ptr);
+            if (name == "st_edge") return i32(t.edge);
+            if (name == "st_role") return u8(t.role);
+            if (name == "st_cell_size") return f64(t.cell_size);
+            if (name == "st_uedge") return i32(t.uedge);
+            if (name == "st_contrib_ptr") return i32(t.contrib_ptr);
+            if (name == "st_contrib") return i32(t.contrib);
+            if (name == "st_contrib_we") return f64(t.contrib_we);
+        }
         if (name == "hanging_pairs") {
             std::vector<int> v;
             for (const auto& h : o.hanging_pairs()) {
@@ -322,7 +340,7 @@ int m3dh_find_stations(m3dh_model* m, const double* xy, int n) {
         std::vector<std::array<double, 2>> v;
         for (int i = 0; i < n; ++i) v.push_back({xy[2 * i], xy[2 * i + 1]});
         m->fwd.find_stations(v);
-        m->cache.erase("station_cell");
+        m->cache.clear();
         m->station_edges.clear();
     });
 }

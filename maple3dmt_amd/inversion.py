@@ -28,22 +28,41 @@ class GradientResult:
     forward: list
     adjoint: list
 
+    @property
+    def unconverged_forward(self) -> int:
+        """Forward systems that ended without meeting the tolerance (stopped
+        by a breakdown guard, the iteration cap or the stagnation rule)."""
+        return sum(not r["converged"] for r in self.forward)
+
+    @property
+    def unconverged_adjoint(self) -> int:
+        """Adjoint systems that ended without meeting the tolerance; the
+        interleaved solve accepts a stagnated system as the reference does
+        (forward_solver_fv.cpp:866-873), and its gradient is built on it."""
+        return sum(not r["converged"] for r in self.adjoint)
+
 
 def weighted_residual(obs: np.ndarray, pred: np.ndarray, err: np.ndarray,
                       weight: Optional[np.ndarray] = None) -> np.ndarray:
-    """w^2 conj(d_obs - d_pred) with w = 1/error; zero where error or weight
-    is not positive (inversion_fv.cpp:258-271)."""
+    """w w conj(d_obs - d_pred) with w = 1 / error; zero where the error or
+    the weight is not positive (inversion_fv.cpp:258-271, the same
+    operations)."""
     wgt = np.ones_like(err) if weight is None else weight
     ok = (err > 0) & (wgt > 0)
     out = np.zeros_like(obs)
-    out[ok] = np.conj(obs[ok] - pred[ok]) / err[ok] ** 2
+    w = 1.0 / err[ok]
+    out[ok] = (w * w) * np.conj(obs[ok] - pred[ok])
     return out
 
 
 def misfit(obs, pred, err, weight=None) -> float:
+    """sum of 0.5 w w |d_obs - d_pred|^2 over the used data
+    (data_misfit_from_predicted_, inversion_fv.cpp:75-98)."""
     wgt = np.ones_like(err) if weight is None else weight
     ok = (err > 0) & (wgt > 0)
-    return 0.5 * float(np.sum(np.abs(obs[ok] - pred[ok]) ** 2 / err[ok] ** 2))
+    r = obs[ok] - pred[ok]
+    w = 1.0 / err[ok]
+    return float(np.sum(0.5 * w * w * (r.real * r.real + r.imag * r.imag)))
 
 
 def gradient(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
@@ -73,15 +92,18 @@ def gradient(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
 
 def gradient_device(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
                     params: Optional[SolveParams] = None,
-                    adjoint_params: Optional[SolveParams] = None) -> GradientResult:
-    """gradient() with the fields resident on the device: the forward batch
-    (FORWARD), the adjoint batch (INTERLEAVED) and the sensitivity sum over
-    every frequency and polarisation (m3d_sensitivity_device) run on the GPU.
-    Only the station cells' edges cross to the host: E is gathered there on
-    the device, Z and the station right-hand sides come from the compact
-    values (the reference's formulas, equal to the full-field path), and the
-    right-hand sides are scattered into zeroed device vectors.  torch
-    tensors are the device buffers."""
+                    adjoint_params: Optional[SolveParams] = None,
+                    z_weight: Optional[np.ndarray] = None) -> GradientResult:
+    """gradient() with everything between the two batches on the device: the
+    forward batch (FORWARD), the impedance of every frequency
+    (m3d_impedance_device), the weighted residual and misfit
+    (m3d_weighted_residual_device: w w conj(d_obs - d_pred), inversion_fv.cpp:
+    258-271, and the misfit in the order of :75-98), the adjoint right-hand
+    sides (m3d_adjoint_rhs_device), the adjoint batch (INTERLEAVED) and the
+    sensitivity sum over every frequency and polarisation
+    (m3d_sensitivity_device).  No field or station value crosses the bus
+    between the forward and the adjoint batch; Z, phi and the gradient are
+    read back once at the end.  torch tensors are the device buffers."""
     import torch
     dev = torch.device("cuda:0")
     freqs = np.asarray(freqs, dtype=np.float64)
@@ -90,28 +112,31 @@ def gradient_device(model: Model, freqs, z_obs: np.ndarray, z_err: np.ndarray,
     fwd_prm = params or model.fwd_params.solve_params()
     rhs = np.stack([model.rhs_total(model.bc_values(w, p % 2)) for p, w in enumerate(omegas)])
     aux = np.stack([model.primary_field(w, p % 2) for p, w in enumerate(omegas)])
+    st = model.device_stations(nf)
     d_rhs = torch.from_numpy(rhs).to(dev)
     d_aux = torch.from_numpy(aux).to(dev)
+    d_obs = torch.from_numpy(np.ascontiguousarray(z_obs, np.complex128)).to(dev)
+    d_err = torch.from_numpy(np.ascontiguousarray(z_err, np.float64)).to(dev)
+    d_wgt = (None if z_weight is None
+             else torch.from_numpy(np.ascontiguousarray(z_weight, np.float64)).to(dev))
     d_e = torch.zeros_like(d_rhs)
-    torch.cuda.synchronize()
+    d_z = torch.zeros((nf, st.ns, 4), dtype=torch.complex128, device=dev)
+    d_wr = torch.zeros_like(d_z)
+    d_phi = torch.zeros(1, dtype=torch.float64, device=dev)
+    d_arhs = torch.empty_like(d_e)
+    d_lam = torch.zeros_like(d_e)
+    d_g = torch.empty(model.nc, dtype=torch.float64, device=dev)
+    # torch's stream only: a device-wide synchronize would also wait on (and
+    # break the graph capture of) the streams other host threads are
+    # solving on (multimodel.run).
+    torch.cuda.current_stream().synchronize()
     fres = model.device.solve_device(MODE_FORWARD, omegas, d_rhs.data_ptr(), d_aux.data_ptr(),
                                      d_e.data_ptr(), 2 * nf, fwd_prm)
-    idx = torch.from_numpy(model.station_edges().astype(np.int64)).to(dev)
-    Ec = d_e.index_select(1, idx).cpu().numpy()          # [2 nf, station edges]
-    Z = np.zeros_like(z_obs)
-    arhs_c = np.zeros_like(Ec)
-    for f in range(nf):
-        w = omegas[2 * f]
-        Z[f] = model.extract_impedance_compact(w, Ec[2 * f], Ec[2 * f + 1])
-        arhs_c[2 * f], arhs_c[2 * f + 1] = model.adjoint_rhs_compact(
-            w, weighted_residual(z_obs[f], Z[f], z_err[f]))
-    phi = misfit(z_obs, Z, z_err)
-    d_arhs = torch.zeros_like(d_e)
-    d_arhs.index_copy_(1, idx, torch.from_numpy(arhs_c).to(dev))
-    d_lam = torch.zeros_like(d_arhs)
-    torch.cuda.synchronize()
+    st.impedance(2 * np.pi * freqs, d_e.data_ptr(), d_z.data_ptr())
+    st.weighted_residual(nf, d_obs.data_ptr(), d_err.data_ptr(),
+                         None if d_wgt is None else d_wgt.data_ptr(), d_wr.data_ptr(), d_phi.data_ptr())
+    st.adjoint_rhs(nf, d_wr.data_ptr(), d_arhs.data_ptr())
     ares = model.device.solve_device(MODE_INTERLEAVED, omegas, d_arhs.data_ptr(), 0, d_lam.data_ptr(),
                                      2 * nf, adjoint_params or fwd_prm)
-    d_g = torch.empty(model.nc, dtype=torch.float64, device=dev)
     model.device.sensitivity_device(omegas, d_e.data_ptr(), d_lam.data_ptr(), d_g.data_ptr())
-    return GradientResult(phi, d_g.cpu().numpy(), Z, fres, ares)
+    return GradientResult(float(d_phi.cpu().item()), d_g.cpu().numpy(), d_z.cpu().numpy(), fres, ares)

@@ -71,21 +71,30 @@ def prepare(jobs: Sequence[ModelJob], contexts: Sequence[Context], n_threads: Op
 
 
 def run(jobs: Sequence[ModelJob], n_workers: int, params: Optional[SolveParams] = None,
-        adjoint_params: Optional[SolveParams] = None, release: bool = False) -> float:
+        adjoint_params: Optional[SolveParams] = None, release: bool = False,
+        device: bool = False) -> float:
     """Gradient of every job; with one worker all jobs run in order on the
-    calling thread's schedule, else worker w runs the jobs assigned to
-    context w in order.  With release, each model (and its device memory) is
-    closed once its gradient is done.  Returns the wall time in seconds."""
+    calling thread, else worker w runs, in order, the jobs whose context
+    index is w modulo n_workers (so every job runs, and a context's jobs stay
+    on one thread).  device: the device-resident gradient
+    (inversion.gradient_device) instead of the host-station one.  With
+    release, each model (and its device memory) is closed once its gradient
+    is done.  Returns the wall time in seconds."""
+    if n_workers < 1:
+        raise ValueError("n_workers must be at least 1")
+    if any(job.worker < 0 or job.model is None for job in jobs):
+        raise ValueError("every job must be prepared (prepare()) before run()")
     errors: List[BaseException] = []
+    grad = inversion.gradient_device if device else inversion.gradient
 
     def work(w: int):
         try:
             for job in jobs:
-                if n_workers > 1 and job.worker != w:
+                if job.worker % n_workers != w:
                     continue
                 t = time.perf_counter()
-                job.result = inversion.gradient(job.model, job.workload.freqs, job.z_obs, job.z_err,
-                                                params, adjoint_params)
+                job.result = grad(job.model, job.workload.freqs, job.z_obs, job.z_err, params,
+                                  adjoint_params)
                 job.seconds = time.perf_counter() - t
                 if release:
                     job.model.close()

@@ -84,6 +84,7 @@ class SolveResultC(C.Structure):
         ("outer", C.c_int),
         ("guess_used", C.c_int),
         ("guess_residual", C.c_double),
+        ("stagnated", C.c_int),
     ]
 
 
@@ -127,6 +128,16 @@ class LaunchShapeC(C.Structure):
     ]
 
 
+class StationDescC(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("ns", C.c_int), ("max_nf", C.c_int),
+        ("valid", C.c_void_p), ("edge_ptr", C.c_void_p), ("edge", C.c_void_p),
+        ("role", C.c_void_p), ("cell_size", C.c_void_p), ("nu", C.c_int),
+        ("uedge", C.c_void_p), ("contrib_ptr", C.c_void_p), ("contrib", C.c_void_p),
+        ("contrib_we", C.c_void_p),
+    ]
+
+
 # Every symbol the two public headers declare.
 DEVICE_SYMBOLS = [
     "m3d_context_create", "m3d_context_destroy", "m3d_last_error", "m3d_operator_create",
@@ -136,6 +147,8 @@ DEVICE_SYMBOLS = [
     "m3d_operator_set_sensitivity", "m3d_sensitivity_device", "m3d_operator_get_shape",
     "m3d_operator_set_shape", "m3d_operator_autotune", "m3d_operator_set_mass_structure",
     "m3d_operator_update_sigma", "m3d_operator_update_sigma_device", "m3d_perturbation_rhs_device",
+    "m3d_stations_create", "m3d_stations_destroy", "m3d_impedance_device",
+    "m3d_weighted_residual_device", "m3d_adjoint_rhs_device",
 ]
 HOST_SYMBOLS = [
     "m3dh_last_error", "m3dh_model_create", "m3dh_model_destroy", "m3dh_get_array",
@@ -192,6 +205,11 @@ def lib() -> C.CDLL:
     L.m3d_operator_get_shape.argtypes = [vp, i, i, p(LaunchShapeC)]
     L.m3d_operator_set_shape.argtypes = [vp, i, i, p(LaunchShapeC)]
     L.m3d_operator_autotune.argtypes = [vp, i, vp, i, vp]
+    L.m3d_stations_create.argtypes = [vp, p(StationDescC), p(vp)]
+    L.m3d_stations_destroy.argtypes = [vp]
+    L.m3d_impedance_device.argtypes = [vp, i, vp, vp, vp, vp]
+    L.m3d_weighted_residual_device.argtypes = [vp, i, vp, vp, vp, vp, vp]
+    L.m3d_adjoint_rhs_device.argtypes = [vp, i, vp, vp]
     L.m3dh_model_create.argtypes = [p(MeshParamsC), vp, i, vp, i, p(ForwardParamsC), vp, p(vp)]
     L.m3dh_model_destroy.argtypes = [vp]
     L.m3dh_get_array.argtypes = [vp, C.c_char_p, p(i), p(C.c_longlong), p(vp)]
@@ -326,7 +344,7 @@ def results_to_dicts(res) -> list:
     return [dict(iterations=r.iterations, residual=r.residual, true_residual=r.true_residual,
                  converged=bool(r.converged), diverged=bool(r.diverged), info=r.info,
                  outer=r.outer, guess_used=bool(r.guess_used),
-                 guess_residual=r.guess_residual) for r in res]
+                 guess_residual=r.guess_residual, stagnated=bool(r.stagnated)) for r in res]
 
 
 # --------------------------------------------------------------------------
@@ -510,6 +528,55 @@ class DeviceOperator:
 _DT = {0: np.float64, 1: np.int32, 2: np.uint8, 3: np.complex128}
 
 
+class DeviceStations:
+    """The station side of the gradient on the device (m3d_stations_*): the
+    impedance of device fields, the weighted residual and misfit, and the
+    adjoint right-hand side, with every buffer in HBM (device pointers as
+    ints, e.g. torch tensors' data_ptr()).  Built from a Model's station
+    table for at most max_nf frequencies per call."""
+
+    def __init__(self, model: "Model", ctx: Context, max_nf: int):
+        keep = {k: model.array("st_" + k) for k in
+                ("valid", "edge_ptr", "edge", "role", "cell_size", "uedge", "contrib_ptr", "contrib",
+                 "contrib_we")}
+        keep = {k: np.ascontiguousarray(v) for k, v in keep.items()}
+        d = StationDescC()
+        d.n, d.ns, d.max_nf = model.n, len(keep["valid"]), int(max_nf)
+        d.nu = len(keep["uedge"])
+        for k, v in keep.items():
+            setattr(d, k, _ptr(v) if v.size else None)
+        h = C.c_void_p()
+        _check_dev(lib().m3d_stations_create(ctx.handle, C.byref(d), C.byref(h)))
+        self.handle = h
+        self.ns, self.max_nf, self.n = d.ns, d.max_nf, d.n
+        self.sigma = np.ascontiguousarray(model.array("st_sigma"), dtype=np.float64)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().m3d_stations_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def impedance(self, omegas, d_e: int, d_z: int) -> None:
+        """Z [nf][ns][4] of the fields [2 nf][n] (m3d_impedance_device)."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64).reshape(-1)
+        _check_dev(lib().m3d_impedance_device(self.handle, len(om), _ptr(om),
+                                              _ptr(self.sigma) if self.ns else None, d_e, d_z))
+
+    def weighted_residual(self, nf: int, d_obs: int, d_err: int, d_weight: Optional[int], d_wr: int,
+                          d_phi: int) -> None:
+        _check_dev(lib().m3d_weighted_residual_device(self.handle, nf, d_obs, d_err, d_weight or None,
+                                                      d_wr, d_phi))
+
+    def adjoint_rhs(self, nf: int, d_wr: int, d_rhs: int) -> None:
+        _check_dev(lib().m3d_adjoint_rhs_device(self.handle, nf, d_wr, d_rhs))
+
+
 class Model:
     """Octree mesh, staggered grid, operators and host solve data of one
     conductivity model; with a Context it also owns the device operator."""
@@ -550,8 +617,10 @@ class Model:
         self.nc = int(self.count("cells"))
         self.nn = int(self.count("nodes"))
         self._stations = None
+        self._dev_stations = None
 
     def close(self) -> None:
+        self._drop_stations()
         if getattr(self, "handle", None):
             lib().m3dh_model_destroy(self.handle)
             self.handle = None
@@ -597,9 +666,32 @@ class Model:
         assert s.shape == (self.nc,)
         _check_host(lib().m3dh_update_sigma_device(self.handle, _ptr(s)))
 
+    def device_stations(self, max_nf: int) -> DeviceStations:
+        """The device station set of the current stations (kept and reused
+        while the stations stay the same and max_nf fits; the station cells'
+        conductivity for the 1-D fallback is taken again from the current
+        sigma)."""
+        if self.ctx is None:
+            raise NativeError("model has no device operator (created without a Context)")
+        st = self._dev_stations
+        if st is None or st.max_nf < max_nf or st.handle is None:
+            if st is not None:
+                st.close()
+            self._dev_stations = st = DeviceStations(self, self.ctx, max_nf)
+        else:
+            st.sigma = np.ascontiguousarray(self.array("st_sigma"), dtype=np.float64)
+        return st
+
+    def _drop_stations(self) -> None:
+        st = getattr(self, "_dev_stations", None)
+        if st is not None:
+            st.close()
+            self._dev_stations = None
+
     def find_stations(self, xy) -> np.ndarray:
         a = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
         _check_host(lib().m3dh_find_stations(self.handle, _ptr(a), a.shape[0]))
+        self._drop_stations()
         self._stations = a.shape[0]
         return self.array("station_cell")
 

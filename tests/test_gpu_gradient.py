@@ -82,5 +82,12 @@ def test_device_resident_gradient_matches_host_path(ctx):
     obs, err = halfspace_data(wl)
     a = inversion.gradient(m, wl.freqs, obs, err)
     b = inversion.gradient_device(m, wl.freqs, obs, err)
+    # The device forms Z and the adjoint right-hand sides with the host's
+    # formulas (equal to 1e-12, test_gpu_stations.py); the adjoint BiCGStab
+    # amplifies last-bit differences of its right-hand side into a different
+    # trajectory to the same 1e-7 residual, so the gradients agree to the
+    # solve tolerance, not bit for bit: the bar of the oracle comparison
+    # (1.2e-6 measured here, 3e-4 on data closer to the prediction).
     assert abs(a.phi - b.phi) <= 1e-12 * abs(a.phi)
-    assert np.linalg.norm(a.grad - b.grad) <= 1e-10 * np.linalg.norm(a.grad)
+    assert np.abs(a.z_pred - b.z_pred).max() <= 1e-12 * np.abs(a.z_pred).max()
+    assert np.linalg.norm(a.grad - b.grad) <= 2e-3 * np.linalg.norm(a.grad)

@@ -5,6 +5,8 @@ import pytest
 
 from maple3dmt_amd import configs, multimodel
 from maple3dmt_amd.native import Context
+from conftest import solved
+from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -28,16 +30,18 @@ def test_concurrent_models_match_sequential():
     multimodel.prepare(par, contexts)
     multimodel.run(par, 3)
     for a, b in zip(seq, par):
-        # Solves stop as the reference's do: converged, or by its absolute
-        # breakdown guards (|rho|, |omega| < 1e-30, bicgstab.cpp:79-85,
-        # 152-157), which a tiny right-hand side can reach near the
-        # tolerance, or (interleaved adjoint) by its stagnation rule
-        # (forward_solver_fv.cpp:866-873); never by diverging.
+        # Forward solves meet the tolerance (converged, or stopped by one of
+        # the reference's absolute breakdown guards at a true residual
+        # already below it).  Interleaved adjoint solves end as the
+        # reference's do: converged, or accepted by its stagnation rule
+        # (a burst gaining less than 5 %, forward_solver_fv.cpp:866-873);
+        # never diverged.
         fw = [(r["converged"], r["info"], r["iterations"], r["true_residual"]) for r in a.result.forward]
-        ad = [(r["converged"], r["diverged"], r["iterations"], r["true_residual"]) for r in a.result.adjoint]
-        assert all(r["converged"] or r["true_residual"] < 1e-5 for r in a.result.forward), (a.workload.name, fw)
+        ad = [(r["converged"], r["stagnated"], r["diverged"], r["true_residual"]) for r in a.result.adjoint]
+        assert all(solved(r) for r in a.result.forward), (a.workload.name, fw)
         assert not any(r["diverged"] for r in a.result.adjoint), (a.workload.name, ad)
-        assert all(r["converged"] or r["true_residual"] < 1e-3 for r in a.result.adjoint), (a.workload.name, ad)
+        assert all(solved(r) or r["stagnated"] for r in a.result.adjoint), (a.workload.name, ad)
+        assert a.result.unconverged_forward == 0
         assert [r["iterations"] for r in a.result.adjoint] == [r["iterations"] for r in b.result.adjoint]
         assert a.result.phi == b.result.phi
         assert np.array_equal(a.result.grad, b.result.grad)
@@ -90,3 +94,53 @@ def test_trim_frees_workspaces_and_keeps_results():
     assert np.array_equal(Z1, Z2)
     m.close()
     ctx.close()
+
+
+def test_uneven_models_concurrent_gradients_match_oracle():
+    """The 16-model batch in small: three models on meshes of uneven size
+    (levels 3-4 and 3-5, two station radii), each with its forward and
+    adjoint systems, run concurrently on three streams with the station
+    steps on the device (multimodel.run(device=True)); each model's misfit
+    and gradient against the oracle's per-frequency loop
+    (inversion_fv.cpp:250-290), to the bar of the single-model gradient
+    test (2e-3)."""
+    contexts = [Context(0) for _ in range(3)]
+    jobs, oracles = [], []
+    for seed, level, radius in ((31, 4, 5e3), (32, 5, 5e3), (33, 5, 12e3)):
+        wl = configs.random3d(seed, min_level=3, max_level=level, n_freq=2, half_width=50e3,
+                              station_radius=radius)
+        jobs.append(multimodel.ModelJob(wl, None, None))
+    multimodel.prepare(jobs, contexts)
+    sizes = [j.model.n for j in jobs]
+    print("edges per model", sizes, flush=True)
+    assert len(set(sizes)) == 3
+    for j in jobs:
+        m, wl = j.model, j.workload
+        o = Oracle(m, sigma_bg=wl.sigma_bg)
+        o.set_sigma(m.array("sigma"))
+        o.find_stations(wl.stations[:, :2])
+        zs = [o.solve_frequency(f)[2] for f in wl.freqs]
+        j.z_obs = np.stack(zs) * (1.05 + 0.02j)
+        floor = 0.05 * np.maximum(np.abs(j.z_obs[:, :, 1]), np.abs(j.z_obs[:, :, 2]))
+        j.z_err = np.repeat(floor[:, :, None], 4, axis=2)
+        oracles.append(o)
+    multimodel.run(jobs, 3, device=True)
+    for j, o in zip(jobs, oracles):
+        res = j.result
+        assert all(solved(r) for r in res.forward), res.forward
+        assert all(solved(r) or r["stagnated"] for r in res.adjoint), res.adjoint
+        phi_o, g_o = 0.0, np.zeros(j.model.nc)
+        for fi, f in enumerate(j.workload.freqs):
+            e1, e2, z, _ = o.solve_frequency(f)
+            r = j.z_obs[fi] - z
+            phi_o += 0.5 * float(np.sum(np.abs(r) ** 2 / j.z_err[fi] ** 2))
+            q1, q2 = o.adjoint_rhs(np.conj(r) / j.z_err[fi] ** 2)
+            g_o += o.sensitivity(e1, o.adjoint(q1)[0]) + o.sensitivity(e2, o.adjoint(q2)[0])
+        print(j.workload.name, "phi", res.phi, phi_o, "grad rel err",
+              np.linalg.norm(res.grad - g_o) / np.linalg.norm(g_o), flush=True)
+        assert abs(res.phi - phi_o) <= 2e-3 * phi_o
+        assert np.linalg.norm(res.grad - g_o) <= 2e-3 * np.linalg.norm(g_o)
+    for j in jobs:
+        j.model.close()
+    for c in contexts:
+        c.close()
