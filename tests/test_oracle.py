@@ -285,3 +285,55 @@ def test_adjoint_gradient_matches_finite_differences():
         gfd = (pp - pm) / (2 * eps)
         scale = max(abs(gfd), abs(g[c]))
         assert abs(g[c] - gfd) / scale < 0.1, (c, g[c], gfd)
+
+
+def test_impedance_exact_on_discrete_1d_solution():
+    # On a uniform mesh the laterally uniform field E_x(z) = q^d (d = depth in
+    # cells) with q + 1/q = 2 + i w mu0 sigma h^2, |q| < 1, solves every
+    # interior earth row of (S + i w Me) E = 0 exactly: the stiffness row of an
+    # x edge is (h / mu0) (2 E_0 - E_up - E_down) and Me = sigma h^3.  The
+    # station cell's averages and finite-difference curl
+    # (forward_solver_fv.cpp:1534-1632) then give, to round-off,
+    #   Zxy = i w mu0 h (1 + q) / (2 (1 - q)),  Zyx = -Zxy,  Zxx = Zyy = 0,
+    # which tends to sqrt(i w mu0 / sigma) as h -> 0.
+    mp = MeshParams(domain=(-50e3, 50e3, -50e3, 50e3, -50e3, 50e3), min_level=3, max_level=3,
+                    sigma_bg=0.01)
+    stations = configs.station_grid(3, 2000.0)
+    m = Model(mp, stations, [0.1])
+    cz = m.array("cell_cz")
+    sig = np.where(cz < 0, 0.01, 1e-8)
+    m.set_sigma(sig)
+    m.find_stations(stations[:, :2])
+    h = 100e3 / 8
+    f = 0.1
+    w = 2 * np.pi * f
+    kap = 1j * w * MU0 * 0.01 * h * h
+    q = ((2 + kap) - np.sqrt((2 + kap) ** 2 - 4)) / 2
+    if abs(q) > 1:
+        q = 1 / q
+    ez, ed = m.array("edge_z"), m.array("edge_dir")
+    depth = np.rint(-ez / h)
+    prof = np.where(ez <= 0, q ** np.maximum(depth, 0), 1.0 + 0.1 * ez / h)
+    e1 = np.where(ed == 0, prof, 0).astype(np.complex128)
+    e2 = np.where(ed == 1, prof, 0).astype(np.complex128)
+    # The field is the discrete solution on the interior earth x edges.
+    S = sp.csr_matrix((m.array("S_val"), m.array("S_col"), m.array("S_ptr")), shape=(m.n, m.n))
+    A = S + sp.diags(1j * w * m.array("me"))
+    ex, ey = m.array("edge_x"), m.array("edge_y")
+    inner = (ed == 0) & (depth >= 1) & (ez > -50e3 + h / 2) & (np.abs(ey) < 50e3 - h / 2)
+    assert inner.sum() > 50
+    r = A @ e1
+    assert np.abs(r[inner]).max() <= 1e-12 * np.abs(S.data).max()
+    zc = 1j * w * MU0 * h * (1 + q) / (2 * (1 - q))
+    o = Oracle(m)
+    o.set_sigma(sig)
+    o.find_stations(stations[:, :2])
+    o.assemble(w)
+    for z in (m.extract_impedance(w, e1, e2), o.impedance(e1, e2)):
+        assert np.abs(z[:, 1] - zc).max() <= 1e-13 * abs(zc)
+        assert np.abs(z[:, 2] + zc).max() <= 1e-13 * abs(zc)
+        assert np.abs(z[:, [0, 3]]).max() <= 1e-13 * abs(zc)
+    # Continuous limit: within the O(kh) discretisation error of the
+    # halfspace value sqrt(i w mu0 / sigma) (|kh| ~ 1.4 here).
+    zh = np.sqrt(1j * w * MU0 / 0.01)
+    assert abs(zc / zh - 1) < 0.3
