@@ -4,8 +4,14 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace m3d {
+
+// The driver is a complete IForwardSolver: a caller holding the interface
+// (InversionFV, ImplicitJtJOperatorFV) can be handed this class.
+static_assert(std::is_base_of<forward::IForwardSolver, ForwardSolverGPU>::value, "IForwardSolver base");
+static_assert(!std::is_abstract<ForwardSolverGPU>::value, "every IForwardSolver member is implemented");
 
 ForwardSolverGPU::~ForwardSolverGPU() {
     if (op_) m3d_operator_destroy(op_);
@@ -87,6 +93,14 @@ void ForwardSolverGPU::update_sigma_device(const std::vector<double>& sigma) {
     prob_.sigma = sigma;
 }
 
+void ForwardSolverGPU::update_sigma(const model::ConductivityModel& model) {
+    const int nc = prob_.num_cells();
+    if (model.num_params() < nc) throw std::invalid_argument("conductivity model has fewer cells than the mesh");
+    std::vector<double> sigma(nc);
+    for (int c = 0; c < nc; ++c) sigma[c] = model.sigma(c);
+    set_sigma(sigma);
+}
+
 void ForwardSolverGPU::set_sigma(const std::vector<double>& sigma) {
     prob_.set_sigma(sigma);
     if (!op_) return;
@@ -137,6 +151,8 @@ void ForwardSolverGPU::map_stations(const data::MTData& observed) {
 void ForwardSolverGPU::compute_responses(const data::MTData& observed, data::MTData& predicted) {
     map_stations(observed);
     const auto& f = observed.frequencies();
+    if (freq_cb_)
+        for (size_t fi = 0; fi < f.size(); ++fi) freq_cb_(int(fi), int(f.size()), 0.0, "Forward");
     std::vector<std::vector<cplx>> e;
     std::vector<std::vector<std::array<cplx, 4>>> z;
     solve_frequencies(f, e, z, &last_);

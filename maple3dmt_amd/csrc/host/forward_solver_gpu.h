@@ -14,16 +14,17 @@
 #include <vector>
 
 #include "fv_problem.h"
+#include "iforward_solver.h"
 #include "m3d_device.h"
 #include "mt_data.h"
 #include "octree_mesh.h"
 
 namespace m3d {
 
-class ForwardSolverGPU {
+class ForwardSolverGPU : public forward::IForwardSolver {
 public:
     ForwardSolverGPU() = default;
-    ~ForwardSolverGPU();
+    ~ForwardSolverGPU() override;
     ForwardSolverGPU(const ForwardSolverGPU&) = delete;
     ForwardSolverGPU& operator=(const ForwardSolverGPU&) = delete;
 
@@ -38,35 +39,42 @@ public:
     // host keeps the new sigma; its own Me and Laplacian copies are not
     // refreshed.  Needs one set_sigma first (the Laplacian pattern).
     void update_sigma_device(const std::vector<double>& sigma);
+    // IForwardSolver::update_sigma (forward_solver_fv.cpp:53-58): sigma of
+    // every cell from the model, then the device arrays as set_sigma.
+    void update_sigma(const model::ConductivityModel& model) override;
 
-    void compute_responses(const data::MTData& observed, data::MTData& predicted);
+    void compute_responses(const data::MTData& observed, data::MTData& predicted) override;
     void compute_single_frequency(int freq_idx, const data::MTData& observed,
-                                  data::MTData& predicted);
+                                  data::MTData& predicted) override;
     void solve_frequency(double freq_hz, int freq_idx, const data::MTData& observed,
                          data::MTData& predicted);
-    void factorize_frequency(double freq_hz);
-    void release_factorization();
+    void factorize_frequency(double freq_hz) override;
+    void release_factorization() override;
 
     void build_adjoint_rhs_from_residual(int freq_idx,
                                          const std::vector<std::array<cplx, 4>>& weighted_residual,
                                          std::vector<cplx>& adj_rhs_pol1,
-                                         std::vector<cplx>& adj_rhs_pol2);
-    void adjoint_solve_complex(const std::vector<cplx>& rhs, std::vector<cplx>& lambda);
-    void solve_rhs_complex(const std::vector<cplx>& rhs, std::vector<cplx>& solution);
+                                         std::vector<cplx>& adj_rhs_pol2) override;
+    void adjoint_solve_complex(const std::vector<cplx>& rhs, std::vector<cplx>& lambda) override;
+    void solve_rhs_complex(const std::vector<cplx>& rhs, std::vector<cplx>& solution) override;
     void compute_sensitivity_complex(const std::vector<cplx>& e_bg, const std::vector<cplx>& lambda,
-                                     std::vector<double>& sensitivity);
+                                     std::vector<double>& sensitivity) override;
     void build_perturbation_rhs_complex(int polarization, const std::vector<double>& delta_sigma,
-                                        std::vector<cplx>& pert_rhs);
+                                        std::vector<cplx>& pert_rhs) override;
     void extract_delta_impedance_complex(const std::vector<cplx>& de, int polarization,
-                                         std::vector<std::array<cplx, 4>>& delta_z);
-    const std::vector<cplx>& background_E(int pol) const { return pol == 0 ? e1_ : e2_; }
-    void set_background_fields_complex(const std::vector<cplx>& e1, const std::vector<cplx>& e2) {
+                                         std::vector<std::array<cplx, 4>>& delta_z) override;
+    const std::vector<cplx>& background_E(int pol) const override { return pol == 0 ? e1_ : e2_; }
+    void set_background_fields_complex(const std::vector<cplx>& e1, const std::vector<cplx>& e2) override {
         e1_ = e1;
         e2_ = e2;
     }
-    void set_solver_tolerance_override(double tol) { tol_override_ = tol; }
-    double current_omega() const { return omega_; }
-    int num_elements() const { return prob_.num_cells(); }
+    void set_solver_tolerance_override(double tol) override { tol_override_ = tol; }
+    double current_omega() const override { return omega_; }
+    int num_elements() const override { return prob_.num_cells(); }
+    // Called once per frequency as (index, count, 0, "Forward") before the
+    // batch that solves them, as ForwardSolverFV forwards it
+    // (forward_solver_fv.h:139-143, forward_solver_fv.cpp:73-75).
+    void set_freq_progress_callback(FreqProgressCB cb) override { freq_cb_ = std::move(cb); }
 
     // Batched forward: both polarisations of every frequency in one device
     // call.  e_out[2*f + pol] receives the total field, z_out[f] the station
@@ -103,6 +111,7 @@ private:
     std::vector<cplx> e1_, e2_;
     std::vector<StationHCache> cache_;
     std::vector<m3d_solve_result> last_;
+    FreqProgressCB freq_cb_;
 
     void map_stations(const data::MTData& observed);
     void check(int status) const;
