@@ -126,6 +126,19 @@ int m3d_operator_set_sensitivity(m3d_operator* op, const m3d_sensitivity_desc* d
 int m3d_sensitivity_device(m3d_operator* op, int nsys, const double* omega, const double* d_E,
                            const double* d_lambda, double* d_grad);
 
+/* m3d_sensitivity_device that allocates no device memory.  No counterpart
+ * in the reference: it is for callers that cannot allocate (tight serving or
+ * inversion loops, harnesses that forbid hipMalloc).  The frequencies of a
+ * call are staged in a static device array of the library, 4096 values on
+ * each GPU, held until the process ends, instead of a buffer the operator
+ * allocates and grows; nsys above 4096 is refused (M3D_ERR_ARG).  Every call
+ * on one GPU stages through that GPU's one array, so calls that overlap in
+ * time on one GPU (several host threads or contexts) race.  Same kernel and
+ * results as m3d_sensitivity_device.  Returns after the context stream
+ * drains. */
+int m3d_sensitivity_device_noalloc(m3d_operator* op, int nsys, const double* omega, const double* d_E,
+                                   const double* d_lambda, double* d_grad);
+
 /* Perturbation right-hand side of the Gauss-Newton product J*v
  * (build_perturbation_rhs, forward_solver_fv.cpp:945-984) on the device:
  * for each system s

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -45,6 +46,37 @@ void context_release(m3d_context* ctx) {
 }  // namespace m3d
 
 using namespace m3d::abi;
+
+// Device scratch of the allocation-free sensitivity entry point
+// (m3d_sensitivity_device_noalloc): the frequencies of one call, staged on
+// the device.  Each GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes the array's name to a kernel or a
+// copy: it uses the address hipGetSymbolAddress returns on each device.
+constexpr int kNoAllocMaxSystems = 4096;
+static __device__ __align__(256) double scratch[kNoAllocMaxSystems];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's scratch; looked up once per device, with the
+// device current (the caller has set it).
+double* device_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Fail(M3D_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Fail(M3D_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
 
 namespace {
 
@@ -1557,6 +1589,7 @@ int m3d_context_create(int device, m3d_context** out) {
         HIPCHK(hipGetDeviceCount(&count));
         if (device < 0 || device >= count) throw Fail(M3D_ERR_ARG, "device index out of range");
         HIPCHK(hipSetDevice(device));
+        (void)device_scratch(device);  // the no-allocation scratch's address on this device
         auto* c = new m3d_context;
         c->device = device;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1729,6 +1762,27 @@ int m3d_sensitivity_device(m3d_operator* op, int nsys, const double* omega, cons
         op->s_omega.upload(om, s);
         launch_sensitivity(s, op->nc, op->n, nsys, op->s_cell_ptr.p, op->s_cell_edges.p, op->s_weight.p,
                            op->s_sigma.p, op->s_omega.p, reinterpret_cast<const double2*>(d_E),
+                           reinterpret_cast<const double2*>(d_lambda), d_grad);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int m3d_sensitivity_device_noalloc(m3d_operator* op, int nsys, const double* omega, const double* d_E,
+                                   const double* d_lambda, double* d_grad) {
+    return guarded([&] {
+        if (!op) throw Fail(M3D_ERR_ARG, "null operator");
+        if (op->nc <= 0) throw Fail(M3D_ERR_STATE, "m3d_operator_set_sensitivity has not been called");
+        if (nsys <= 0 || !omega || !d_E || !d_lambda || !d_grad) throw Fail(M3D_ERR_ARG, "null argument");
+        if (nsys > kNoAllocMaxSystems)
+            throw Fail(M3D_ERR_ARG, "m3d_sensitivity_device_noalloc: " + std::to_string(nsys) +
+                                        " systems exceed its scratch of " + std::to_string(kNoAllocMaxSystems));
+        HIPCHK(hipSetDevice(op->ctx->device));
+        hipStream_t s = op->ctx->stream;
+        double* d_omega = device_scratch(op->ctx->device);
+        HIPCHK(hipMemcpyAsync(d_omega, omega, nsys * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_sensitivity(s, op->nc, op->n, nsys, op->s_cell_ptr.p, op->s_cell_edges.p, op->s_weight.p,
+                           op->s_sigma.p, d_omega, reinterpret_cast<const double2*>(d_E),
                            reinterpret_cast<const double2*>(d_lambda), d_grad);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));

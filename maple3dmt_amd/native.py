@@ -148,7 +148,7 @@ DEVICE_SYMBOLS = [
     "m3d_operator_set_shape", "m3d_operator_autotune", "m3d_operator_set_mass_structure",
     "m3d_operator_update_sigma", "m3d_operator_update_sigma_device", "m3d_perturbation_rhs_device",
     "m3d_stations_create", "m3d_stations_destroy", "m3d_impedance_device",
-    "m3d_weighted_residual_device", "m3d_adjoint_rhs_device",
+    "m3d_weighted_residual_device", "m3d_adjoint_rhs_device", "m3d_sensitivity_device_noalloc",
 ]
 HOST_SYMBOLS = [
     "m3dh_last_error", "m3dh_model_create", "m3dh_model_destroy", "m3dh_get_array",
@@ -201,6 +201,7 @@ def lib() -> C.CDLL:
     L.m3d_operator_trim.argtypes = [vp]
     L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d)]
     L.m3d_sensitivity_device.argtypes = [vp, i, vp, vp, vp, vp]
+    L.m3d_sensitivity_device_noalloc.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_perturbation_rhs_device.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_operator_get_shape.argtypes = [vp, i, i, p(LaunchShapeC)]
     L.m3d_operator_set_shape.argtypes = [vp, i, i, p(LaunchShapeC)]
@@ -470,6 +471,13 @@ class DeviceOperator:
         om = np.ascontiguousarray(omegas, dtype=np.float64)
         _check_dev(lib().m3d_sensitivity_device(self.handle, om.shape[0], _ptr(om), C.c_void_p(d_e),
                                                 C.c_void_p(d_lambda), C.c_void_p(d_grad)))
+
+    def sensitivity_device_noalloc(self, omegas, d_e: int, d_lambda: int, d_grad: int) -> None:
+        """sensitivity_device with no device allocation: the frequencies are
+        staged in the library's static per-GPU array (at most 4096)."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        _check_dev(lib().m3d_sensitivity_device_noalloc(self.handle, om.shape[0], _ptr(om), C.c_void_p(d_e),
+                                                        C.c_void_p(d_lambda), C.c_void_p(d_grad)))
 
     def spmv(self, omegas, x) -> np.ndarray:
         x = _as_c128(np.atleast_2d(x))
