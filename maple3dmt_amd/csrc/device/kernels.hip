@@ -117,6 +117,163 @@ __device__ inline void row_gather(const DevOp<V>& op, int off, int w, int lane, 
     gather_block<8, 1, UP, V>(op, off, w, lane, 0, x, NS, sys, ar, ai);
 }
 
+// Wide batches (NS >= 8): a wavefront holds R = 64 / NS consecutive rows of
+// one slice, and entry k of those R rows is R consecutive words of a SELL
+// block.  They are loaded once per wavefront at uniform addresses (through
+// the scalar cache) and each lane picks its row's, so a coupling costs the
+// wavefront one vector load, the x gather, instead of three (column, value
+// and x; the address unit takes the same time for a vector load whatever few
+// distinct words it returns).  The values of a packed block come from a
+// copy of the value table in LDS.  wrow = the wavefront's first row within
+// the slice, r = the lane's row within the wavefront.  Entries are added in
+// the order of gather_sell / gather_packed, so the sums are the same bit for
+// bit; entries past the width load nothing and add 0 * 0.
+template <int NS>
+__host__ __device__ constexpr bool wave_gather_width() { return NS >= 8 && NS <= 32; }
+
+template <int U, int NS, typename V>
+__device__ inline void gather_wave_packed(const unsigned* __restrict__ pk, const V* __restrict__ ltab, int cbits,
+                                          int off, int w, int wrow, int r, const cvec<V>* __restrict__ x,
+                                          int sys, V& ar, V& ai) {
+    constexpr int R = 64 / NS;
+    off = __builtin_amdgcn_readfirstlane(off);
+    w = __builtin_amdgcn_readfirstlane(w);
+    wrow = __builtin_amdgcn_readfirstlane(wrow);
+    const unsigned cm = (1u << cbits) - 1u;
+    const unsigned* __restrict__ base = pk + off + wrow;
+    for (int k = 0; k < w; k += U) {
+        unsigned wd[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + u;
+            unsigned sel = 0u;
+            if (kk < w) {
+                const unsigned* __restrict__ q = base + kk * kSlice;
+                sel = q[0];
+#pragma unroll
+                for (int j = 1; j < R; ++j) {
+                    const unsigned v = q[j];
+                    sel = r == j ? v : sel;
+                }
+            }
+            wd[u] = sel;
+        }
+        cvec<V> xv[U];
+        V a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xv[u].x = V(0);
+            xv[u].y = V(0);
+            a[u] = V(0);
+            if (k + u < w) {
+                xv[u] = x[size_t(wd[u] & cm) * NS + sys];
+                a[u] = ltab[wd[u] >> cbits];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ar += a[u] * xv[u].x;
+            ai += a[u] * xv[u].y;
+        }
+    }
+}
+
+template <int U, int NS, typename V>
+__device__ inline void gather_wave_sell(const V* __restrict__ val, const int* __restrict__ col, int off, int w,
+                                        int wrow, int r, const cvec<V>* __restrict__ x, int sys, V& ar,
+                                        V& ai) {
+    constexpr int R = 64 / NS;
+    off = __builtin_amdgcn_readfirstlane(off);
+    w = __builtin_amdgcn_readfirstlane(w);
+    wrow = __builtin_amdgcn_readfirstlane(wrow);
+    for (int k = 0; k < w; k += U) {
+        int c[U];
+        V a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + u;
+            int cs = 0;
+            V as = V(0);
+            if (kk < w) {
+                const int e = off + kk * kSlice + wrow;
+                cs = col[e];
+                as = val[e];
+#pragma unroll
+                for (int j = 1; j < R; ++j) {
+                    const int cj = col[e + j];
+                    const V aj = val[e + j];
+                    cs = r == j ? cj : cs;
+                    as = r == j ? aj : as;
+                }
+            }
+            c[u] = cs;
+            a[u] = as;
+        }
+        cvec<V> xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xv[u].x = V(0);
+            xv[u].y = V(0);
+            if (k + u < w) xv[u] = x[size_t(c[u]) * NS + sys];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ar += a[u] * xv[u].x;
+            ai += a[u] * xv[u].y;
+        }
+    }
+}
+
+// Batch of entries per gather step: the scalar registers one step holds
+// (R words per packed entry, R columns and R values per separate entry) kept
+// near 32-48.
+template <int NS, bool PACKED>
+__host__ __device__ constexpr int wave_unroll() {
+    return PACKED ? (NS >= 16 ? 8 : 4) : (NS >= 32 ? 8 : NS >= 16 ? 4 : 2);
+}
+
+template <bool UP, int NS, typename V>
+__device__ inline void gather_wave_block(const DevOp<V>& op, const V* __restrict__ ltab, int off, int w, int wrow,
+                                         int r, const cvec<V>* __restrict__ x, int sys, V& ar, V& ai) {
+    if (ltab)
+        gather_wave_packed<wave_unroll<NS, true>(), NS, V>(UP ? op.u_pk : op.l_pk, ltab, op.cbits, off, w, wrow, r,
+                                                           x, sys, ar, ai);
+    else
+        gather_wave_sell<wave_unroll<NS, false>(), NS, V>(UP ? op.u_val : op.l_val, UP ? op.u_col : op.l_col, off,
+                                                          w, wrow, r, x, sys, ar, ai);
+}
+
+// (A x)_row of apply_row by wavefront-shared couplings; ltab = the LDS copy
+// of the value table when this width gathers from the packed blocks.
+template <int NS, typename V>
+__device__ inline cvec<V> apply_row_wave(const DevOp<V>& op, const V* __restrict__ ltab, int row, int sys, V w,
+                                         const cvec<V>* __restrict__ x) {
+    const int4 sc = op.slc[row >> 6];
+    const int r = (threadIdx.x & 63) / NS;
+    const int wrow = (row & 63) - r;
+    V ar = 0, ai = 0;
+    gather_wave_block<false, NS, V>(op, ltab, sc.x, sc.y, wrow, r, x, sys, ar, ai);
+    gather_wave_block<true, NS, V>(op, ltab, sc.z, sc.w, wrow, r, x, sys, ar, ai);
+    const V dr = op.diag_s[row], di = w * op.me[row];
+    const cvec<V> xi = x[size_t(row) * NS + sys];
+    cvec<V> y;
+    y.x = ar + (dr * xi.x - di * xi.y);
+    y.y = ai + (dr * xi.y + di * xi.x);
+    return y;
+}
+
+// The value table of a packed operator copied into LDS by the whole block
+// (all threads call it); null when this width gathers from the separate
+// blocks.  The launch gives the block ntab * sizeof(V) bytes of dynamic LDS.
+template <int NS, typename V>
+__device__ inline const V* stage_value_table(const DevOp<V>& op, unsigned char* lds) {
+    if (!(op.vtab && op.shape[width_index(NS)].packed)) return nullptr;
+    V* t = reinterpret_cast<V*>(lds);
+    for (int i = threadIdx.x; i < op.ntab; i += blockDim.x) t[i] = op.vtab[i];
+    __syncthreads();
+    return t;
+}
+
 // Sweeps may split each row over SUB lanes (entries k = sub, sub + SUB, ...;
 // lanes of one row are NS apart), so that one colour of ~1e5 rows still puts
 // enough wavefronts on the GPU to hide gather latency at narrow batch widths.
@@ -632,13 +789,16 @@ __global__ __launch_bounds__(kFinBlock) void k_bicg_xr_fin(SysState* st, const d
 // Plain operator application and residuals.
 // ---------------------------------------------------------------------------
 
-// y = A x, or y = b - A x when b != nullptr.
-template <int NS, typename V>
+// y = A x, or y = b - A x when b != nullptr.  WAVE: couplings shared by
+// the wavefront (wide batches, apply_row_wave).
+template <int NS, typename V, bool WAVE>
 __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
                        const cvec<V>* __restrict__ x, const cvec<V>* __restrict__ b,
                        cvec<V>* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char k_spmv_lds[];
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
+    const V* ltab = WAVE ? stage_value_table<NS, V>(op, k_spmv_lds) : nullptr;
     const V w = V(omega[sys]);
     const int nrb = (op.n + rpb - 1) / rpb;
     const int* __restrict__ order = op.order[width_index(NS)];
@@ -648,7 +808,7 @@ __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
         const size_t i = size_t(row) * NS + sys;
-        cvec<V> a = apply_row(op, row, NS, sys, w, x);
+        cvec<V> a = WAVE ? apply_row_wave<NS, V>(op, ltab, row, sys, w, x) : apply_row(op, row, NS, sys, w, x);
         if (b) {
             const cvec<V> bi = b[i];
             a.x = bi.x - a.x;
@@ -1479,8 +1639,14 @@ void Launch<NS, V>::bicg_xr(hipStream_t s, int n, cvec<V>* x, const cvec<V>* ph,
 template <int NS, typename V>
 void Launch<NS, V>::spmv(hipStream_t s, const DevOp<V>& op, const double* omega, const cvec<V>* x,
                          const cvec<V>* b, cvec<V>* y) {
-    hipLaunchKernelGGL((k_spmv<NS, V>), dim3(row_grid<NS>(op.n, kStreamCap)), dim3(kBlock), 0, s,
-                       op, omega, x, b, y);
+    if (wave_gather_width<NS>() && op.shape[width_index(NS)].wave) {
+        const bool pk = op.vtab && op.shape[width_index(NS)].packed;
+        hipLaunchKernelGGL((k_spmv<NS, V, wave_gather_width<NS>()>), dim3(row_grid<NS>(op.n, kStreamCap)),
+                           dim3(kBlock), pk ? op.ntab * sizeof(V) : 0, s, op, omega, x, b, y);
+    } else {
+        hipLaunchKernelGGL((k_spmv<NS, V, false>), dim3(row_grid<NS>(op.n, kStreamCap)), dim3(kBlock), 0, s,
+                           op, omega, x, b, y);
+    }
 }
 template <int NS, typename V>
 void Launch<NS, V>::fill_zero(hipStream_t s, size_t count, cvec<V>* x) {

@@ -311,7 +311,7 @@ DevOp<double> m3d_operator::view<double>() const {
     for (int i = 0; i < kWidths; ++i) o.shape[i] = shape[0][i];
     o.slc = slc.p;
     o.l_val = l_val.p; o.l_col = l_col.p; o.u_val = u_val.p; o.u_col = u_col.p;
-    o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab.p : nullptr; o.cbits = cbits;
+    o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab.p : nullptr; o.cbits = cbits; o.ntab = ntab;
     o.diag_s = diag_s.p; o.me = me.p;
     for (int i = 0; i < kWidths; ++i) o.order[i] = order[i].p;
     return o;
@@ -323,7 +323,7 @@ DevOp<float> m3d_operator::view<float>() const {
     for (int i = 0; i < kWidths; ++i) o.shape[i] = shape[1][i];
     o.slc = slc.p;
     o.l_val = l_val32.p; o.l_col = l_col.p; o.u_val = u_val32.p; o.u_col = u_col.p;
-    o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab32.p : nullptr; o.cbits = cbits;
+    o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab32.p : nullptr; o.cbits = cbits; o.ntab = ntab;
     o.diag_s = diag_s32.p; o.me = me32.p;
     for (int i = 0; i < kWidths; ++i) o.order[i] = order[i].p;
     return o;
@@ -1148,6 +1148,10 @@ LaunchShape default_shape(int width) {
     // BiCGStab iteration -12 % at 1 system, -5 % at 2 on a 1.45 M-edge mesh);
     // at 16 systems the separate blocks were 3 % faster.
     sh.packed = width <= 4 ? 1 : 0;
+    // Wavefront-shared coupling loads measured 10-17 % slower than per-lane
+    // loads for the SpMV at 8, 16 and 32 systems on a 1.45 M-edge mesh, so
+    // they are off by default; the autotuner still tries them from 8.
+    sh.wave = 0;
     return sh;
 }
 
@@ -2068,7 +2072,8 @@ bool valid_shape(int width, const LaunchShape& sh) {
     const int ch = sh.sweep_chunk;
     return sub_ok && sh.sweep_cap >= kXcds && sh.sweep_cap <= 65536 && sh.sweep_cap % kXcds == 0 &&
            sh.row_cap >= kXcds && sh.row_cap <= kMaxBlocks && sh.row_cap % kXcds == 0 &&
-           (ch == 1 || ch == 2 || ch == 4 || ch == 8 || ch == 16) && (sh.packed == 0 || sh.packed == 1);
+           (ch == 1 || ch == 2 || ch == 4 || ch == 8 || ch == 16) && (sh.packed == 0 || sh.packed == 1) &&
+           (sh.wave == 0 || (sh.wave == 1 && width >= 8));
 }
 
 int width_slot(int width) {
@@ -2147,6 +2152,11 @@ double tune_one(m3d_operator& op, int p, int slot, const double* omega, int reps
     if (op.ntab) {
         LaunchShape c = best;
         c.packed = 1 - c.packed;
+        consider(c);
+    }
+    if ((1 << slot) >= 8) {
+        LaunchShape c = best;
+        c.wave = 1 - c.wave;
         consider(c);
     }
     op.shape[p][slot] = best;
@@ -2240,6 +2250,7 @@ int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launc
         out->row_cap = sh.row_cap;
         out->sweep_chunk = sh.sweep_chunk;
         out->packed = sh.packed;
+        out->wave = sh.wave;
     });
 }
 
@@ -2254,6 +2265,7 @@ int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d
         sh.row_cap = shape->row_cap;
         sh.sweep_chunk = shape->sweep_chunk;
         sh.packed = shape->packed;
+        sh.wave = shape->wave;
         if (!valid_shape(width, sh)) throw Fail(M3D_ERR_ARG, "invalid launch shape for this width");
         HIPCHK(hipSetDevice(op->ctx->device));
         HIPCHK(hipStreamSynchronize(op->ctx->stream));

@@ -125,6 +125,7 @@ class LaunchShapeC(C.Structure):
         ("row_cap", C.c_int),
         ("sweep_chunk", C.c_int),
         ("packed", C.c_int),
+        ("wave", C.c_int),
     ]
 
 
@@ -431,14 +432,15 @@ class DeviceOperator:
         sh = LaunchShapeC()
         _check_dev(lib().m3d_operator_get_shape(self.handle, precision, width, C.byref(sh)))
         return {"sweep_sub": sh.sweep_sub, "sweep_cap": sh.sweep_cap, "row_cap": sh.row_cap,
-                "sweep_chunk": sh.sweep_chunk, "packed": sh.packed}
+                "sweep_chunk": sh.sweep_chunk, "packed": sh.packed, "wave": sh.wave}
 
     def set_shape(self, precision: int, width: int, sweep_sub: int, sweep_cap: int, row_cap: int,
-                  sweep_chunk: int = 1, packed: Optional[int] = None) -> None:
-        """packed=None keeps the width's current layout choice."""
-        if packed is None:
-            packed = self.get_shape(precision, width)["packed"]
-        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap, sweep_chunk, packed)
+                  sweep_chunk: int = 1, packed: Optional[int] = None, wave: Optional[int] = None) -> None:
+        """packed=None / wave=None keep the width's current choice."""
+        cur = self.get_shape(precision, width)
+        packed = cur["packed"] if packed is None else packed
+        wave = cur["wave"] if wave is None else wave
+        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap, sweep_chunk, packed, wave)
         _check_dev(lib().m3d_operator_set_shape(self.handle, precision, width, C.byref(sh)))
 
     def autotune(self, omegas, widths=(1, 2, 4, 8, 16), reps: int = 16) -> dict:

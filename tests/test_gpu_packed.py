@@ -87,6 +87,34 @@ def test_spmv_and_ssor_bitwise(pair, all_packed, k):
     assert np.array_equal(mp.device.ssor_apply(om, x), mu.device.ssor_apply(om, x))
 
 
+@pytest.mark.parametrize("k", [8, 13, 16, 32])
+def test_wavefront_shared_couplings_bitwise(pair, k):
+    """From 8 systems the SpMV loads each wavefront's couplings once and
+    shares them across its lanes (launch shape wave = 1); the sums are the
+    per-lane gather's bit for bit, from either block layout, and equal the
+    twin operator built without packed blocks."""
+    wl, mp, mu = pair
+    rng = np.random.default_rng(100 + k)
+    om = 2 * np.pi * 10.0 ** rng.uniform(-3, 2, k)
+    x = rng.standard_normal((k, mp.n)) + 1j * rng.standard_normal((k, mp.n))
+    w = 8 if k <= 8 else 16 if k <= 16 else 32
+    dev = mp.device
+    base = dev.get_shape(0, w)
+    outs = {}
+    try:
+        for packed in (0, 1):
+            for wave in (0, 1):
+                dev.set_shape(0, w, base["sweep_sub"], base["sweep_cap"], base["row_cap"], base["sweep_chunk"],
+                              packed, wave)
+                outs[(packed, wave)] = dev.spmv(om, x)
+    finally:
+        dev.set_shape(0, w, **base)
+    ref = outs[(0, 0)]
+    for key, y in outs.items():
+        assert np.array_equal(y, ref), key
+    assert np.array_equal(mu.device.spmv(om, x), ref)
+
+
 @pytest.mark.parametrize("precision", [0, PREC_MIXED])
 def test_solve_bitwise(pair, all_packed, precision):
     wl, mp, mu = pair
