@@ -18,6 +18,16 @@ set_solver_tolerance_override and interleaved DivCorr (solve_rhs_complex /
 adjoint_solve_complex, :914-940).  The regularisation term lambda W^T W x of
 Mult belongs to the inversion driver, which is not part of this path.
 
+Station H.  delta Z and the adjoint right-hand side divide by the station
+H of the background field.  The reference's Mult never refreshes its station
+cache between frequencies (only an impedance extraction does, and the
+background pass ends on the last frequency, implicit_jtj_fv.cpp:37-55,
+117-119), so every frequency uses the H of the last one; h_cache="reference"
+(the default) does the same, h_cache="per_frequency" uses each frequency's
+own H.  The reference's inner solves also run on the operator as
+factorize_frequency leaves it, before the Dirichlet / dead-edge elimination;
+here they run on the eliminated operator, the only one the device keeps.
+
 Instead of the reference's loop over frequencies and polarisations, every
 perturbation system runs in one device batch and every adjoint system in a
 second.  The fields stay on the device (the perturbation right-hand sides
@@ -38,7 +48,10 @@ class GaussNewton:
     in the constructor)."""
 
     def __init__(self, model: Model, freqs, z_err: np.ndarray, params: Optional[SolveParams] = None,
-                 inner_tol: float = 1e-4):
+                 inner_tol: float = 1e-4, h_cache: str = "reference"):
+        if h_cache not in ("reference", "per_frequency"):
+            raise ValueError("h_cache is 'reference' or 'per_frequency'")
+        self.h_cache = h_cache
         import torch
         self.torch = torch
         self.dev = torch.device("cuda:0")
@@ -75,14 +88,19 @@ class GaussNewton:
                                              d_de.data_ptr(), len(self.omegas), self.inner)
         return d_de, res
 
+    def _station_h(self, f):
+        """Load the station H cache used at frequency f: that of the last
+        background frequency (h_cache "reference") or of f itself."""
+        g = len(self.freqs) - 1 if self.h_cache == "reference" else f
+        self.model.extract_impedance_compact(self.omegas[2 * g], self.ebg_c[2 * g], self.ebg_c[2 * g + 1])
+
     def _delta_z(self, dec):
         """[nf, 2, stations, 4] delta Z per polarisation from compact delta E."""
         m = self.model
         nf = len(self.freqs)
         out = np.zeros((nf, 2, m.count("stations"), 4), np.complex128)
         for f in range(nf):
-            w = self.omegas[2 * f]
-            m.extract_impedance_compact(w, self.ebg_c[2 * f], self.ebg_c[2 * f + 1])   # H cache of f
+            self._station_h(f)
             for p in range(2):
                 out[f, p] = m.delta_impedance_compact(dec[2 * f + p], p + 1)
         return out
@@ -104,7 +122,7 @@ class GaussNewton:
         arhs = np.zeros((2 * nf, k), np.complex128)
         for f in range(nf):
             w = self.omegas[2 * f]
-            m.extract_impedance_compact(w, self.ebg_c[2 * f], self.ebg_c[2 * f + 1])
+            self._station_h(f)
             for p in range(2):
                 r1, r2 = m.adjoint_rhs_compact(w, self.w2[f] * dz[f, p])
                 arhs[2 * f + p] = r1 if p == 0 else r2
@@ -120,7 +138,7 @@ class GaussNewton:
 
 
 def matvec_host(model: Model, freqs, z_err, x, params: Optional[SolveParams] = None,
-                inner_tol: float = 1e-4) -> np.ndarray:
+                inner_tol: float = 1e-4, h_cache: str = "reference") -> np.ndarray:
     """The same product in the reference's order — per frequency and
     polarisation, host buffers, one system at a time — for checking the
     batched operator."""
@@ -134,9 +152,11 @@ def matvec_host(model: Model, freqs, z_err, x, params: Optional[SolveParams] = N
     ds = sigma * np.asarray(x, dtype=np.float64)
     idx = model.station_edges()
     y = np.zeros(model.nc)
+    last = len(freqs) - 1
     for f, fr in enumerate(freqs):
         w = 2 * np.pi * fr
-        model.extract_impedance(w, E[2 * f], E[2 * f + 1])
+        g = last if h_cache == "reference" else f
+        model.extract_impedance(2 * np.pi * freqs[g], E[2 * g], E[2 * g + 1])
         for p in range(2):
             prhs = model.perturbation_rhs(w, E[2 * f + p], ds)
             de, _ = model.device.solve(MODE_INTERLEAVED, [w], prhs[None, :], params=inner)

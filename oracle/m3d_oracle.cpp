@@ -832,6 +832,53 @@ public:
         }
     }
 
+    // build_perturbation_rhs (:945-984): delta b = -i w sum_c dMe/dsigma_c
+    // delta_sigma_c E_bg with the reference's harmonic-mean, L^3 mass
+    // derivative, cells in order, each cell's edges in cell-edge order.
+    void perturbation_rhs(const ComplexVec& Eb, const RealVec& dsig, ComplexVec& rhs) const {
+        rhs.assign(ne, Complex(0, 0));
+        const Complex iw(0, w);
+        for (int c = 0; c < nc; ++c) {
+            if (std::abs(dsig[c]) < 1e-30) continue;
+            const Real sc = sigma[c];
+            for (int e : cedges[c]) {
+                const Real L = el[e], vol = L * L * L;
+                Real sum_inv = 0.0;
+                int na = 0;
+                for (int a : adj[e])
+                    if (a >= 0 && a < nc && sigma[a] > 1e-30) { sum_inv += 1.0 / sigma[a]; ++na; }
+                const Real savg = (na > 0 && sum_inv > 1e-30) ? na / sum_inv : 1e-8;
+                const Real dme = vol * savg * savg / (na * sc * sc);
+                rhs[e] -= iw * Complex(dsig[c] * dme, 0) * Eb[e];
+            }
+        }
+    }
+
+    // extract_delta_impedance_complex (:2018-2065): station averages of the
+    // x and y components of dE over all the cell's x / y edges, times the
+    // column of H^-1 of the polarisation (1 or 2), H from the cache of the
+    // last impedance call; delta H is not formed.
+    void delta_impedance(const ComplexVec& dE, int pol, std::vector<std::array<Complex, 4>>& dz) const {
+        dz.assign(station_cell.size(), {Complex(0, 0), Complex(0, 0), Complex(0, 0), Complex(0, 0)});
+        for (size_t s = 0; s < station_cell.size(); ++s) {
+            const int c = station_cell[s];
+            if (c < 0) continue;
+            Complex dx(0, 0), dy(0, 0);
+            int nx = 0, ny = 0;
+            for (int e : cedges[c]) {
+                if (ed[e] == 0) { dx += dE[e]; ++nx; }
+                else if (ed[e] == 1) { dy += dE[e]; ++ny; }
+            }
+            if (nx > 0) dx /= Complex(nx, 0);
+            if (ny > 0) dy /= Complex(ny, 0);
+            const HC& h = hcache[s];
+            if (std::abs(h.det) < 1e-30) continue;
+            const Complex inv = Complex(1, 0) / h.det;
+            if (pol == 1) dz[s] = {dx * h.Hy2 * inv, dx * (-h.Hx2) * inv, dy * h.Hy2 * inv, dy * (-h.Hx2) * inv};
+            else dz[s] = {dx * (-h.Hy1) * inv, dx * h.Hx1 * inv, dy * (-h.Hy1) * inv, dy * h.Hx1 * inv};
+        }
+    }
+
     orc_params prm{};
     int ne = 0, nf = 0, nc = 0, nn = 0;
     std::vector<double> dom;
@@ -1038,6 +1085,29 @@ int orc_adjoint_rhs(void* h, const double* wr, double* r1, double* r2) {
         o->adjoint_rhs(w, a, b);
         std::memcpy(r1, a.data(), a.size() * sizeof(Complex));
         std::memcpy(r2, b.data(), b.size() * sizeof(Complex));
+    });
+}
+
+int orc_perturbation_rhs(void* h, const double* ebg, const double* dsigma, double* rhs) {
+    Oracle* o = static_cast<Oracle*>(h);
+    return guard([&] {
+        ComplexVec a(cptr(ebg), cptr(ebg) + o->ne), r;
+        RealVec d(dsigma, dsigma + o->nc);
+        o->perturbation_rhs(a, d, r);
+        std::copy(r.begin(), r.end(), reinterpret_cast<Complex*>(rhs));
+    });
+}
+
+int orc_delta_impedance(void* h, const double* de, int pol, double* dz) {
+    Oracle* o = static_cast<Oracle*>(h);
+    return guard([&] {
+        if (pol != 1 && pol != 2) throw std::invalid_argument("polarisation is 1 or 2");
+        ComplexVec a(cptr(de), cptr(de) + o->ne);
+        std::vector<std::array<Complex, 4>> z;
+        o->delta_impedance(a, pol, z);
+        Complex* out = reinterpret_cast<Complex*>(dz);
+        for (size_t s = 0; s < z.size(); ++s)
+            for (int k = 0; k < 4; ++k) out[4 * s + k] = z[s][k];
     });
 }
 

@@ -93,6 +93,8 @@ def lib():
         L.orc_impedance.argtypes = [vp, vp, vp, vp]
         L.orc_adjoint_rhs.argtypes = [vp, vp, vp, vp]
         L.orc_sensitivity.argtypes = [vp, vp, vp, vp]
+        L.orc_perturbation_rhs.argtypes = [vp, vp, vp, vp]
+        L.orc_delta_impedance.argtypes = [vp, vp, C.c_int, vp]
         _lib = L
     return _lib
 
@@ -246,6 +248,22 @@ class Oracle:
         _chk(lib().orc_sensitivity(self.handle, _p(a), _p(b), _p(g)))
         return g
 
+    def perturbation_rhs(self, e_bg, dsigma):
+        """build_perturbation_rhs at the assembled frequency (dsigma in sigma space)."""
+        a = np.ascontiguousarray(e_bg, np.complex128)
+        d = np.ascontiguousarray(dsigma, np.float64)
+        r = np.zeros(self.ne, np.complex128)
+        _chk(lib().orc_perturbation_rhs(self.handle, _p(a), _p(d), _p(r)))
+        return r
+
+    def delta_impedance(self, de, pol):
+        """extract_delta_impedance_complex (pol 1 or 2) with the H cache of the
+        last impedance call."""
+        a = np.ascontiguousarray(de, np.complex128)
+        z = np.zeros((self.ns, 4), np.complex128)
+        _chk(lib().orc_delta_impedance(self.handle, _p(a), int(pol), _p(z)))
+        return z
+
     def solve_frequency(self, freq):
         """Forward solve of both polarisations + impedance, as solve_frequency."""
         self.assemble(2 * np.pi * freq)
@@ -253,6 +271,59 @@ class Oracle:
         e2, r2 = self.solve_pol(1)
         z = self.impedance(e1, e2)
         return e1, e2, z, (r1, r2)
+
+
+def jtj_matvec(model, freqs, z_err, x, inner_tol=1e-4, stale_h=False, fwd=None, sigma_bg=None,
+               stations=None):
+    """y = J^T W^2 J x (x = delta ln sigma per cell) by the loop of
+    ImplicitJtJOperatorFV::Mult (implicit_jtj_fv.cpp:78-192) on the oracle:
+    background fields per frequency (compute_single_frequency), then per
+    frequency and polarisation p = 1, 2: the perturbation right-hand side with
+    E_bg of p, the interleaved solve at inner_tol, delta Z of p, W^2 delta Z
+    (w = 1 / error, data with error <= 0 dropped), the adjoint right-hand
+    side, the adjoint solve of p's right-hand side at inner_tol, and the
+    sensitivity with E_bg of p.  The station H of each frequency comes from
+    that frequency's background fields; stale_h=True keeps instead the H of
+    the last background frequency for every frequency, as the reference's
+    loop does (its station cache is refreshed only by an impedance
+    extraction, implicit_jtj_fv.cpp:37-55, 117-119).  Also returns delta Z
+    [nf, 2, stations, 4].  The regularisation term of Mult is not formed."""
+    from types import SimpleNamespace
+    fwd = fwd or model.fwd_params
+    inner = SimpleNamespace(**{**fwd.__dict__, "bicgstab_tol": inner_tol})
+    o = Oracle(model, fwd=fwd, sigma_bg=sigma_bg)
+    oi = Oracle(model, fwd=inner, sigma_bg=sigma_bg)
+    sig = model.array("sigma")
+    for q in (o, oi):
+        q.set_sigma(sig)
+        q.find_stations(stations)
+    err = np.asarray(z_err)
+    w2 = np.where(err > 0, 1.0 / np.where(err > 0, err, 1.0), 0.0) ** 2
+    ds = sig * np.asarray(x, np.float64)
+    bg = [o.solve_frequency(f)[:2] for f in freqs]
+    y = np.zeros(model.nc)
+    dz_all = np.zeros((len(freqs), 2, len(stations), 4), np.complex128)
+    last = len(freqs) - 1
+    for fi, f in enumerate(freqs):
+        oi.assemble(2 * np.pi * f)
+        # H cache: this frequency's background, or the last one's.
+        hf = last if stale_h else fi
+        if hf == fi:
+            oi.impedance(*bg[fi])
+        else:
+            oi.assemble(2 * np.pi * freqs[hf])
+            oi.impedance(*bg[hf])
+            oi.assemble(2 * np.pi * f)
+        for p in (1, 2):
+            eb = bg[fi][p - 1]
+            prhs = oi.perturbation_rhs(eb, ds)
+            de, _ = oi.adjoint(prhs)
+            dz = oi.delta_impedance(de, p)
+            dz_all[fi, p - 1] = dz
+            r1, r2 = oi.adjoint_rhs(w2[fi] * dz)
+            lam, _ = oi.adjoint(r1 if p == 1 else r2)
+            y += oi.sensitivity(eb, lam)
+    return y, dz_all
 
 
 def bicgstab_csr(ptr, col, val, b, x0=None, tol=1e-7, maxiter=2000, ssor=True):

@@ -337,3 +337,78 @@ def test_impedance_exact_on_discrete_1d_solution():
     # halfspace value sqrt(i w mu0 / sigma) (|kh| ~ 1.4 here).
     zh = np.sqrt(1j * w * MU0 / 0.01)
     assert abs(zc / zh - 1) < 0.3
+
+
+def test_perturbation_rhs_pins():
+    """build_perturbation_rhs (forward_solver_fv.cpp:945-984) in the oracle:
+    bit for bit the product's host restatement on a heterogeneous model, and,
+    in a uniform medium on a uniform mesh, where the reference's mass
+    derivative L^3 savg^2 / (n_adj sigma_c^2) is the true derivative of the
+    edge mass, -i w (dMe/dsigma . dsigma) E with the derivative taken by
+    central differences of the product's Me."""
+    wl = configs.random3d(8, min_level=3, max_level=4, n_freq=1, half_width=50e3, station_radius=5e3)
+    m = configs.build_model(wl)
+    o = Oracle(m, sigma_bg=wl.sigma_bg)
+    o.set_sigma(m.array("sigma"))
+    w = 2 * np.pi * 0.7
+    o.assemble(w)
+    rng = np.random.default_rng(12)
+    e = _cvec(rng, m.n)
+    ds = rng.standard_normal(m.nc) * m.array("sigma")
+    ds[rng.uniform(size=m.nc) < 0.3] = 0.0
+    assert np.array_equal(o.perturbation_rhs(e, ds), m.perturbation_rhs(w, e, ds))
+
+    mp = MeshParams(domain=(-8e3, 8e3, -8e3, 8e3, -8e3, 8e3), min_level=3, max_level=3, sigma_bg=0.01)
+    mu = Model(mp, np.zeros((0, 3)), [1.0])
+    s0 = np.full(mu.nc, 0.01)
+    mu.set_sigma(s0)
+    ou = Oracle(mu)
+    ou.set_sigma(s0)
+    ou.assemble(w)
+    eu = _cvec(rng, mu.n)
+    dsu = 0.01 * rng.uniform(0.5, 1.5, mu.nc)
+    eps = 1e-3
+    mu.set_sigma(s0 + eps * dsu)
+    me_p = mu.array("me")
+    mu.set_sigma(s0 - eps * dsu)
+    me_m = mu.array("me")
+    dme = (me_p - me_m) / (2 * eps)
+    fd = -1j * w * dme * eu
+    got = ou.perturbation_rhs(eu, dsu)
+    # Edges whose adjacent cells are all inside the mesh (four of them) and
+    # that carry mass.
+    adj = np.diff(mu.array("edge_adj_ptr"))
+    inner = (adj == 4) & (np.abs(dme) > 0)
+    assert inner.sum() > 100
+    assert np.abs(got[inner] - fd[inner]).max() <= 1e-9 * np.abs(fd[inner]).max()
+
+
+def test_delta_impedance_columns_sum_to_impedance():
+    """extract_delta_impedance_complex (:2018-2065) gives the column of
+    Z = E H^-1 that one polarisation's field carries, with H held fixed: on a
+    uniform mesh, whose station cells average as many top as bottom x / y
+    edges, delta Z(E1, pol 1) + delta Z(E2, pol 2) is Z(E1, E2) of the
+    impedance extraction, for any fields.  The product's compact form agrees."""
+    mp = MeshParams(domain=(-50e3, 50e3, -50e3, 50e3, -50e3, 50e3), min_level=3, max_level=3,
+                    sigma_bg=0.01)
+    st = configs.station_grid(3, 2000.0)
+    m = Model(mp, st, [0.1])
+    sig = np.where(m.array("cell_cz") < 0, 0.01, 1e-8)
+    m.set_sigma(sig)
+    m.find_stations(st[:, :2])
+    o = Oracle(m)
+    o.set_sigma(sig)
+    o.find_stations(st[:, :2])
+    w = 2 * np.pi * 0.1
+    o.assemble(w)
+    rng = np.random.default_rng(13)
+    e1, e2 = _cvec(rng, m.n), _cvec(rng, m.n)
+    z = o.impedance(e1, e2)
+    d1, d2 = o.delta_impedance(e1, 1), o.delta_impedance(e2, 2)
+    assert np.abs(d1 + d2 - z).max() <= 1e-12 * np.abs(z).max()
+    # Columns: pol 1 alone is Z of (E1, 0) with H of (E1, E2) held.
+    assert np.abs(d1).max() > 0 and np.abs(d2).max() > 0
+    idx = m.station_edges()
+    m.extract_impedance_compact(w, e1[idx], e2[idx])
+    for p, (d, e) in enumerate(((d1, e1), (d2, e2))):
+        assert np.abs(m.delta_impedance_compact(e[idx], p + 1) - d).max() <= 1e-13 * np.abs(d).max()
