@@ -94,38 +94,53 @@ def features(logsig: np.ndarray, freqs) -> np.ndarray:
     return x
 
 
-def deposit(model: Model, box: Box, e_sec_pol1: np.ndarray, e_sec_pol2: np.ndarray) -> np.ndarray:
-    """Trilinear deposition of edge values onto the voxel centres ([G^3, 12]:
-    channel pol*6 + 2*dir + {re, im})."""
+def deposit_weights(model: Model, box: Box):
+    """Trilinear deposition of a model's edges onto the box's voxel centres:
+    per corner c of the 8, the voxel index and weight of every edge inside
+    the box, with the edges' direction and indices (the geometry part of
+    deposit, shared by every field of the model)."""
     ex, ey, ez = model.array("edge_x"), model.array("edge_y"), model.array("edge_z")
     d = model.array("edge_dir").astype(np.int64)
     hx, hy, hz = (box.x1 - box.x0) / G, (box.y1 - box.y0) / G, (box.z1 - box.z0) / G
     fx, fy, fz = (ex - box.x0) / hx - 0.5, (ey - box.y0) / hy - 0.5, (ez - box.z0) / hz - 0.5
     inside = (fx >= 0) & (fy >= 0) & (fz >= 0) & (fx <= G - 1) & (fy <= G - 1) & (fz <= G - 1)
-    fx, fy, fz, d = fx[inside], fy[inside], fz[inside], d[inside]
-    v1, v2 = e_sec_pol1[inside], e_sec_pol2[inside]
+    sel = np.flatnonzero(inside)
+    fx, fy, fz, d = fx[sel], fy[sel], fz[sel], d[sel]
     ix = np.minimum(fx.astype(np.int64), G - 2)
     iy = np.minimum(fy.astype(np.int64), G - 2)
     iz = np.minimum(fz.astype(np.int64), G - 2)
     tx, ty, tz = fx - ix, fy - iy, fz - iz
-    acc = np.zeros((G * G * G, NCH))
-    wsum = np.zeros((G * G * G, 3))
+    vox, wts = [], []
     for c in range(8):
         cx, cy, cz = c & 1, (c >> 1) & 1, c >> 2
-        w = (tx if cx else 1 - tx) * (ty if cy else 1 - ty) * (tz if cz else 1 - tz)
-        vox = ((iz + cz) * G + iy + cy) * G + ix + cx
-        np.add.at(wsum, (vox, d), w)
-        for pol, v in ((0, v1), (1, v2)):
-            np.add.at(acc, (vox, pol * 6 + 2 * d), w * v.real)
-            np.add.at(acc, (vox, pol * 6 + 2 * d + 1), w * v.imag)
-    out = np.zeros_like(acc)
-    for pol in range(2):
-        for dd in range(3):
-            w = wsum[:, dd]
-            ok = w > 0
-            for k in range(2):
-                ch = pol * 6 + 2 * dd + k
-                out[ok, ch] = acc[ok, ch] / w[ok]
+        wts.append((tx if cx else 1 - tx) * (ty if cy else 1 - ty) * (tz if cz else 1 - tz))
+        vox.append(((iz + cz) * G + iy + cy) * G + ix + cx)
+    return sel, d, np.stack(vox), np.stack(wts)
+
+
+def deposit(model: Model, box: Box, e_sec_pol1: np.ndarray, e_sec_pol2: np.ndarray,
+            geometry=None) -> np.ndarray:
+    """Trilinear deposition of edge values onto the voxel centres ([G^3, 12]:
+    channel pol*6 + 2*dir + {re, im}): per voxel and direction, the
+    weight-averaged values of the edges of that direction around it (0 where
+    there is none).  geometry: deposit_weights of the model, when reused."""
+    sel, d, vox, wts = geometry if geometry is not None else deposit_weights(model, box)
+    n3 = G * G * G
+    out = np.zeros((n3, NCH))
+    vals = (e_sec_pol1[sel], e_sec_pol2[sel])
+    for dd in range(3):
+        m = d == dd
+        if not m.any():
+            continue
+        v_ = vox[:, m].ravel()
+        w_ = wts[:, m].ravel()
+        wsum = np.bincount(v_, weights=w_, minlength=n3)
+        ok = wsum > 0
+        for pol in range(2):
+            v = np.tile(vals[pol][m], 8)
+            for k, part in enumerate((v.real, v.imag)):
+                acc = np.bincount(v_, weights=w_ * part, minlength=n3)
+                out[ok, pol * 6 + 2 * dd + k] = acc[ok] / wsum[ok]
     return out.astype(np.float32)
 
 
@@ -259,14 +274,33 @@ def make_sample(model: Model, wl: configs.Workload, box: Box) -> Optional[Sample
     esec, res = solve_model(model, wl)
     if not all(r["converged"] for r in res):
         return None
+    return sample_from_fields(model, wl, box, esec, res)
+
+
+def sample_from_fields(model: Model, wl: configs.Workload, box: Box, esec: np.ndarray, res) -> Sample:
+    """Training sample from a model's scattered fields [2 nf, n]."""
     logsig = sample_sigma(model, model.array("sigma"), box)
     x = features(logsig, wl.freqs)
-    y = np.stack([deposit(model, box, esec[2 * f], esec[2 * f + 1]) for f in range(len(wl.freqs))])
+    geo = deposit_weights(model, box)
+    y = np.stack([deposit(model, box, esec[2 * f], esec[2 * f + 1], geo) for f in range(len(wl.freqs))])
     return Sample(x, normalise(y), [r["iterations"] for r in res])
 
 
 def training_workload(seed: int, max_level: int = 5) -> configs.Workload:
     return configs.random3d(seed, max_level=max_level, half_width=100e3, station_radius=20e3)
+
+
+def seeded_level(seed: int, levels=(5, 6, 7), weights=(0.5, 0.35, 0.15)) -> int:
+    """Finest octree level of the model of one seed, drawn from `levels` with
+    `weights` (the same level for a seed every time)."""
+    rng = np.random.default_rng(seed + 7_777_777)
+    w = np.asarray(weights, np.float64)
+    return int(levels[int(rng.choice(len(levels), p=w / w.sum()))])
+
+
+def domain_box(wl: configs.Workload) -> Box:
+    """The voxel box over the whole mesh domain."""
+    return Box(*[float(v) for v in wl.mesh.domain])
 
 
 def train(net: CoarseNet, samples: List[Sample], steps: int, lr: float = 5e-4, batch: int = 8,

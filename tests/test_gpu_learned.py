@@ -197,3 +197,64 @@ def test_forward_initial_guess_and_fallback(ctx, torch_dev):
     assert not any(r["guess_used"] for r in r4)
     assert all(r["guess_residual"] > 0.5 for r in r4)
     assert [r["iterations"] for r in r4] == [r["iterations"] for r in r_ref]
+
+
+def test_trained_network_on_held_out_models(ctx, torch_dev):
+    """The committed network (trained on solver-generated fields of seeded
+    random models, levels 3-5 to 3-7, whole-domain voxel box) on held-out
+    seeds: the fast path from the zero start and from the network's e0 at the
+    same tolerance.  Both converge, their impedances agree (tol 1e-9, so the
+    two solutions differ by less than 1e-6), the fallback keeps the guess only
+    where its least-squares scaled residual is below 0.5 of ||b|| and starts
+    from zero elsewhere; iterations and solves/s of both starts are printed."""
+    import os
+    import time
+    from maple3dmt_amd.learned import domain_box
+    from maple3dmt_amd.train_e0 import EVAL_SEED0
+    torch, dev = torch_dev
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "maple3dmt_amd", "data",
+                        "e0_net.npz")
+    d = np.load(path)
+    net = CoarseNet(ctx, max_batch=8, dilations=tuple(int(v) for v in d["dilations"]))
+    net.set_params(d["w"], d["b"])
+    box = Box(*d["box"].tolist())
+    tot = {"zero": [0, 0.0], "e0": [0, 0.0]}
+    for seed, level in ((EVAL_SEED0 + 150, 5), (EVAL_SEED0 + 151, 6), (EVAL_SEED0 + 152, 6)):
+        wl = configs.random3d(seed, max_level=level, half_width=100e3, station_radius=20e3, maxiter=40000)
+        assert np.allclose(box.arr(), domain_box(wl).arr())
+        m = configs.build_model(wl, ctx)
+        om = np.repeat(2 * np.pi * wl.freqs, 2)
+        rhs = np.stack([m.rhs_total(m.bc_values(w, p % 2)) for p, w in enumerate(om)])
+        aux = np.stack([m.primary_field(w, p % 2) for p, w in enumerate(om)])
+        d_rhs, d_aux = torch.from_numpy(rhs).to(dev), torch.from_numpy(aux).to(dev)
+        zs = {}
+        for start in ("zero", "e0"):
+            prm = wl.fwd.fast_params(tol=1e-9)
+            t = time.perf_counter()
+            if start == "zero":
+                d_x = torch.zeros_like(d_rhs)
+            else:
+                d_x = initial_guess(net, m, wl, box)
+                prm.use_initial_guess = 2
+            torch.cuda.synchronize()
+            res = m.device.solve_device(MODE_FORWARD, om, d_rhs.data_ptr(), d_aux.data_ptr(), d_x.data_ptr(),
+                                        len(om), prm)
+            dt = time.perf_counter() - t
+            assert all(solved(r, 1e-9) for r in res), res
+            if start == "e0":
+                for r in res:
+                    assert r["guess_used"] == (r["guess_residual"] < 0.5), r
+            x = d_x.cpu().numpy()
+            zs[start] = np.stack([m.extract_impedance(om[2 * f], x[2 * f], x[2 * f + 1])
+                                  for f in range(len(wl.freqs))])
+            its = [r["iterations"] for r in res]
+            tot[start][0] += sum(its)
+            tot[start][1] += dt
+            print(f"seed {seed} level {level} {m.n} edges {start}: {sum(its)} iterations, {dt:.3f}s"
+                  + (f", guess used {sum(r['guess_used'] for r in res)}/16" if start == "e0" else ""), flush=True)
+        rel = np.abs(zs["e0"][..., 1:3] - zs["zero"][..., 1:3]) / np.abs(zs["zero"][..., 1:3])
+        assert rel.max() <= 1e-6, rel.max()
+        m.close()
+    for k, (it, sec) in tot.items():
+        print(f"{k} start: {it} iterations, {48 / sec:.1f} solves/s", flush=True)
+    net.close()
