@@ -20,6 +20,16 @@ namespace {
 template <int NS>
 __device__ inline int rows_per_block() { return kBlock / NS; }
 
+// Descriptor of the slice holding a wavefront's rows.  Every kernel maps a
+// wavefront onto consecutive rows of one 64-row slice (a wavefront covers
+// 64 / (NS * SUB) rows starting at a multiple of that, and colours and the
+// padded row count are multiples of 64), so the index is uniform: read from
+// the first lane, the load goes through the scalar cache and does not wait
+// behind the wavefront's vector loads.
+__device__ inline int4 slice_desc(const int4* __restrict__ slc, int row) {
+    return slc[__builtin_amdgcn_readfirstlane(row) >> 6];
+}
+
 // Entries k = k0, k0 + STEP, ... < w of one row of a SELL block, added into
 // (ar, ai) in increasing k.  The column indices and values of U entries are
 // loaded before any of their x entries, so a row waits for two memory round
@@ -248,7 +258,7 @@ __device__ inline void gather_wave_block(const DevOp<V>& op, const V* __restrict
 template <int NS, typename V>
 __device__ inline cvec<V> apply_row_wave(const DevOp<V>& op, const V* __restrict__ ltab, int row, int sys, V w,
                                          const cvec<V>* __restrict__ x) {
-    const int4 sc = op.slc[row >> 6];
+    const int4 sc = slice_desc(op.slc, row);
     const int r = (threadIdx.x & 63) / NS;
     const int wrow = (row & 63) - r;
     V ar = 0, ai = 0;
@@ -298,7 +308,7 @@ __device__ inline void row_gather_split(const DevOp<V>& op, int off, int w, int 
 template <typename V>
 __device__ inline cvec<V> apply_row(const DevOp<V>& op, int row, int NS, int sys, V w,
                                     const cvec<V>* __restrict__ x) {
-    const int4 sc = op.slc[row >> 6];
+    const int4 sc = slice_desc(op.slc, row);
     const int lane = row & 63;
     V ar = 0, ai = 0;
     row_gather<false>(op, sc.x, sc.y, lane, x, NS, sys, ar, ai);
@@ -317,7 +327,7 @@ __device__ inline cvec<V> apply_row(const DevOp<V>& op, int row, int NS, int sys
 template <typename V>
 __device__ inline cvec<V> apply_row_eis(const DevOp<V>& op, int row, int NS, int sys, V w,
                                         const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ y) {
-    const int4 sc = op.slc[row >> 6];
+    const int4 sc = slice_desc(op.slc, row);
     const int lane = row & 63;
     V ar = 0, ai = 0;
     row_gather<false>(op, sc.x, sc.y, lane, z, NS, sys, ar, ai);
@@ -499,7 +509,7 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
         // Every load the row needs besides the gather goes out before it.
-        const int4 sc = op.slc[row >> 6];
+        const int4 sc = slice_desc(op.slc, row);
         const V ds = op.diag_s[row], me = op.me[row];
         cvec<V> src = in[i];
         if (MODE == kInPUpdate) {
@@ -564,7 +574,7 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
         if (row >= r1) continue;
         const size_t i = size_t(row) * NS + sys;
         // Every load the row needs besides the gather goes out before it.
-        const int4 sc = op.slc[row >> 6];
+        const int4 sc = slice_desc(op.slc, row);
         const V ds = op.diag_s[row], me = op.me[row];
         const cvec<V> yi = y[i];
         const int lane = row & 63;
