@@ -64,15 +64,16 @@ TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 MFMA = os.path.join(ROOT, "profiles", "mfma_e0_network.json")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (None when it holds no entry for the kernel)."""
+def measured_traffic(kernel: str, field: str = "traffic_bytes"):
+    """HBM bytes per launch of `kernel` (or another field of its entry, e.g.
+    the rocprofv3 mean launch time) from the committed counter summary; None
+    when it holds no entry for the kernel."""
     try:
         with open(TRAFFIC) as f:
             k = json.load(f)["kernels"].get(kernel)
     except (OSError, ValueError, KeyError):
         return None
-    return None if k is None else float(k["traffic_bytes"])
+    return None if k is None or field not in k else float(k[field])
 
 
 def parse():
@@ -105,6 +106,9 @@ def parse():
     ap.add_argument("--c3", action="store_true",
                     help="separate mode: the 16-model batch (1024 forward + adjoint systems on "
                          "meshes of 0.24-2.4 M edges), one JSON line of its own")
+    ap.add_argument("--fast-only", action="store_true",
+                    help="profiling mode: build the workload, tune the launch shapes, then only solve "
+                         "the batch twice with the fast path (mixed precision + interleaved DivCorr)")
     ap.add_argument("--network-only", action="store_true",
                     help="profiling mode: only the e0 network's inference and training steps")
     ap.add_argument("--reps", type=int, default=20)
@@ -128,7 +132,18 @@ def kernel_only(args):
     ctx = Context(0)
     model = configs.build_model(wl, ctx)
     omegas = np.repeat(2 * np.pi * wl.freqs, 2)
-    ms, by = model.device.bench_kernel(0, omegas, reps=args.reps)
+    dev = model.device
+    # The 16-system SpMV in both block layouts, separate blocks first, then
+    # packed (the counter passes tell the two apart by launch order).
+    base = dev.get_shape(0, 16)
+    spmv = {}
+    for packed in (0, 1):
+        dev.set_shape(0, 16, base["sweep_sub"], base["sweep_cap"], base["row_cap"], base["sweep_chunk"], packed)
+        t, b = dev.bench_kernel(0, omegas, reps=args.reps)
+        spmv[f"packed{packed}"] = {"ms": t, "alg_bytes": b}
+        log(f"[kernel-only] SpMV x16 packed {packed}: {t:.3f} ms, {b / t / 1e6:.0f} GB/s")
+    dev.set_shape(0, 16, **base)
+    ms, by = spmv["packed0"]["ms"], spmv["packed0"]["alg_bytes"]
     ms2, by2 = model.device.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
     ms4, by4 = model.device.bench_kernel(4, omegas[:args.tail_width], reps=args.reps)
     iters = {}
@@ -142,10 +157,38 @@ def kernel_only(args):
     for _ in range(4):
         y.copy_(x)
     torch.cuda.synchronize()
-    print(json.dumps({"spmv_ms": ms, "spmv_alg_bytes": by, "ssor_ms": ms2, "ssor_alg_bytes": by2,
+    print(json.dumps({"spmv_ms": ms, "spmv_alg_bytes": by, "spmv_layouts": spmv, "ssor_ms": ms2, "ssor_alg_bytes": by2,
                       "sweep_bwd_ms": ms4, "sweep_bwd_alg_bytes": by4, "iteration": iters,
                       "copy_bytes_each_way": x.numel() * 8, "copies": 4, "edges": model.n}),
           flush=True)
+
+
+def fast_only(args):
+    """Only the fast-path solves of the bench batch (for rocprofv3 runs)."""
+    import torch
+
+    from maple3dmt_amd import MODE_FORWARD, Context, configs
+    wl = configs.random3d(args.seed, max_level=args.max_level, half_width=args.half_width,
+                          station_radius=args.station_radius, maxiter=args.maxiter)
+    ctx = Context(0)
+    model = configs.build_model(wl, ctx)
+    om = np.repeat(2 * np.pi * wl.freqs, 2)
+    if not args.no_autotune:
+        model.device.autotune(np.resize(om, 32), widths=(1, 2, 4, 8, 16), reps=16)
+    rhs = np.stack([model.rhs_total(model.bc_values(w, p % 2)) for p, w in enumerate(om)])
+    aux = np.stack([model.primary_field(w, p % 2) for p, w in enumerate(om)])
+    dev = torch.device("cuda:0")
+    d_rhs, d_aux = torch.from_numpy(rhs).to(dev), torch.from_numpy(aux).to(dev)
+    prm = wl.fwd.fast_params(dc_interval=args.dc_interval)
+    for _ in range(2):
+        d_x = torch.zeros_like(d_rhs)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = model.device.solve_device(MODE_FORWARD, om, d_rhs.data_ptr(), d_aux.data_ptr(), d_x.data_ptr(),
+                                      len(om), prm)
+        log(f"[fast-only] {time.perf_counter() - t:.3f}s, iterations {[q['iterations'] for q in r]}")
+    model.close()
+    ctx.close()
 
 
 def impedances(model, omegas, x, nf):
@@ -166,6 +209,9 @@ def main():
 
     if args.kernel_only:
         kernel_only(args)
+        return
+    if args.fast_only:
+        fast_only(args)
         return
     if args.c3:
         print(json.dumps(c3_leg(args)), flush=True)
@@ -318,6 +364,7 @@ def main():
     ach = bytes_sw / (ms_sw * 1e-3) / 1e9
     ms_spmv, bytes_spmv = op.bench_kernel(0, omegas, reps=args.reps)
     ach_spmv = bytes_spmv / (ms_spmv * 1e-3) / 1e9
+    spmv_layout = f"packed{op.get_shape(0, 16)['packed']}"
     ms_ssor, bytes_ssor = op.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
     info = op.info()
     info["device_bytes"] = op.device_bytes()
@@ -397,10 +444,17 @@ def main():
                      "kernel": f"{sweep_name} (backward SSOR sweep, one colour per launch)",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": measured_traffic(sweep_name),
-                     "alg_bytes_per_launch": bytes_sw, "avg_launch_ms": ms_sw},
-        "spmv": {"kernel": f"k_spmv<{ns},double>", "avg_launch_ms": ms_spmv, "alg_bytes": bytes_spmv,
-                 "achieved_GBs": ach_spmv, "TBs": ach_spmv / 1e3,
-                 "traffic": measured_traffic(f"k_spmv<{ns}, double>")},
+                     "alg_bytes_per_launch": bytes_sw, "avg_launch_ms": ms_sw,
+                     # The same launches in the committed kernel trace: rocprofv3
+                     # waits on every dispatch's completion signal, about 1 us more
+                     # per short launch than back to back in a graph.
+                     "rocprof_avg_launch_ms": (lambda v: None if v is None else v / 1e3)(
+                         measured_traffic(sweep_name, "rocprof_avg_launch_us"))},
+        "spmv": {"kernel": f"k_spmv<{ns}, double> {spmv_layout}", "avg_launch_ms": ms_spmv, "alg_bytes": bytes_spmv,
+                 "achieved_GBs": ach_spmv, "TBs": ach_spmv / 1e3, "frac": ach_spmv / HBM_PEAK_GBS,
+                 "traffic": measured_traffic(f"k_spmv<{ns}, double> {spmv_layout}"),
+                 "rocprof_avg_launch_ms": (lambda v: None if v is None else v / 1e3)(
+                     measured_traffic(f"k_spmv<{ns}, double> {spmv_layout}", "rocprof_avg_launch_us"))},
         "ssor_apply": {"systems": ns, "avg_ms": ms_ssor, "alg_bytes": bytes_ssor,
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
         "operator": info,
