@@ -332,17 +332,20 @@ typedef struct m3d_launch_shape {
                         m3d_operator_format) when the operator has them; 0: from
                         the separate value and column blocks.  Same values, same
                         order of summation: results are identical either way */
-    int wave;        /* widths 8-32 only (0 below): 1 = each wavefront (64 / width
-                        rows) loads its rows' couplings once, through the scalar
-                        cache, and its lanes share them (a packed block's values
-                        from an LDS copy of the value table); 0 = every lane loads
-                        its own.  Same order of summation either way */
+    int wave;        /* widths 8-32 only (0 below): 1 = the width lanes of a row
+                        load the couplings of up to width consecutive entries at
+                        once, one entry per lane, and pass them to each other
+                        through the LDS crossbar (a packed block's values from an
+                        LDS copy of the value table), in the SpMV, the BiCGStab
+                        products and the SSOR sweeps at sweep_sub 1; 0 = every
+                        lane loads each coupling itself.  Same order of summation
+                        either way */
 } m3d_launch_shape;
 
 /* width is a batch width 1, 2, 4, 8, 16 or 32; precision M3D_PREC_FP64
  * (complex128 kernels) or M3D_PREC_MIXED (the complex64 inner kernels).  The
  * default shape of a new operator: sweep_sub 1, sweep_cap 4096, row_cap 2048,
- * sweep_chunk 1, packed 1 at widths up to 4 and 0 above, wave 0. */
+ * sweep_chunk 1, packed 1, wave 1 at widths 8-32 (0 below). */
 int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launch_shape* out);
 int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d_launch_shape* shape);
 

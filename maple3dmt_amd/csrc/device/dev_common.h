@@ -93,7 +93,7 @@ struct LaunchShape {
     int sweep_chunk;  // 1: sweeps grid-stride over row blocks; k > 1: each workgroup
                       // takes k consecutive row blocks (neighbouring rows on one CU)
     int packed;       // 1: gather from the packed blocks when the operator has them
-    int wave;         // 1 (widths 8-32): couplings loaded once per wavefront and shared
+    int wave;         // 1 (widths 8-32): couplings loaded once per row and passed among its lanes
 };
 constexpr int kWidths = 6;  // batch widths 1, 2, 4, 8, 16, 32
 __host__ __device__ constexpr int width_index(int ns) {

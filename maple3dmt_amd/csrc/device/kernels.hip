@@ -127,57 +127,85 @@ __device__ inline void row_gather(const DevOp<V>& op, int off, int w, int lane, 
     gather_block<8, 1, UP, V>(op, off, w, lane, 0, x, NS, sys, ar, ai);
 }
 
-// Wide batches (NS >= 8): a wavefront holds R = 64 / NS consecutive rows of
-// one slice, and entry k of those R rows is R consecutive words of a SELL
-// block.  They are loaded once per wavefront at uniform addresses (through
-// the scalar cache) and each lane picks its row's, so a coupling costs the
-// wavefront one vector load, the x gather, instead of three (column, value
-// and x; the address unit takes the same time for a vector load whatever few
-// distinct words it returns).  The values of a packed block come from a
-// copy of the value table in LDS.  wrow = the wavefront's first row within
-// the slice, r = the lane's row within the wavefront.  Entries are added in
-// the order of gather_sell / gather_packed, so the sums are the same bit for
-// bit; entries past the width load nothing and add 0 * 0.
+// Wide batches (NS >= 8): the NS lanes of a row (NS consecutive lanes of a
+// wavefront, vectors being system-minor) load the couplings of NS consecutive
+// entries of the row at once, lane j the coupling of entry k0 + j, and hand
+// each entry's coupling to the row's lanes through ds_bpermute (the LDS
+// crossbar; no memory access).  An entry then costs the wavefront one vector
+// memory load, its x gather, instead of three (column or packed word, value,
+// x), and every NS entries one more.  The values of a packed block come from a
+// copy of the value table in LDS.  rl = the row's lane within its slice, gl =
+// the row's first lane within the wavefront.  Entries are added in the order
+// of gather_sell / gather_packed, increasing k (padding adds 0 * 0, which
+// leaves a row sum unchanged), so the sums are the same bit for bit.
 template <int NS>
 __host__ __device__ constexpr bool wave_gather_width() { return NS >= 8 && NS <= 32; }
 
-template <int U, int NS, typename V>
-__device__ inline void gather_wave_packed(const unsigned* __restrict__ pk, const V* __restrict__ ltab, int cbits,
-                                          int off, int w, int wrow, int r, const cvec<V>* __restrict__ x,
-                                          int sys, V& ar, V& ai) {
-    constexpr int R = 64 / NS;
-    off = __builtin_amdgcn_readfirstlane(off);
-    w = __builtin_amdgcn_readfirstlane(w);
-    wrow = __builtin_amdgcn_readfirstlane(wrow);
+__device__ inline unsigned lane_get(unsigned v, int src) {
+    return unsigned(__builtin_amdgcn_ds_bpermute(src << 2, int(v)));
+}
+__device__ inline int lane_get(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+__device__ inline float lane_get(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ inline double lane_get(double v, int src) {
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(v));
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// One lane's share of a chunk of a SELL block: the coupling of entry k of
+// its row (zero past the width).
+template <typename V>
+struct LaneCoupling {
+    unsigned word;   // packed word, or the column of a separate block
+    V val;           // value of a separate block (unused when packed)
+};
+
+template <bool PACKED, typename V>
+__device__ inline LaneCoupling<V> lane_load(const unsigned* __restrict__ pk, const V* __restrict__ val,
+                                            const int* __restrict__ col, int off, int w, int k, int rl) {
+    LaneCoupling<V> c{0u, V(0)};
+    if (k < w) {
+        const int idx = off + k * kSlice + rl;
+        if (PACKED) {
+            c.word = pk[idx];
+        } else {
+            c.word = unsigned(col[idx]);
+            c.val = val[idx];
+        }
+    }
+    return c;
+}
+
+// Adds the n (<= NS, uniform) entries of one chunk, whose couplings lane
+// gl + j holds for entry j, in batches of U gathers in flight.  Every lane of
+// the row passes its coupling on; a lane whose system is not running (run =
+// false) gathers nothing and adds zeros.
+template <int U, int NS, bool PACKED, typename V>
+__device__ inline void lane_chunk(const LaneCoupling<V>& mine, int n, const V* __restrict__ ltab, int cbits,
+                                  int gl, const cvec<V>* __restrict__ x, int sys, bool run, V& ar, V& ai) {
+    static_assert(NS % U == 0, "a batch stays within the row's lanes");
     const unsigned cm = (1u << cbits) - 1u;
-    const unsigned* __restrict__ base = pk + off + wrow;
-    for (int k = 0; k < w; k += U) {
+    for (int j = 0; j < n; j += U) {
         unsigned wd[U];
+        V a[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int kk = k + u;
-            unsigned sel = 0u;
-            if (kk < w) {
-                const unsigned* __restrict__ q = base + kk * kSlice;
-                sel = q[0];
-#pragma unroll
-                for (int j = 1; j < R; ++j) {
-                    const unsigned v = q[j];
-                    sel = r == j ? v : sel;
-                }
-            }
-            wd[u] = sel;
+            wd[u] = lane_get(mine.word, gl + j + u);
+            if (!PACKED) a[u] = lane_get(mine.val, gl + j + u);
         }
         cvec<V> xv[U];
-        V a[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             xv[u].x = V(0);
             xv[u].y = V(0);
-            a[u] = V(0);
-            if (k + u < w) {
-                xv[u] = x[size_t(wd[u] & cm) * NS + sys];
-                a[u] = ltab[wd[u] >> cbits];
+            if (PACKED) a[u] = V(0);
+            if (run && j + u < n) {
+                xv[u] = x[size_t(PACKED ? (wd[u] & cm) : wd[u]) * NS + sys];
+                if (PACKED) a[u] = ltab[wd[u] >> cbits];
+            } else if (!PACKED) {
+                a[u] = V(0);
             }
         }
 #pragma unroll
@@ -188,88 +216,80 @@ __device__ inline void gather_wave_packed(const unsigned* __restrict__ pk, const
     }
 }
 
-template <int U, int NS, typename V>
-__device__ inline void gather_wave_sell(const V* __restrict__ val, const int* __restrict__ col, int off, int w,
-                                        int wrow, int r, const cvec<V>* __restrict__ x, int sys, V& ar,
-                                        V& ai) {
-    constexpr int R = 64 / NS;
+// One block of a row (UP: the upper block), chunk by chunk.
+template <int NS, bool PACKED, bool UP, typename V>
+__device__ inline void lane_block_at(const DevOp<V>& op, const V* __restrict__ ltab, int off, int w, int rl,
+                                     int gl, const cvec<V>* __restrict__ x, int sys, bool run, V& ar, V& ai) {
     off = __builtin_amdgcn_readfirstlane(off);
     w = __builtin_amdgcn_readfirstlane(w);
-    wrow = __builtin_amdgcn_readfirstlane(wrow);
-    for (int k = 0; k < w; k += U) {
-        int c[U];
-        V a[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = k + u;
-            int cs = 0;
-            V as = V(0);
-            if (kk < w) {
-                const int e = off + kk * kSlice + wrow;
-                cs = col[e];
-                as = val[e];
-#pragma unroll
-                for (int j = 1; j < R; ++j) {
-                    const int cj = col[e + j];
-                    const V aj = val[e + j];
-                    cs = r == j ? cj : cs;
-                    as = r == j ? aj : as;
-                }
-            }
-            c[u] = cs;
-            a[u] = as;
-        }
-        cvec<V> xv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            xv[u].x = V(0);
-            xv[u].y = V(0);
-            if (k + u < w) xv[u] = x[size_t(c[u]) * NS + sys];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            ar += a[u] * xv[u].x;
-            ai += a[u] * xv[u].y;
-        }
+    for (int k0 = 0; k0 < w; k0 += NS) {
+        const LaneCoupling<V> c = lane_load<PACKED, V>(UP ? op.u_pk : op.l_pk, UP ? op.u_val : op.l_val,
+                                                       UP ? op.u_col : op.l_col, off, w, k0 + sys, rl);
+        lane_chunk<8, NS, PACKED, V>(c, min(NS, w - k0), ltab, op.cbits, gl, x, sys, run, ar, ai);
     }
 }
 
-// Batch of entries per gather step: the scalar registers one step holds
-// (R words per packed entry, R columns and R values per separate entry) kept
-// near 32-48.
-template <int NS, bool PACKED>
-__host__ __device__ constexpr int wave_unroll() {
-    return PACKED ? (NS >= 16 ? 8 : 4) : (NS >= 32 ? 8 : NS >= 16 ? 4 : 2);
-}
-
-template <bool UP, int NS, typename V>
-__device__ inline void gather_wave_block(const DevOp<V>& op, const V* __restrict__ ltab, int off, int w, int wrow,
-                                         int r, const cvec<V>* __restrict__ x, int sys, V& ar, V& ai) {
+// lane_block_at from the packed blocks when ltab (the LDS value table) is
+// given, else from the separate blocks.
+template <int NS, bool UP, typename V>
+__device__ inline void lane_block(const DevOp<V>& op, const V* __restrict__ ltab, int off, int w, int row,
+                                  const cvec<V>* __restrict__ x, int sys, bool run, V& ar, V& ai) {
+    const int gl = (threadIdx.x & 63) - sys;
     if (ltab)
-        gather_wave_packed<wave_unroll<NS, true>(), NS, V>(UP ? op.u_pk : op.l_pk, ltab, op.cbits, off, w, wrow, r,
-                                                           x, sys, ar, ai);
+        lane_block_at<NS, true, UP, V>(op, ltab, off, w, row & 63, gl, x, sys, run, ar, ai);
     else
-        gather_wave_sell<wave_unroll<NS, false>(), NS, V>(UP ? op.u_val : op.l_val, UP ? op.u_col : op.l_col, off,
-                                                          w, wrow, r, x, sys, ar, ai);
+        lane_block_at<NS, false, UP, V>(op, ltab, off, w, row & 63, gl, x, sys, run, ar, ai);
 }
 
-// (A x)_row of apply_row by wavefront-shared couplings; ltab = the LDS copy
-// of the value table when this width gathers from the packed blocks.
+// Lower then upper block of a row; the first chunk of both blocks is loaded
+// before either is gathered, so a row with both widths <= NS waits for two
+// memory round trips (couplings, then x) instead of four.
+template <int NS, bool PACKED, typename V>
+__device__ inline void lane_row(const DevOp<V>& op, const V* __restrict__ ltab, int4 sc, int rl, int gl,
+                                const cvec<V>* __restrict__ x, int sys, bool run, V& ar, V& ai) {
+    constexpr int U = 8;
+    const int lo = __builtin_amdgcn_readfirstlane(sc.x), lw = __builtin_amdgcn_readfirstlane(sc.y);
+    const int uo = __builtin_amdgcn_readfirstlane(sc.z), uw = __builtin_amdgcn_readfirstlane(sc.w);
+    const LaneCoupling<V> l0 = lane_load<PACKED, V>(op.l_pk, op.l_val, op.l_col, lo, lw, sys, rl);
+    const LaneCoupling<V> u0 = lane_load<PACKED, V>(op.u_pk, op.u_val, op.u_col, uo, uw, sys, rl);
+    lane_chunk<U, NS, PACKED, V>(l0, min(NS, lw), ltab, op.cbits, gl, x, sys, run, ar, ai);
+    for (int k0 = NS; k0 < lw; k0 += NS) {
+        const LaneCoupling<V> c = lane_load<PACKED, V>(op.l_pk, op.l_val, op.l_col, lo, lw, k0 + sys, rl);
+        lane_chunk<U, NS, PACKED, V>(c, min(NS, lw - k0), ltab, op.cbits, gl, x, sys, run, ar, ai);
+    }
+    lane_chunk<U, NS, PACKED, V>(u0, min(NS, uw), ltab, op.cbits, gl, x, sys, run, ar, ai);
+    for (int k0 = NS; k0 < uw; k0 += NS) {
+        const LaneCoupling<V> c = lane_load<PACKED, V>(op.u_pk, op.u_val, op.u_col, uo, uw, k0 + sys, rl);
+        lane_chunk<U, NS, PACKED, V>(c, min(NS, uw - k0), ltab, op.cbits, gl, x, sys, run, ar, ai);
+    }
+}
+
+// (A x)_row of apply_row by couplings shared among the row's lanes; ltab =
+// the LDS copy of the value table when this width gathers from the packed
+// blocks.
 template <int NS, typename V>
 __device__ inline cvec<V> apply_row_wave(const DevOp<V>& op, const V* __restrict__ ltab, int row, int sys, V w,
-                                         const cvec<V>* __restrict__ x) {
+                                         const cvec<V>* __restrict__ x, bool run = true) {
     const int4 sc = slice_desc(op.slc, row);
-    const int r = (threadIdx.x & 63) / NS;
-    const int wrow = (row & 63) - r;
+    const int rl = row & 63;
+    const int gl = (threadIdx.x & 63) - sys;
     V ar = 0, ai = 0;
-    gather_wave_block<false, NS, V>(op, ltab, sc.x, sc.y, wrow, r, x, sys, ar, ai);
-    gather_wave_block<true, NS, V>(op, ltab, sc.z, sc.w, wrow, r, x, sys, ar, ai);
+    if (ltab)
+        lane_row<NS, true, V>(op, ltab, sc, rl, gl, x, sys, run, ar, ai);
+    else
+        lane_row<NS, false, V>(op, ltab, sc, rl, gl, x, sys, run, ar, ai);
     const V dr = op.diag_s[row], di = w * op.me[row];
     const cvec<V> xi = x[size_t(row) * NS + sys];
     cvec<V> y;
     y.x = ar + (dr * xi.x - di * xi.y);
     y.y = ai + (dr * xi.y + di * xi.x);
     return y;
+}
+
+// The dynamic LDS of a launch (one declaration shared by every kernel).
+__device__ inline unsigned char* dyn_lds() {
+    extern __shared__ __align__(16) unsigned char m3d_dyn_lds[];
+    return m3d_dyn_lds;
 }
 
 // The value table of a packed operator copied into LDS by the whole block
@@ -331,6 +351,22 @@ __device__ inline cvec<V> apply_row_eis(const DevOp<V>& op, int row, int NS, int
     const int lane = row & 63;
     V ar = 0, ai = 0;
     row_gather<false>(op, sc.x, sc.y, lane, z, NS, sys, ar, ai);
+    const V dr = op.diag_s[row], di = w * op.me[row];
+    const cvec<V> yi = y[size_t(row) * NS + sys];
+    cvec<V> out;
+    out.x = ar + (dr * yi.x - di * yi.y);
+    out.y = ai + (dr * yi.y + di * yi.x);
+    return out;
+}
+
+// apply_row_eis by couplings shared among the row's lanes (wide batches).
+template <int NS, typename V>
+__device__ inline cvec<V> apply_row_eis_wave(const DevOp<V>& op, const V* __restrict__ ltab, int row, int sys,
+                                             V w, const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ y,
+                                             bool run) {
+    const int4 sc = slice_desc(op.slc, row);
+    V ar = 0, ai = 0;
+    lane_block<NS, false, V>(op, ltab, sc.x, sc.y, row, z, sys, run, ar, ai);
     const V dr = op.diag_s[row], di = w * op.me[row];
     const cvec<V> yi = y[size_t(row) * NS + sys];
     cvec<V> out;
@@ -486,14 +522,20 @@ __global__ __launch_bounds__(kFinBlock) void k_bicg_init_fin(SysState* st, const
 // With zl (the last colour, which has no later colours) the backward step of
 // these rows is done here too: z_i = y_i - (0 / d_i), the backward sweep's
 // arithmetic with an empty upper block.
-template <int NS, typename V, int SUB, int MODE>
+// LANES (wide batches, one lane per row): couplings shared among the row's
+// lanes; the lanes of systems that are not running stay to pass their
+// couplings on and store nothing.
+template <int NS, typename V, int SUB, int MODE, bool LANES>
 __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
                             const cvec<V>* __restrict__ in, cvec<V>* __restrict__ p,
                             const cvec<V>* __restrict__ v, cvec<V>* __restrict__ y,
                             const SysState* __restrict__ st, cvec<V>* __restrict__ zl) {
+    static_assert(!LANES || SUB == 1, "shared couplings at one lane per row");
     const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
     constexpr int rpb = sweep_rpb<NS, SUB>();
-    if (!((status_mask<NS>(st, kRunning) >> sys) & 1u)) return;
+    const V* ltab = LANES ? stage_value_table<NS, V>(op, dyn_lds()) : nullptr;
+    const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
+    if (!LANES && !run) return;
     unsigned first_m = 0;
 #pragma unroll
     for (int q = 0; q < NS; ++q) first_m |= (st[q].iter == 0 ? 1u : 0u) << q;
@@ -521,7 +563,7 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
                 src.x = src.x + (br * tr - bi * ti);
                 src.y = src.y + (br * ti + bi * tr);
             }
-            if (sub == 0) p[i] = src;
+            if (sub == 0 && run) p[i] = src;
         } else if (MODE == kInSUpdate) {
             // alpha v with explicit fused multiply-adds: the rounding of s
             // is fixed by the source, not by what contraction the compiler
@@ -532,18 +574,20 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
             const V avi = fmav(ai, vi.x, ar * vi.y);
             src.x = src.x - avr;
             src.y = src.y - avi;
-            if (sub == 0) p[i] = src;
+            if (sub == 0 && run) p[i] = src;
         }
-        const int lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather_split<SUB, false>(op, sc.x, sc.y, lane, y, NS, sys, sub, ar, ai);
+        if constexpr (LANES)
+            lane_block<NS, false, V>(op, ltab, sc.x, sc.y, row, y, sys, run, ar, ai);
+        else
+            row_gather_split<SUB, false>(op, sc.x, sc.y, row & 63, y, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(ds, w * me, dr, di);
         const V er = src.x - ar, ei = src.y - ai;
         cvec<V> out;
         out.x = er * dr - ei * di;
         out.y = er * di + ei * dr;
-        if (sub == 0) {
+        if (sub == 0 && run) {
             y[i] = out;
             if (zl) {
                 const V zr = 0, zi = 0;
@@ -558,13 +602,17 @@ __global__ void k_sweep_fwd(DevOp<V> op, const double* __restrict__ omega, int r
 
 // Backward sweep over one colour: z_i = y_i - (sum_{j in later colours} a_ij z_j) / d_i
 // (ssor.h:71-78).
-template <int NS, typename V, int SUB>
+// LANES as in k_sweep_fwd.
+template <int NS, typename V, int SUB, bool LANES>
 __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r0, int r1,
                             const cvec<V>* __restrict__ y, cvec<V>* __restrict__ z,
                             const SysState* __restrict__ st) {
+    static_assert(!LANES || SUB == 1, "shared couplings at one lane per row");
     const int tid = threadIdx.x, sys = tid % NS, sub = (tid / NS) % SUB;
     constexpr int rpb = sweep_rpb<NS, SUB>();
-    if (!((status_mask<NS>(st, kRunning) >> sys) & 1u)) return;
+    const V* ltab = LANES ? stage_value_table<NS, V>(op, dyn_lds()) : nullptr;
+    const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
+    if (!LANES && !run) return;
     const V w = V(omega[sys]);
     const int nrows = r1 - r0;
     const int nrb = (nrows + rpb - 1) / rpb;
@@ -577,27 +625,42 @@ __global__ void k_sweep_bwd(DevOp<V> op, const double* __restrict__ omega, int r
         const int4 sc = slice_desc(op.slc, row);
         const V ds = op.diag_s[row], me = op.me[row];
         const cvec<V> yi = y[i];
-        const int lane = row & 63;
         V ar = 0, ai = 0;
-        row_gather_split<SUB, true>(op, sc.z, sc.w, lane, z, NS, sys, sub, ar, ai);
+        if constexpr (LANES)
+            lane_block<NS, true, V>(op, ltab, sc.z, sc.w, row, z, sys, run, ar, ai);
+        else
+            row_gather_split<SUB, true>(op, sc.z, sc.w, row & 63, z, NS, sys, sub, ar, ai);
         V dr, di;
         cinv(ds, w * me, dr, di);
         cvec<V> out;
         out.x = yi.x - (ar * dr - ai * di);
         out.y = yi.y - (ar * di + ai * dr);
-        if (sub == 0) z[i] = out;
+        if (sub == 0 && run) z[i] = out;
     }
 }
 
 // v = A p_hat and (r0, v); alpha = rho / (r0, v) (bicgstab.cpp:76-87).
 // With ys (the forward-sweep output of p_hat) v = L p_hat + D ys.
-template <int NS, typename V>
+// (A z)_row of the BiCGStab products: Eisenstat's form with ys, else the
+// plain product; LANES: couplings shared among the row's lanes.
+template <int NS, typename V, bool LANES>
+__device__ inline cvec<V> product_row(const DevOp<V>& op, const V* __restrict__ ltab, int row, int sys, V w,
+                                      const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys, bool run) {
+    if constexpr (LANES)
+        return ys ? apply_row_eis_wave<NS, V>(op, ltab, row, sys, w, z, ys, run)
+                  : apply_row_wave<NS, V>(op, ltab, row, sys, w, z, run);
+    else
+        return ys ? apply_row_eis(op, row, NS, sys, w, z, ys) : apply_row(op, row, NS, sys, w, z);
+}
+
+template <int NS, typename V, bool LANES>
 __global__ void k_spmv_v(DevOp<V> op, const double* __restrict__ omega,
                          const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys,
                          const cvec<V>* __restrict__ r0, cvec<V>* __restrict__ v,
                          const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
+    const V* ltab = LANES ? stage_value_table<NS, V>(op, dyn_lds()) : nullptr;
     const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
     const V w = V(omega[sys]);
     double acc[2] = {0.0, 0.0};
@@ -605,9 +668,10 @@ __global__ void k_spmv_v(DevOp<V> op, const double* __restrict__ omega,
     const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
-        if (row >= op.n || !run) continue;
+        if (row >= op.n || !(run || LANES)) continue;
         const size_t i = size_t(row) * NS + sys;
-        const cvec<V> y = ys ? apply_row_eis(op, row, NS, sys, w, z, ys) : apply_row(op, row, NS, sys, w, z);
+        const cvec<V> y = product_row<NS, V, LANES>(op, ltab, row, sys, w, z, ys, run);
+        if (!run) continue;
         v[i] = y;
         const double2 d = cconjmul(to_d<V>(r0[i]), to_d<V>(y));
         acc[0] += d.x;
@@ -640,13 +704,14 @@ __global__ __launch_bounds__(kFinBlock) void k_spmv_v_fin(SysState* st, const do
 // of s_hat); the finish tests ||s|| (bicgstab.cpp:94-103), then
 // omega = (t,s)/(t,t) (:112-125).  With ys (the forward-sweep output of
 // s_hat) t = L s_hat + D ys.
-template <int NS, typename V>
+template <int NS, typename V, bool LANES>
 __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
                          const cvec<V>* __restrict__ z, const cvec<V>* __restrict__ ys,
                          const cvec<V>* __restrict__ s, cvec<V>* __restrict__ t,
                          const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
+    const V* ltab = LANES ? stage_value_table<NS, V>(op, dyn_lds()) : nullptr;
     const bool run = (status_mask<NS>(st, kRunning) >> sys) & 1u;
     const V w = V(omega[sys]);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -654,9 +719,10 @@ __global__ void k_spmv_t(DevOp<V> op, const double* __restrict__ omega,
     const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int row = rb * rpb + tid / NS;
-        if (row >= op.n || !run) continue;
+        if (row >= op.n || !(run || LANES)) continue;
         const size_t i = size_t(row) * NS + sys;
-        const cvec<V> y = ys ? apply_row_eis(op, row, NS, sys, w, z, ys) : apply_row(op, row, NS, sys, w, z);
+        const cvec<V> y = product_row<NS, V, LANES>(op, ltab, row, sys, w, z, ys, run);
+        if (!run) continue;
         t[i] = y;
         const double2 td = to_d<V>(y);
         acc[0] += cabs2(td);
@@ -799,16 +865,15 @@ __global__ __launch_bounds__(kFinBlock) void k_bicg_xr_fin(SysState* st, const d
 // Plain operator application and residuals.
 // ---------------------------------------------------------------------------
 
-// y = A x, or y = b - A x when b != nullptr.  WAVE: couplings shared by
-// the wavefront (wide batches, apply_row_wave).
+// y = A x, or y = b - A x when b != nullptr.  WAVE: couplings shared among
+// a row's lanes (wide batches, apply_row_wave).
 template <int NS, typename V, bool WAVE>
 __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
                        const cvec<V>* __restrict__ x, const cvec<V>* __restrict__ b,
                        cvec<V>* __restrict__ y) {
-    extern __shared__ __align__(16) unsigned char k_spmv_lds[];
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
-    const V* ltab = WAVE ? stage_value_table<NS, V>(op, k_spmv_lds) : nullptr;
+    const V* ltab = WAVE ? stage_value_table<NS, V>(op, dyn_lds()) : nullptr;
     const V w = V(omega[sys]);
     const int nrb = (op.n + rpb - 1) / rpb;
     const int* __restrict__ order = op.order[width_index(NS)];
@@ -818,7 +883,11 @@ __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
         const int row = rb * rpb + tid / NS;
         if (row >= op.n) continue;
         const size_t i = size_t(row) * NS + sys;
-        cvec<V> a = WAVE ? apply_row_wave<NS, V>(op, ltab, row, sys, w, x) : apply_row(op, row, NS, sys, w, x);
+        cvec<V> a;
+        if constexpr (WAVE)
+            a = apply_row_wave<NS, V>(op, ltab, row, sys, w, x);
+        else
+            a = apply_row(op, row, NS, sys, w, x);
         if (b) {
             const cvec<V> bi = b[i];
             a.x = bi.x - a.x;
@@ -1580,26 +1649,58 @@ int sweep_grid(const DevOp<V>& op, int rows) {
     return grid_rows(nrb, sh.sweep_chunk, 1 << 30);
 }
 
+// Couplings shared among a row's lanes at this width (launch shape wave = 1;
+// the sweeps only at one lane per row), and the dynamic LDS such a launch
+// needs: the value table when the width gathers from the packed blocks.
+template <int NS, typename V>
+bool lanes_on(const DevOp<V>& op) {
+    return wave_gather_width<NS>() && op.shape[width_index(NS)].wave;
+}
+template <int NS, typename V>
+size_t lanes_lds(const DevOp<V>& op) {
+    return op.vtab && op.shape[width_index(NS)].packed ? op.ntab * sizeof(V) : 0;
+}
+
+template <int NS, typename V, int SUB, bool LANES>
+void sweep_fwd_launch(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1, int mode,
+                      const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, const SysState* st,
+                      cvec<V>* zl) {
+    const dim3 g(sweep_grid<NS, SUB>(op, r1 - r0));
+    const size_t lds = LANES ? lanes_lds<NS>(op) : 0;
+    if (mode == kInPUpdate)
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInPUpdate, LANES>), g, dim3(kBlock), lds, s, op, omega, r0,
+                           r1, in, p, v, y, st, zl);
+    else if (mode == kInSUpdate)
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInSUpdate, LANES>), g, dim3(kBlock), lds, s, op, omega, r0,
+                           r1, in, p, v, y, st, zl);
+    else
+        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInPlain, LANES>), g, dim3(kBlock), lds, s, op, omega, r0,
+                           r1, in, p, v, y, st, zl);
+}
 template <int NS, typename V, int SUB>
 void sweep_fwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1, int mode,
                   const cvec<V>* in, cvec<V>* p, const cvec<V>* v, cvec<V>* y, const SysState* st,
                   cvec<V>* zl) {
-    const dim3 g(sweep_grid<NS, SUB>(op, r1 - r0));
-    if (mode == kInPUpdate)
-        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInPUpdate>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
-                           in, p, v, y, st, zl);
-    else if (mode == kInSUpdate)
-        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInSUpdate>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
-                           in, p, v, y, st, zl);
-    else
-        hipLaunchKernelGGL((k_sweep_fwd<NS, V, SUB, kInPlain>), g, dim3(kBlock), 0, s, op, omega, r0, r1,
-                           in, p, v, y, st, zl);
+    if constexpr (SUB == 1 && wave_gather_width<NS>()) {
+        if (lanes_on<NS>(op)) {
+            sweep_fwd_launch<NS, V, 1, true>(s, op, omega, r0, r1, mode, in, p, v, y, st, zl);
+            return;
+        }
+    }
+    sweep_fwd_launch<NS, V, SUB, false>(s, op, omega, r0, r1, mode, in, p, v, y, st, zl);
 }
 template <int NS, typename V, int SUB>
 void sweep_bwd_at(hipStream_t s, const DevOp<V>& op, const double* omega, int r0, int r1,
                   const cvec<V>* y, cvec<V>* z, const SysState* st) {
     const dim3 g(sweep_grid<NS, SUB>(op, r1 - r0));
-    hipLaunchKernelGGL((k_sweep_bwd<NS, V, SUB>), g, dim3(kBlock), 0, s, op, omega, r0, r1, y, z, st);
+    if constexpr (SUB == 1 && wave_gather_width<NS>()) {
+        if (lanes_on<NS>(op)) {
+            hipLaunchKernelGGL((k_sweep_bwd<NS, V, 1, true>), g, dim3(kBlock), lanes_lds<NS>(op), s, op, omega,
+                               r0, r1, y, z, st);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_sweep_bwd<NS, V, SUB, false>), g, dim3(kBlock), 0, s, op, omega, r0, r1, y, z, st);
 }
 
 template <int NS, typename V>
@@ -1626,7 +1727,12 @@ void Launch<NS, V>::spmv_v(hipStream_t s, const DevOp<V>& op, const double* omeg
                            const cvec<V>* z, const cvec<V>* ys, const cvec<V>* r0, cvec<V>* v,
                            SysState* st, double* part) {
     const int g = row_grid<NS>(op.n, op.shape[width_index(NS)].row_cap);
-    hipLaunchKernelGGL((k_spmv_v<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, r0, v, st, part);
+    if (lanes_on<NS>(op))
+        hipLaunchKernelGGL((k_spmv_v<NS, V, wave_gather_width<NS>()>), dim3(g), dim3(kBlock), lanes_lds<NS>(op), s,
+                           op, omega, z, ys, r0, v, st, part);
+    else
+        hipLaunchKernelGGL((k_spmv_v<NS, V, false>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, r0, v, st,
+                           part);
     hipLaunchKernelGGL((k_spmv_v_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
@@ -1634,7 +1740,12 @@ void Launch<NS, V>::spmv_t(hipStream_t s, const DevOp<V>& op, const double* omeg
                            const cvec<V>* z, const cvec<V>* ys, const cvec<V>* sv, cvec<V>* t,
                            SysState* st, double* part) {
     const int g = row_grid<NS>(op.n, op.shape[width_index(NS)].row_cap);
-    hipLaunchKernelGGL((k_spmv_t<NS, V>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, sv, t, st, part);
+    if (lanes_on<NS>(op))
+        hipLaunchKernelGGL((k_spmv_t<NS, V, wave_gather_width<NS>()>), dim3(g), dim3(kBlock), lanes_lds<NS>(op), s,
+                           op, omega, z, ys, sv, t, st, part);
+    else
+        hipLaunchKernelGGL((k_spmv_t<NS, V, false>), dim3(g), dim3(kBlock), 0, s, op, omega, z, ys, sv, t, st,
+                           part);
     hipLaunchKernelGGL((k_spmv_t_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS, typename V>
@@ -1649,10 +1760,9 @@ void Launch<NS, V>::bicg_xr(hipStream_t s, int n, cvec<V>* x, const cvec<V>* ph,
 template <int NS, typename V>
 void Launch<NS, V>::spmv(hipStream_t s, const DevOp<V>& op, const double* omega, const cvec<V>* x,
                          const cvec<V>* b, cvec<V>* y) {
-    if (wave_gather_width<NS>() && op.shape[width_index(NS)].wave) {
-        const bool pk = op.vtab && op.shape[width_index(NS)].packed;
+    if (lanes_on<NS>(op)) {
         hipLaunchKernelGGL((k_spmv<NS, V, wave_gather_width<NS>()>), dim3(row_grid<NS>(op.n, kStreamCap)),
-                           dim3(kBlock), pk ? op.ntab * sizeof(V) : 0, s, op, omega, x, b, y);
+                           dim3(kBlock), lanes_lds<NS>(op), s, op, omega, x, b, y);
     } else {
         hipLaunchKernelGGL((k_spmv<NS, V, false>), dim3(row_grid<NS>(op.n, kStreamCap)), dim3(kBlock), 0, s,
                            op, omega, x, b, y);

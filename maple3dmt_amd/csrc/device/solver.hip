@@ -1143,15 +1143,14 @@ LaunchShape default_shape(int width) {
     sh.sweep_cap = 4096;
     sh.row_cap = kMaxBlocks;
     sh.sweep_chunk = 1;
-    // Packed blocks pay at narrow widths, where a row's few gathers wait on
-    // memory latency and fewer bytes per entry shorten every sweep (one
-    // BiCGStab iteration -12 % at 1 system, -5 % at 2 on a 1.45 M-edge mesh);
-    // at 16 systems the separate blocks were 3 % faster.
-    sh.packed = width <= 4 ? 1 : 0;
-    // Wavefront-shared coupling loads measured 10-17 % slower than per-lane
-    // loads for the SpMV at 8, 16 and 32 systems on a 1.45 M-edge mesh, so
-    // they are off by default; the autotuner still tries them from 8.
-    sh.wave = 0;
+    // Packed blocks: fewer bytes per entry shorten every gather (one
+    // BiCGStab iteration on a 1.45 M-edge mesh -12 % at 1 system, -5 % at 2,
+    // -3 to -14 % at 8-32 with the row's lanes sharing the couplings).
+    sh.packed = 1;
+    // Couplings shared among a row's lanes: the 16-system SpMV 0.58 -> 0.45 ms,
+    // an FP64 BiCGStab iteration -3 % at 8 systems and -5 % at 16, a complex64
+    // one -3 % and -8 % on the same mesh.
+    sh.wave = width >= 8 ? 1 : 0;
     return sh;
 }
 

@@ -29,11 +29,11 @@ def block(ctx):
 
 
 def _default_packed(w):
-    return 1 if w <= 4 else 0
+    return 1
 
 
 def _default_wave(w):
-    return 0
+    return 1 if w >= 8 else 0
 
 
 def _reset(dev):
@@ -61,7 +61,7 @@ def test_shape_defaults_and_validation(block):
         dev.set_shape(PREC_FP64, 8, 1, 4096, 2048, 1, 2)   # packed 2
     with pytest.raises(NativeError):
         dev.set_shape(PREC_FP64, 4, 1, 4096, 2048, 1, 1, 1)   # wave below width 8
-    dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4)
+    dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4, 0, 0)
     assert dev.get_shape(PREC_FP64, 8) == {"sweep_sub": 4, "sweep_cap": 1024, "row_cap": 512,
                                            "sweep_chunk": 4, "packed": 0, "wave": 0}
     dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4, 0, 1)

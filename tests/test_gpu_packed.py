@@ -7,8 +7,8 @@ additions, so every result must be identical bit for bit: SpMV, SSOR apply,
 and whole FP64 and mixed-precision solves (same iteration counts, same
 fields).  Compared between two operators, one built without packed blocks
 (M3D_PACKED_SELL=0), and within one operator by switching the launch shape's
-layout choice.  The other GPU tests run the default choice (packed up to 4
-systems) against the oracle.
+layout choice.  The other GPU tests run the default choice (packed blocks at
+every width) against the oracle.
 """
 import os
 
@@ -69,7 +69,7 @@ def all_packed(pair):
     for prec in (0, PREC_MIXED):
         for w in (1, 2, 4, 8, 16, 32):
             sh = mp.device.get_shape(prec, w)
-            sh["packed"] = 1 if w <= 4 else 0
+            sh["packed"] = 1
             mp.device.set_shape(prec, w, **sh)
 
 
@@ -89,10 +89,11 @@ def test_spmv_and_ssor_bitwise(pair, all_packed, k):
 
 @pytest.mark.parametrize("k", [8, 13, 16, 32])
 def test_wavefront_shared_couplings_bitwise(pair, k):
-    """From 8 systems the SpMV loads each wavefront's couplings once and
-    shares them across its lanes (launch shape wave = 1); the sums are the
-    per-lane gather's bit for bit, from either block layout, and equal the
-    twin operator built without packed blocks."""
+    """From 8 systems the lanes of a row can load its couplings once, lane j
+    entry j of a chunk, and pass them on through the LDS crossbar (launch
+    shape wave = 1); the SpMV sums are the per-lane gather's bit for bit, from
+    either block layout, and equal the twin operator built without packed
+    blocks."""
     wl, mp, mu = pair
     rng = np.random.default_rng(100 + k)
     om = 2 * np.pi * 10.0 ** rng.uniform(-3, 2, k)
@@ -113,6 +114,53 @@ def test_wavefront_shared_couplings_bitwise(pair, k):
     for key, y in outs.items():
         assert np.array_equal(y, ref), key
     assert np.array_equal(mu.device.spmv(om, x), ref)
+
+
+def _set_wave(dev, wave, packed=None):
+    for prec in (0, PREC_MIXED):
+        for w in (8, 16, 32):
+            sh = dev.get_shape(prec, w)
+            sh["wave"] = wave
+            if packed is not None:
+                sh["packed"] = packed
+            dev.set_shape(prec, w, **sh)
+
+
+@pytest.mark.parametrize("packed", [0, 1])
+@pytest.mark.parametrize("k", [8, 13, 16])
+def test_lane_shared_sweeps_and_solves_bitwise(pair, packed, k):
+    """wave = 1 also shares couplings among a row's lanes in the SSOR sweeps
+    and the BiCGStab products.  SSOR application and whole FP64 and mixed
+    solves (systems that finish at different iterations, and 13 systems at
+    width 16, whose three idle lanes pass couplings on without storing) equal
+    the per-lane path's bit for bit."""
+    wl, mp, _ = pair
+    dev = mp.device
+    saved = {(p, w): dev.get_shape(p, w) for p in (0, PREC_MIXED) for w in (8, 16, 32)}
+    rng = np.random.default_rng(200 + k)
+    om = 2 * np.pi * 10.0 ** rng.uniform(-3, 2, k)
+    x = rng.standard_normal((k, mp.n)) + 1j * rng.standard_normal((k, mp.n))
+    rhs = np.stack([mp.rhs_total(mp.bc_values(w, q % 2)) for q, w in enumerate(om)])
+    aux = np.stack([mp.primary_field(w, q % 2) for q, w in enumerate(om)])
+    outs = []
+    try:
+        for wave in (0, 1):
+            _set_wave(dev, wave, packed)
+            o = {"ssor": dev.ssor_apply(om, x)}
+            for precision in (0, PREC_MIXED):
+                prm = mp.fwd_params.solve_params(precision=precision, inner_tol=1e-3 if precision else 0.0)
+                xs, res = dev.solve(MODE_FORWARD, om, rhs, aux, params=prm)
+                o[precision] = (xs, [r["iterations"] for r in res], [r["info"] for r in res])
+            outs.append(o)
+    finally:
+        for (p, w), sh in saved.items():
+            dev.set_shape(p, w, **sh)
+    print("iterations", outs[0][0][1], outs[1][0][1], outs[0][PREC_MIXED][1], flush=True)
+    assert np.array_equal(outs[0]["ssor"], outs[1]["ssor"])
+    for precision in (0, PREC_MIXED):
+        assert outs[0][precision][1] == outs[1][precision][1]
+        assert outs[0][precision][2] == outs[1][precision][2]
+        assert np.array_equal(outs[0][precision][0], outs[1][precision][0])
 
 
 @pytest.mark.parametrize("precision", [0, PREC_MIXED])
