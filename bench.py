@@ -106,6 +106,8 @@ def parse():
     ap.add_argument("--c3", action="store_true",
                     help="separate mode: the 16-model batch (1024 forward + adjoint systems on "
                          "meshes of 0.24-2.4 M edges), one JSON line of its own")
+    ap.add_argument("--c3-precision", choices=("fp64", "mixed"), default="fp64",
+                    help="inner precision of the 16-model batch (--c3)")
     ap.add_argument("--fast-only", action="store_true",
                     help="profiling mode: build the workload, tune the launch shapes, then only solve "
                          "the batch twice with the fast path (mixed precision + interleaved DivCorr)")
@@ -528,7 +530,7 @@ def c3_leg(args):
     models at a time on four streams.  Each model
     is released once its gradient is done, so at most four models' solve
     workspaces are resident."""
-    from maple3dmt_amd import Context, configs, multimodel
+    from maple3dmt_amd import PREC_MIXED, Context, configs, multimodel
     sizes = [(6, 80e3), (7, 40e3), (7, 60e3), (7, 80e3), (7, 100e3)]
     wls = []
     for i in range(16):
@@ -546,9 +548,12 @@ def c3_leg(args):
     t_build = time.time() - t
     edges = [j.model.n for j in jobs]
     log(f"[c3] 16 models built in {t_build:.1f}s, edges {edges}")
+    prec = PREC_MIXED if args.c3_precision == "mixed" else 0
     prm = configs.ForwardParams(bicgstab_tol=1e-7, bicgstab_maxiter=args.maxiter).fast_params(
-        precision=0, dc_interval=args.dc_interval)
-    wall = multimodel.run(jobs, n_streams, prm, prm, release=True)
+        precision=prec, dc_interval=args.dc_interval)
+    # Device-resident gradients: the station steps between the forward and
+    # the adjoint batch run on the device.
+    wall = multimodel.run(jobs, n_streams, prm, prm, release=True, device=True)
     n_sys = multimodel.systems(jobs)
     iters = [int(sum(r["iterations"] for r in j.result.forward + j.result.adjoint)) for j in jobs]
     # A system is solved when it converged, or broke down (info 2) with its
@@ -568,8 +573,13 @@ def c3_leg(args):
            "wall_s": wall, "build_s": t_build, "streams": n_streams, "converged": conv,
            "iterations_per_model": iters, "per_model_s": [j.seconds for j in jobs],
            "unconverged": unconverged,
-           "method": "complex128 BiCGStab + SSOR with DivCorr every 100 iterations (forward and adjoint)",
-           "dtype": "complex128 (f64)", "data": "synthetic (seeded random 3-D models, halfspace observations)"}
+           "unconverged_per_model": [{"forward": j.result.unconverged_forward,
+                                      "adjoint": j.result.unconverged_adjoint} for j in jobs],
+           "method": ("complex64 inner BiCGStab in FP64 refinement" if prec else "complex128 BiCGStab")
+                     + " + SSOR with DivCorr every 100 iterations (forward and adjoint), device-resident "
+                       "gradients",
+           "dtype": "complex128 (f64)" if not prec else "complex64 inner, complex128 outer",
+           "data": "synthetic (seeded random 3-D models, halfspace observations)"}
     log(f"[c3] {n_sys} systems in {wall:.1f}s, {n_solved} solved = {n_solved / wall:.1f} solved systems/s")
     for c in contexts:
         c.close()
