@@ -138,14 +138,16 @@ def kernel_only(args):
     # The 16-system SpMV in both block layouts, separate blocks first, then
     # packed (the counter passes tell the two apart by launch order).
     base = dev.get_shape(0, 16)
+    # Then the LDS-staged row tiles (the default at 16 systems).
     spmv = {}
-    for packed in (0, 1):
-        dev.set_shape(0, 16, base["sweep_sub"], base["sweep_cap"], base["row_cap"], base["sweep_chunk"], packed)
+    for name, packed, tiled in (("packed0", 0, 0), ("packed1", 1, 0), ("tiled", 1, 1)):
+        dev.set_shape(0, 16, base["sweep_sub"], base["sweep_cap"], base["row_cap"], base["sweep_chunk"], packed,
+                      tiled=tiled)
         t, b = dev.bench_kernel(0, omegas, reps=args.reps)
-        spmv[f"packed{packed}"] = {"ms": t, "alg_bytes": b}
-        log(f"[kernel-only] SpMV x16 packed {packed}: {t:.3f} ms, {b / t / 1e6:.0f} GB/s")
+        spmv[name] = {"ms": t, "alg_bytes": b}
+        log(f"[kernel-only] SpMV x16 {name}: {t:.3f} ms, {b / t / 1e6:.0f} GB/s")
     dev.set_shape(0, 16, **base)
-    ms, by = spmv["packed0"]["ms"], spmv["packed0"]["alg_bytes"]
+    ms, by = spmv["tiled"]["ms"], spmv["tiled"]["alg_bytes"]
     ms2, by2 = model.device.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
     ms4, by4 = model.device.bench_kernel(4, omegas[:args.tail_width], reps=args.reps)
     iters = {}
@@ -366,7 +368,13 @@ def main():
     ach = bytes_sw / (ms_sw * 1e-3) / 1e9
     ms_spmv, bytes_spmv = op.bench_kernel(0, omegas, reps=args.reps)
     ach_spmv = bytes_spmv / (ms_spmv * 1e-3) / 1e9
-    spmv_layout = f"packed{op.get_shape(0, 16)['packed']}"
+    sh16 = op.get_shape(0, 16)
+    spmv_kernel = (f"k_spmv_tiled<{ns}, double>" if sh16["tiled"]
+                   else f"k_spmv<{ns}, double> packed{sh16['packed']}")
+    # The same product gathering x from device memory (SELL blocks), for comparison.
+    op.set_shape(0, 16, **{**sh16, "tiled": 0})
+    ms_sell, _ = op.bench_kernel(0, omegas, reps=args.reps)
+    op.set_shape(0, 16, **sh16)
     ms_ssor, bytes_ssor = op.bench_kernel(1, omegas, reps=max(1, args.reps // 4))
     info = op.info()
     info["device_bytes"] = op.device_bytes()
@@ -452,11 +460,12 @@ def main():
                      # per short launch than back to back in a graph.
                      "rocprof_avg_launch_ms": (lambda v: None if v is None else v / 1e3)(
                          measured_traffic(sweep_name, "rocprof_avg_launch_us"))},
-        "spmv": {"kernel": f"k_spmv<{ns}, double> {spmv_layout}", "avg_launch_ms": ms_spmv, "alg_bytes": bytes_spmv,
+        "spmv": {"kernel": spmv_kernel, "avg_launch_ms": ms_spmv, "alg_bytes": bytes_spmv,
                  "achieved_GBs": ach_spmv, "TBs": ach_spmv / 1e3, "frac": ach_spmv / HBM_PEAK_GBS,
-                 "traffic": measured_traffic(f"k_spmv<{ns}, double> {spmv_layout}"),
+                 "traffic": measured_traffic(spmv_kernel),
                  "rocprof_avg_launch_ms": (lambda v: None if v is None else v / 1e3)(
-                     measured_traffic(f"k_spmv<{ns}, double> {spmv_layout}", "rocprof_avg_launch_us"))},
+                     measured_traffic(spmv_kernel, "rocprof_avg_launch_us")),
+                 "sell_gather_avg_launch_ms": ms_sell},
         "ssor_apply": {"systems": ns, "avg_ms": ms_ssor, "alg_bytes": bytes_ssor,
                        "achieved_GBs": bytes_ssor / (ms_ssor * 1e-3) / 1e9},
         "operator": info,

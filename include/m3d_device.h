@@ -340,12 +340,20 @@ typedef struct m3d_launch_shape {
                         products and the SSOR sweeps at sweep_sub 1; 0 = every
                         lane loads each coupling itself.  Same order of summation
                         either way */
+    int tiled;       /* widths 8-32 only (0 below): 1 = the plain product (m3d_spmv,
+                        the forward right-hand side b - A E_pri) runs over row
+                        tiles of consecutive caller edges whose x rows are first
+                        copied into LDS, every coupling then reading x from LDS;
+                        built at m3d_operator_create with the packed blocks
+                        (M3D_TILED_SPMV=0 switches them off).  0 = the product
+                        gathers x from device memory.  Same order of summation
+                        either way */
 } m3d_launch_shape;
 
 /* width is a batch width 1, 2, 4, 8, 16 or 32; precision M3D_PREC_FP64
  * (complex128 kernels) or M3D_PREC_MIXED (the complex64 inner kernels).  The
  * default shape of a new operator: sweep_sub 1, sweep_cap 4096, row_cap 2048,
- * sweep_chunk 1, packed 1, wave 1 at widths 8-32 (0 below). */
+ * sweep_chunk 1, packed 1, wave 1 and tiled 1 at widths 8-32 (0 below). */
 int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launch_shape* out);
 int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d_launch_shape* shape);
 

@@ -15,6 +15,7 @@ constexpr int kSlice = 64;         // SELL slice height = one wavefront of rows
 constexpr int kMaxBlocks = 2048;   // grid cap for reduction kernels (8 blocks per CU)
 constexpr int kMaxSys = 32;        // systems per batched launch
 constexpr int kXcds = 8;           // accelerator complex dies (one L2 each)
+constexpr int kTileWaves = 16;     // wavefronts of a tiled-product workgroup (one slice each)
 
 // Complex value types per precision.
 template <typename V> struct CT;
@@ -94,11 +95,36 @@ struct LaunchShape {
                       // takes k consecutive row blocks (neighbouring rows on one CU)
     int packed;       // 1: gather from the packed blocks when the operator has them
     int wave;         // 1 (widths 8-32): couplings loaded once per row and passed among its lanes
+    int tiled;        // 1 (widths 8-32): the plain product runs over the LDS-staged row tiles
 };
 constexpr int kWidths = 6;  // batch widths 1, 2, 4, 8, 16, 32
 __host__ __device__ constexpr int width_index(int ns) {
     return ns <= 1 ? 0 : ns <= 2 ? 1 : ns <= 4 ? 2 : ns <= 8 ? 3 : ns <= 16 ? 4 : 5;
 }
+
+// Row tiles of the LDS-staged products at one batch width.  A tile is a run of
+// rows consecutive in the caller's edge order (the Morton order of the mesh
+// cells), so its rows gather from a compact neighbourhood: the distinct rows
+// its couplings and diagonals read (stage_rows[stage_off[t] ..
+// stage_off[t + 1]), ascending) are copied into LDS once, and every coupling
+// then reads x from LDS by its local index.  The tile's rows are cut into
+// slices of H = 64 / width rows, one wavefront each, whose entries are stored
+// column-major (words[slc.x + k * H + l], k < slc.y): the local index in the
+// low lbits bits and the index of the value in the operator's value table
+// above them, word 0 for padding (value 0); each row keeps the entry order of
+// the SELL blocks (lower couplings, then upper).  slot[slc.z + l] = {device
+// row, local index of that row} or {-1, 0} for a padding slot.
+struct DevTiles {
+    int ntiles = 0;     // 0: no tiles at this width
+    int lbits = 0;
+    int max_stage = 0;  // most rows any tile stages
+    const int* stage_off = nullptr;
+    const int* stage_rows = nullptr;
+    const int* slice_off = nullptr;
+    const int4* slc = nullptr;
+    const int2* slot = nullptr;
+    const unsigned* words = nullptr;
+};
 
 // Device view of one mesh operator (device row order, padded to slices).
 template <typename V>
@@ -125,6 +151,9 @@ struct DevOp {
     // smallest caller edge index among their rows, or null.  Used by the
     // kernels without reductions, which may visit row blocks in any order.
     const int* order[kWidths];
+    // Per batch width: row tiles of the LDS-staged plain product (ntiles = 0:
+    // none, the product gathers from the SELL blocks).
+    DevTiles tiles[kWidths];
 };
 
 // Row-block schedule: a plain grid stride over the row blocks.  The

@@ -897,6 +897,151 @@ __global__ void k_spmv(DevOp<V> op, const double* __restrict__ omega,
     }
 }
 
+// Bytes of LDS before a tile's staged rows: the value table, 16-byte aligned.
+template <typename V>
+__host__ __device__ constexpr size_t tile_table_bytes(int ntab) {
+    return (size_t(ntab) * sizeof(V) + 15) / 16 * 16;
+}
+constexpr int kTileBlock = kTileWaves * 64;  // threads of a tiled-product workgroup (two per CU by LDS)
+constexpr int kTileLoads = 4;     // staging loads in flight per thread
+
+// Copies the x rows a tile stages into LDS: element i (row i / NS, system
+// i % NS) of the staged block, kTileLoads per thread in flight (all row
+// indices first, then all x loads, then the LDS stores).
+template <int NS, typename V>
+__device__ inline void stage_tile(const DevTiles& T, int s0, int ns, const cvec<V>* __restrict__ x,
+                                  cvec<V>* xs) {
+    const int total = ns * NS;
+    for (int base = threadIdx.x; base < total; base += kTileLoads * kTileBlock) {
+        int r[kTileLoads];
+#pragma unroll
+        for (int j = 0; j < kTileLoads; ++j) {
+            const int i = base + j * kTileBlock;
+            r[j] = i < total ? T.stage_rows[s0 + i / NS] : 0;
+        }
+        cvec<V> v[kTileLoads];
+#pragma unroll
+        for (int j = 0; j < kTileLoads; ++j) {
+            const int i = base + j * kTileBlock;
+            v[j].x = V(0);
+            v[j].y = V(0);
+            if (i < total) v[j] = x[size_t(r[j]) * NS + (i % NS)];
+        }
+#pragma unroll
+        for (int j = 0; j < kTileLoads; ++j) {
+            const int i = base + j * kTileBlock;
+            if (i < total) xs[i] = v[j];
+        }
+    }
+}
+
+// y = A x (or b - A x) over the row tiles of this width (DevTiles), one tile
+// per workgroup.  The workgroup copies the rows of x its tile stages into LDS
+// (the NS systems of a row are NS consecutive complex values, so NS
+// consecutive threads read them in one coalesced segment), then each
+// wavefront computes slices of H = 64 / NS rows, the couplings of a row
+// shared among its NS lanes as in apply_row_wave but every x read from LDS.
+// A wavefront loads its first slice's descriptor, rows, diagonal pairs and
+// couplings before the staging completes.  A row adds its entries in the
+// order of its SELL blocks and its padding adds 0 * 0, so y is k_spmv's bit
+// for bit.  Tiles go out in caller order, a contiguous eighth per XCD
+// (workgroup i runs on XCD i % 8), so neighbouring tiles, which stage
+// overlapping rows, share an L2; results do not depend on where a workgroup
+// runs.
+template <int NS, typename V>
+__global__ __launch_bounds__(kTileBlock) void k_spmv_tiled(DevOp<V> op, const double* __restrict__ omega,
+                                                         const cvec<V>* __restrict__ x,
+                                                         const cvec<V>* __restrict__ b,
+                                                         cvec<V>* __restrict__ y) {
+    static_assert(wave_gather_width<NS>(), "couplings shared among a row's lanes (widths 8-32)");
+    constexpr int H = kSlice / NS;
+    constexpr int U = 4;
+    const DevTiles& T = op.tiles[width_index(NS)];
+    const int per = (T.ntiles + kXcds - 1) / kXcds;
+    const int t = int(blockIdx.x % kXcds) * per + int(blockIdx.x / kXcds);
+    if (t >= T.ntiles) return;  // the whole workgroup
+    unsigned char* lds = dyn_lds();
+    V* ltab = reinterpret_cast<V*>(lds);
+    cvec<V>* xs = reinterpret_cast<cvec<V>*>(lds + tile_table_bytes<V>(op.ntab));
+    const int tid = threadIdx.x, lane = tid & 63, sys = lane % NS, l = lane / NS, gl = lane - sys;
+    const V w = V(omega[sys]);
+    const int lb = T.lbits;
+    const unsigned lm = (1u << lb) - 1u;
+    const int c1 = T.slice_off[t + 1];
+    int c = T.slice_off[t] + (tid >> 6);
+    // The first slice's loads, which do not depend on x: descriptor, rows,
+    // diagonal pairs and the first two chunks of couplings.
+    int4 d = make_int4(0, 0, 0, 0);
+    int2 so = make_int2(-1, 0);
+    V dr = 0, dm = 0;
+    unsigned mine0 = 0u, mine1 = 0u;
+    auto slice_loads = [&](int cc) {
+        d = T.slc[__builtin_amdgcn_readfirstlane(cc)];
+        so = T.slot[__builtin_amdgcn_readfirstlane(d.z) + l];
+        if (so.x >= 0) {
+            dr = op.diag_s[so.x];
+            dm = op.me[so.x];
+        }
+        mine0 = sys < d.y ? T.words[d.x + sys * H + l] : 0u;
+        mine1 = NS + sys < d.y ? T.words[d.x + (NS + sys) * H + l] : 0u;
+    };
+    if (c < c1) slice_loads(c);
+    for (int i = tid; i < op.ntab; i += kTileBlock) ltab[i] = op.vtab[i];
+    const int s0 = T.stage_off[t];
+    stage_tile<NS, V>(T, s0, T.stage_off[t + 1] - s0, x, xs);
+    __syncthreads();
+    for (; c < c1; c += kTileWaves) {
+        const int off = __builtin_amdgcn_readfirstlane(d.x), wd = __builtin_amdgcn_readfirstlane(d.y);
+        V ar = 0, ai = 0;
+        for (int k0 = 0; k0 < wd; k0 += NS) {
+            const unsigned mine = k0 == 0        ? mine0
+                                  : k0 == NS     ? mine1
+                                  : k0 + sys < wd ? T.words[off + (k0 + sys) * H + l]
+                                                  : 0u;
+            const int n = min(NS, wd - k0);
+            for (int j = 0; j < n; j += U) {
+                unsigned q[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) q[u] = lane_get(mine, gl + j + u);
+                cvec<V> xv[U];
+                V a[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    xv[u].x = V(0);
+                    xv[u].y = V(0);
+                    a[u] = V(0);
+                    if (j + u < n) {
+                        xv[u] = xs[(q[u] & lm) * NS + sys];
+                        a[u] = ltab[q[u] >> lb];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ar += a[u] * xv[u].x;
+                    ai += a[u] * xv[u].y;
+                }
+            }
+        }
+        if (so.x >= 0) {
+            const V di = w * dm;
+            const cvec<V> xi = xs[so.y * NS + sys];
+            cvec<V> out;
+            out.x = ar + (dr * xi.x - di * xi.y);
+            out.y = ai + (dr * xi.y + di * xi.x);
+            const size_t i = size_t(so.x) * NS + sys;
+            if (b) {
+                const cvec<V> bi = b[i];
+                out.x = bi.x - out.x;
+                out.y = bi.y - out.y;
+            }
+            y[i] = out;
+        }
+        // The next slice of this wavefront (none when the tile holds at most
+        // one slice per wavefront, as the tiles are built).
+        if (c + kTileWaves < c1) slice_loads(c + kTileWaves);
+    }
+}
+
 // ||A x - b|| / ||b|| in FP64 (forward_solver_fv.cpp:843-848, 786-790); with
 // r != nullptr also stores r = b - A x in FP64 (iterative refinement).
 template <int NS>
@@ -1820,6 +1965,16 @@ void Launch<NS, V>::bicg_xr(hipStream_t s, int n, cvec<V>* x, const cvec<V>* ph,
 template <int NS, typename V>
 void Launch<NS, V>::spmv(hipStream_t s, const DevOp<V>& op, const double* omega, const cvec<V>* x,
                          const cvec<V>* b, cvec<V>* y) {
+    if constexpr (wave_gather_width<NS>()) {
+        const DevTiles& T = op.tiles[width_index(NS)];
+        if (T.ntiles > 0 && op.vtab && op.shape[width_index(NS)].tiled) {
+            const int per = (T.ntiles + kXcds - 1) / kXcds;
+            const size_t lds = tile_table_bytes<V>(op.ntab) + size_t(T.max_stage) * NS * sizeof(cvec<V>);
+            hipLaunchKernelGGL((k_spmv_tiled<NS, V>), dim3(per * kXcds), dim3(kTileBlock), lds, s, op, omega, x,
+                               b, y);
+            return;
+        }
+    }
     if (lanes_on<NS>(op)) {
         hipLaunchKernelGGL((k_spmv<NS, V, wave_gather_width<NS>()>), dim3(row_grid<NS>(op.n, kStreamCap)),
                            dim3(kBlock), lanes_lds<NS>(op), s, op, omega, x, b, y);

@@ -203,6 +203,27 @@ bool packed_sell() {
     return !(v && v[0] == '0');
 }
 
+// Row tiles for the LDS-staged plain product at widths 8-32 unless
+// M3D_TILED_SPMV=0 (read at each operator build).
+bool tiled_spmv() {
+    const char* v = std::getenv("M3D_TILED_SPMV");
+    return !(v && v[0] == '0');
+}
+
+// LDS a tile may stage per workgroup (the value table takes the rest of
+// 64 KB); M3D_TILE_STAGE_BYTES (8192-64512) sets another budget.
+int tile_stage_bytes() {
+    const char* v = std::getenv("M3D_TILE_STAGE_BYTES");
+    const int b = v ? std::atoi(v) : 63 * 1024;
+    return std::min(63 * 1024, std::max(8192, b));
+}
+
+// Entries of every device row in the order of its SELL blocks (lower
+// couplings, then upper): columns as device rows, values as table indices.
+struct RowEntries {
+    std::vector<int> ptr, col, tab;
+};
+
 // Spatial row-block order for the plain SpMV (M3D_SPMV_ORDER=0: device
 // order).  The order changes which rows run together, never a row's values.
 bool spmv_order() {
@@ -269,6 +290,14 @@ struct m3d_operator {
     bool last_upper_empty = false;
     // Row blocks per batch width in spatial order (empty: device order).
     DBuf<int> order[kWidths];
+    // Row tiles of the LDS-staged plain product per batch width.
+    struct Tiles {
+        int ntiles = 0, lbits = 0, max_stage = 0;
+        DBuf<int> stage_off, stage_rows, slice_off;
+        DBuf<int4> slc;
+        DBuf<int2> slot;
+        DBuf<unsigned> words;
+    } tiles[kWidths];
     DBuf<unsigned char> bc, constrained;
     std::vector<unsigned char> h_constrained;
     bool mass_set = false;
@@ -302,6 +331,20 @@ struct m3d_operator {
 
     template <typename V>
     DevOp<V> view() const;
+    void set_tiles(DevTiles* t) const {
+        for (int i = 0; i < kWidths; ++i) {
+            const Tiles& h = tiles[i];
+            t[i].ntiles = h.ntiles;
+            t[i].lbits = h.lbits;
+            t[i].max_stage = h.max_stage;
+            t[i].stage_off = h.stage_off.p;
+            t[i].stage_rows = h.stage_rows.p;
+            t[i].slice_off = h.slice_off.p;
+            t[i].slc = h.slc.p;
+            t[i].slot = h.slot.p;
+            t[i].words = h.words.p;
+        }
+    }
 };
 
 template <>
@@ -314,6 +357,7 @@ DevOp<double> m3d_operator::view<double>() const {
     o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab.p : nullptr; o.cbits = cbits; o.ntab = ntab;
     o.diag_s = diag_s.p; o.me = me.p;
     for (int i = 0; i < kWidths; ++i) o.order[i] = order[i].p;
+    set_tiles(o.tiles);
     return o;
 }
 template <>
@@ -326,6 +370,7 @@ DevOp<float> m3d_operator::view<float>() const {
     o.l_pk = l_pk.p; o.u_pk = u_pk.p; o.vtab = ntab ? vtab32.p : nullptr; o.cbits = cbits; o.ntab = ntab;
     o.diag_s = diag_s32.p; o.me = me32.p;
     for (int i = 0; i < kWidths; ++i) o.order[i] = order[i].p;
+    set_tiles(o.tiles);
     return o;
 }
 
@@ -1151,7 +1196,94 @@ LaunchShape default_shape(int width) {
     // an FP64 BiCGStab iteration -3 % at 8 systems and -5 % at 16, a complex64
     // one -3 % and -8 % on the same mesh.
     sh.wave = width >= 8 ? 1 : 0;
+    // The plain product over LDS-staged row tiles where the operator has them.
+    sh.tiled = width >= 8 ? 1 : 0;
     return sh;
+}
+
+// Row tiles of the LDS-staged plain product at batch width 1 << wi (the
+// layout DevTiles describes): runs of rows in the caller's edge order, then
+// the padding rows, each tile closed before the distinct rows it stages (its
+// rows and the rows their couplings read) would exceed max_stage, or when it
+// holds max_slices slices of H rows (one per wavefront of the workgroup).
+void build_tiles(m3d_operator& op, const RowEntries& E, int wi, int max_stage, int max_slices, hipStream_t s) {
+    const int H = kSlice >> wi;
+    const size_t max_rows = size_t(max_slices) * H;
+    int lbits = 1;
+    while ((1 << lbits) < max_stage) ++lbits;
+    std::vector<int> stage_off{0}, stage_rows, slice_off{0};
+    std::vector<int4> slc;
+    std::vector<int2> slot;
+    std::vector<unsigned> words;
+    std::vector<int> mark(op.npad, -1), local(op.npad, 0);
+    std::vector<int> rows, st;  // the open tile's rows and staged rows
+    int tile = 0, most = 0;
+    auto close = [&]() {
+        if (rows.empty()) return;
+        std::sort(st.begin(), st.end());
+        for (size_t i = 0; i < st.size(); ++i) local[st[i]] = int(i);
+        stage_rows.insert(stage_rows.end(), st.begin(), st.end());
+        stage_off.push_back(int(stage_rows.size()));
+        most = std::max(most, int(st.size()));
+        // Longest rows first, so the rows of a slice have similar widths.
+        auto len = [&](int r) { return E.ptr[r + 1] - E.ptr[r]; };
+        std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return len(a) > len(b); });
+        for (size_t r0 = 0; r0 < rows.size(); r0 += H) {
+            const size_t r1 = std::min(rows.size(), r0 + H);
+            int W = 0;
+            for (size_t q = r0; q < r1; ++q) W = std::max(W, len(rows[q]));
+            const size_t off = words.size();
+            words.resize(off + size_t(W) * H, 0u);
+            slc.push_back(make_int4(int(off), W, int(slot.size()), int(r1 - r0)));
+            for (int l = 0; l < H; ++l) {
+                if (r0 + l >= r1) {
+                    slot.push_back(make_int2(-1, 0));
+                    continue;
+                }
+                const int r = rows[r0 + l];
+                slot.push_back(make_int2(r, local[r]));
+                for (int k = E.ptr[r]; k < E.ptr[r + 1]; ++k)
+                    words[off + size_t(k - E.ptr[r]) * H + l] =
+                        unsigned(local[E.col[k]]) | (unsigned(E.tab[k]) << lbits);
+            }
+        }
+        slice_off.push_back(int(slc.size()));
+        ++tile;
+        rows.clear();
+        st.clear();
+    };
+    auto add_row = [&](int r) {
+        int fresh = mark[r] != tile ? 1 : 0;
+        for (int k = E.ptr[r]; k < E.ptr[r + 1]; ++k) fresh += mark[E.col[k]] != tile ? 1 : 0;
+        if (!rows.empty() && (int(st.size()) + fresh > max_stage || rows.size() == max_rows)) close();
+        auto stage = [&](int j) {
+            if (mark[j] != tile) {
+                mark[j] = tile;
+                st.push_back(j);
+            }
+        };
+        stage(r);
+        for (int k = E.ptr[r]; k < E.ptr[r + 1]; ++k) stage(E.col[k]);
+        rows.push_back(r);
+    };
+    for (int e = 0; e < op.n; ++e) add_row(op.iperm[e]);
+    close();
+    for (int r = 0; r < op.npad; ++r)
+        if (op.perm[r] < 0) add_row(r);
+    close();
+    if (words.size() > size_t(std::numeric_limits<int>::max()))
+        throw Fail(M3D_ERR_ARG, "operator too large for int32 tile offsets");
+    auto& T = op.tiles[wi];
+    T.stage_off.upload(stage_off, s);
+    T.stage_rows.upload(stage_rows, s);
+    T.slice_off.upload(slice_off, s);
+    T.slc.upload(slc, s);
+    T.slot.upload(slot, s);
+    T.words.upload(words, s);
+    T.ntiles = tile;
+    T.lbits = lbits;
+    T.max_stage = most;
+    HIPCHK(hipStreamSynchronize(s));
 }
 
 void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
@@ -1334,6 +1466,25 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
             op.vtab32.upload(tab32, s);
             op.cbits = cbits;
             op.ntab = int(tab.size());
+            if (tiled_spmv()) {
+                RowEntries E;
+                E.ptr.assign(op.npad + 1, 0);
+                E.col.reserve(size_t(nz));
+                E.tab.reserve(size_t(nz));
+                for (int r = 0; r < op.npad; ++r) {
+                    const int e = op.perm[r];
+                    for (int upper = 0; e >= 0 && upper < 2; ++upper)
+                        for (int k = d.rowptr[e]; k < d.rowptr[e + 1]; ++k) {
+                            const int j = op.iperm[d.colidx[k]];
+                            if ((j < r) == (upper == 1)) continue;
+                            E.col.push_back(j);
+                            E.tab.push_back(tix.find(bits_of(d.values[k])));
+                        }
+                    E.ptr[r + 1] = int(E.col.size());
+                }
+                for (int wi = width_index(8); wi < kWidths; ++wi)
+                    build_tiles(op, E, wi, tile_stage_bytes() / (int(sizeof(double2)) << wi), kTileWaves, s);
+            }
         }
     }
     // The separate blocks stay resident beside the packed ones: the launch
@@ -1404,7 +1555,9 @@ size_t work64_bytes(const Work64<NS>* W) {
 }
 
 size_t operator_bytes(const m3d_operator& op) {
-    return nbytes(op.d_perm, op.l_col, op.u_col, op.hang, op.slc, op.l_val, op.u_val, op.diag_s, op.me,
+    size_t tiles = 0;
+    for (const auto& T : op.tiles) tiles += nbytes(T.stage_off, T.stage_rows, T.slice_off, T.slc, T.slot, T.words);
+    return tiles + nbytes(op.d_perm, op.l_col, op.u_col, op.hang, op.slc, op.l_val, op.u_val, op.diag_s, op.me,
                   op.l_val32, op.u_val32, op.diag_s32, op.me32, op.l_pk, op.u_pk, op.vtab, op.vtab32, op.bc,
                   op.constrained, op.nt_ptr, op.nt_edge, op.g_col, op.l_ptr, op.l_colidx, op.nt_val,
                   op.g_val, op.ms, op.l_vals, op.l_diag, op.dc_bc_edge, op.dc_dead, op.dc_bc_node,
@@ -2043,7 +2196,9 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     const double m = double(op.nnz), n = double(nfree), k = double(nsys);
     // Algorithmic bytes: values + columns of the m off-diagonal nonzeros, the
     // (S_ii, Me_i) diagonal pair, and each vector touched once.
-    const bool pk = op.ntab && op.shape[kind == 3 ? 1 : 0][width_index(NS)].packed;
+    // One 4-byte word per coupling in the packed blocks and in the row tiles.
+    const LaunchShape& shp = op.shape[kind == 3 ? 1 : 0][width_index(NS)];
+    const bool pk = op.ntab && (shp.packed || (kind == 0 && shp.tiled && op.tiles[width_index(NS)].ntiles));
     auto ent = [&](double wv) { return pk ? 4.0 : wv + 4.0; };
     auto spmv_b = [&](double wv, double wc) { return m * ent(wv) + 2.0 * n * wv + 2.0 * k * n * wc; };
     auto ssor_b = [&](double wv, double wc) { return m * ent(wv) + 4.0 * n * wv + 4.0 * k * n * wc; };
@@ -2072,7 +2227,7 @@ bool valid_shape(int width, const LaunchShape& sh) {
     return sub_ok && sh.sweep_cap >= kXcds && sh.sweep_cap <= 65536 && sh.sweep_cap % kXcds == 0 &&
            sh.row_cap >= kXcds && sh.row_cap <= kMaxBlocks && sh.row_cap % kXcds == 0 &&
            (ch == 1 || ch == 2 || ch == 4 || ch == 8 || ch == 16) && (sh.packed == 0 || sh.packed == 1) &&
-           (sh.wave == 0 || (sh.wave == 1 && width >= 8));
+           (sh.wave == 0 || (sh.wave == 1 && width >= 8)) && (sh.tiled == 0 || (sh.tiled == 1 && width >= 8));
 }
 
 int width_slot(int width) {
@@ -2250,6 +2405,7 @@ int m3d_operator_get_shape(m3d_operator* op, int precision, int width, m3d_launc
         out->sweep_chunk = sh.sweep_chunk;
         out->packed = sh.packed;
         out->wave = sh.wave;
+        out->tiled = sh.tiled;
     });
 }
 
@@ -2265,6 +2421,7 @@ int m3d_operator_set_shape(m3d_operator* op, int precision, int width, const m3d
         sh.sweep_chunk = shape->sweep_chunk;
         sh.packed = shape->packed;
         sh.wave = shape->wave;
+        sh.tiled = shape->tiled;
         if (!valid_shape(width, sh)) throw Fail(M3D_ERR_ARG, "invalid launch shape for this width");
         HIPCHK(hipSetDevice(op->ctx->device));
         HIPCHK(hipStreamSynchronize(op->ctx->stream));

@@ -126,6 +126,7 @@ class LaunchShapeC(C.Structure):
         ("sweep_chunk", C.c_int),
         ("packed", C.c_int),
         ("wave", C.c_int),
+        ("tiled", C.c_int),
     ]
 
 
@@ -432,15 +433,17 @@ class DeviceOperator:
         sh = LaunchShapeC()
         _check_dev(lib().m3d_operator_get_shape(self.handle, precision, width, C.byref(sh)))
         return {"sweep_sub": sh.sweep_sub, "sweep_cap": sh.sweep_cap, "row_cap": sh.row_cap,
-                "sweep_chunk": sh.sweep_chunk, "packed": sh.packed, "wave": sh.wave}
+                "sweep_chunk": sh.sweep_chunk, "packed": sh.packed, "wave": sh.wave, "tiled": sh.tiled}
 
     def set_shape(self, precision: int, width: int, sweep_sub: int, sweep_cap: int, row_cap: int,
-                  sweep_chunk: int = 1, packed: Optional[int] = None, wave: Optional[int] = None) -> None:
-        """packed=None / wave=None keep the width's current choice."""
+                  sweep_chunk: int = 1, packed: Optional[int] = None, wave: Optional[int] = None,
+                  tiled: Optional[int] = None) -> None:
+        """packed=None / wave=None / tiled=None keep the width's current choice."""
         cur = self.get_shape(precision, width)
         packed = cur["packed"] if packed is None else packed
         wave = cur["wave"] if wave is None else wave
-        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap, sweep_chunk, packed, wave)
+        tiled = cur["tiled"] if tiled is None else tiled
+        sh = LaunchShapeC(sweep_sub, sweep_cap, row_cap, sweep_chunk, packed, wave, tiled)
         _check_dev(lib().m3d_operator_set_shape(self.handle, precision, width, C.byref(sh)))
 
     def autotune(self, omegas, widths=(1, 2, 4, 8, 16), reps: int = 16) -> dict:
