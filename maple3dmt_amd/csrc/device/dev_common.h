@@ -112,8 +112,10 @@ __host__ __device__ constexpr int width_index(int ns) {
 // column-major (words[slc.x + k * H + l], k < slc.y): the local index in the
 // low lbits bits and the index of the value in the operator's value table
 // above them, word 0 for padding (value 0); each row keeps the entry order of
-// the SELL blocks (lower couplings, then upper).  slot[slc.z + l] = {device
-// row, local index of that row} or {-1, 0} for a padding slot.
+// the SELL blocks (lower couplings, then upper).  Every slice has H slots,
+// slot[c * H + l] (= slot[slc.z + l]) = {device row, local index of that
+// row} or {-1, 0} for a padding slot, so a slice's rows load without waiting
+// for its descriptor.
 struct DevTiles {
     int ntiles = 0;     // 0: no tiles at this width
     int lbits = 0;

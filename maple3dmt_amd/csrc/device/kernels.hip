@@ -977,7 +977,7 @@ __global__ __launch_bounds__(kTileBlock) void k_spmv_tiled(DevOp<V> op, const do
     unsigned mine0 = 0u, mine1 = 0u;
     auto slice_loads = [&](int cc) {
         d = T.slc[__builtin_amdgcn_readfirstlane(cc)];
-        so = T.slot[__builtin_amdgcn_readfirstlane(d.z) + l];
+        so = T.slot[cc * H + l];
         if (so.x >= 0) {
             dr = op.diag_s[so.x];
             dm = op.me[so.x];
