@@ -36,10 +36,14 @@ def _default_wave(w):
     return 1 if w >= 8 else 0
 
 
+def _default_tiled(w):
+    return 1 if w >= 8 else 0
+
+
 def _reset(dev):
     for prec in (PREC_FP64, PREC_MIXED):
         for w in (1, 2, 4, 8, 16, 32):
-            dev.set_shape(prec, w, 1, 4096, 2048, 1, _default_packed(w), _default_wave(w))
+            dev.set_shape(prec, w, 1, 4096, 2048, 1, _default_packed(w), _default_wave(w), _default_tiled(w))
 
 
 def test_shape_defaults_and_validation(block):
@@ -49,7 +53,7 @@ def test_shape_defaults_and_validation(block):
         for w in (1, 2, 4, 8, 16, 32):
             assert dev.get_shape(prec, w) == {"sweep_sub": 1, "sweep_cap": 4096, "row_cap": 2048,
                                               "sweep_chunk": 1, "packed": _default_packed(w),
-                                              "wave": _default_wave(w)}
+                                              "wave": _default_wave(w), "tiled": _default_tiled(w)}
     for bad in ((3, 4096, 2048), (8, 4096, 2048), (1, 100, 2048), (1, 4096, 4096), (1, 0, 2048)):
         with pytest.raises(NativeError):
             dev.set_shape(PREC_FP64, 8, *bad)   # width 8: sub 8 gives 64 lanes per row
@@ -61,9 +65,11 @@ def test_shape_defaults_and_validation(block):
         dev.set_shape(PREC_FP64, 8, 1, 4096, 2048, 1, 2)   # packed 2
     with pytest.raises(NativeError):
         dev.set_shape(PREC_FP64, 4, 1, 4096, 2048, 1, 1, 1)   # wave below width 8
+    with pytest.raises(NativeError):
+        dev.set_shape(PREC_FP64, 4, 1, 4096, 2048, 1, 1, 0, 1)   # tiles below width 8
     dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4, 0, 0)
     assert dev.get_shape(PREC_FP64, 8) == {"sweep_sub": 4, "sweep_cap": 1024, "row_cap": 512,
-                                           "sweep_chunk": 4, "packed": 0, "wave": 0}
+                                           "sweep_chunk": 4, "packed": 0, "wave": 0, "tiled": 1}
     dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4, 0, 1)
     assert dev.get_shape(PREC_FP64, 8)["wave"] == 1
     dev.set_shape(PREC_FP64, 8, 4, 1024, 512, 4, 1)
