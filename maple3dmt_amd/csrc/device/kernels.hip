@@ -904,6 +904,7 @@ __host__ __device__ constexpr size_t tile_table_bytes(int ntab) {
 }
 constexpr int kTileBlock = kTileWaves * 64;  // threads of a tiled-product workgroup (two per CU by LDS)
 constexpr int kTileLoads = 4;     // staging loads in flight per thread
+constexpr int kTileGroup = 8;     // consecutive tiles one XCD takes from a window
 
 // Copies the x rows a tile stages into LDS: element i (row i / NS, system
 // i % NS) of the staged block, kTileLoads per thread in flight (all row
@@ -944,10 +945,16 @@ __device__ inline void stage_tile(const DevTiles& T, int s0, int ns, const cvec<
 // A wavefront loads its first slice's descriptor, rows, diagonal pairs and
 // couplings before the staging completes.  A row adds its entries in the
 // order of its SELL blocks and its padding adds 0 * 0, so y is k_spmv's bit
-// for bit.  Tiles go out in caller order, a contiguous eighth per XCD
-// (workgroup i runs on XCD i % 8), so neighbouring tiles, which stage
-// overlapping rows, share an L2; results do not depend on where a workgroup
-// runs.
+// for bit.  Tiles go out in caller order in windows of kXcds x kTileGroup
+// tiles, each XCD taking kTileGroup consecutive tiles of a window (workgroup
+// i runs on XCD i % 8): neighbouring tiles, which stage overlapping rows,
+// share an L2, and all XCDs work on one window of the mesh, whose rows the
+// memory-side Infinity Cache keeps for the tiles that stage them again.
+// Measured on a 1.45 M-edge mesh at 16 systems: 0.315 ms, against 0.331 ms
+// for a contiguous eighth of the tiles per XCD (less fetched beyond L2, 598
+// against 869 MB, but each XCD on its own region) and 0.319 ms for single
+// tiles dealt round robin (1287 MB).  Results do not depend on where a
+// workgroup runs.
 template <int NS, typename V>
 __global__ __launch_bounds__(kTileBlock) void k_spmv_tiled(DevOp<V> op, const double* __restrict__ omega,
                                                          const cvec<V>* __restrict__ x,
@@ -957,8 +964,9 @@ __global__ __launch_bounds__(kTileBlock) void k_spmv_tiled(DevOp<V> op, const do
     constexpr int H = kSlice / NS;
     constexpr int U = 4;
     const DevTiles& T = op.tiles[width_index(NS)];
-    const int per = (T.ntiles + kXcds - 1) / kXcds;
-    const int t = int(blockIdx.x % kXcds) * per + int(blockIdx.x / kXcds);
+    constexpr int W = kXcds * kTileGroup;  // tiles per window
+    const int wg = int(blockIdx.x);
+    const int t = wg / W * W + wg % kXcds * kTileGroup + wg / kXcds % kTileGroup;
     if (t >= T.ntiles) return;  // the whole workgroup
     unsigned char* lds = dyn_lds();
     V* ltab = reinterpret_cast<V*>(lds);
@@ -1968,9 +1976,10 @@ void Launch<NS, V>::spmv(hipStream_t s, const DevOp<V>& op, const double* omega,
     if constexpr (wave_gather_width<NS>()) {
         const DevTiles& T = op.tiles[width_index(NS)];
         if (T.ntiles > 0 && op.vtab && op.shape[width_index(NS)].tiled) {
-            const int per = (T.ntiles + kXcds - 1) / kXcds;
+            constexpr int W = kXcds * kTileGroup;
             const size_t lds = tile_table_bytes<V>(op.ntab) + size_t(T.max_stage) * NS * sizeof(cvec<V>);
-            hipLaunchKernelGGL((k_spmv_tiled<NS, V>), dim3(per * kXcds), dim3(kTileBlock), lds, s, op, omega, x,
+            hipLaunchKernelGGL((k_spmv_tiled<NS, V>), dim3((T.ntiles + W - 1) / W * W), dim3(kTileBlock), lds, s,
+                               op, omega, x,
                                b, y);
             return;
         }
