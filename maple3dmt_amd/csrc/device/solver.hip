@@ -210,12 +210,15 @@ bool tiled_spmv() {
     return !(v && v[0] == '0');
 }
 
-// LDS a tile may stage per workgroup (the value table takes the rest of
-// 64 KB); M3D_TILE_STAGE_BYTES (8192-64512) sets another budget.
-int tile_stage_bytes() {
+// LDS a tile may stage per workgroup: 63 KB, or M3D_TILE_STAGE_BYTES
+// (8192-64512), and never more than the 64 KB a workgroup's launch may ask
+// for beside the value table of ntab doubles (16-byte aligned, as the
+// kernels lay it out).
+int tile_stage_bytes(int ntab) {
     const char* v = std::getenv("M3D_TILE_STAGE_BYTES");
-    const int b = v ? std::atoi(v) : 63 * 1024;
-    return std::min(63 * 1024, std::max(8192, b));
+    const int b = std::min(63 * 1024, std::max(8192, v ? std::atoi(v) : 63 * 1024));
+    const int table = (ntab * int(sizeof(double)) + 15) / 16 * 16;
+    return std::min(b, 64 * 1024 - table);
 }
 
 // Entries of every device row in the order of its SELL blocks (lower
@@ -1483,7 +1486,7 @@ void build_operator(m3d_operator& op, const m3d_operator_desc& d) {
                     E.ptr[r + 1] = int(E.col.size());
                 }
                 for (int wi = width_index(8); wi < kWidths; ++wi)
-                    build_tiles(op, E, wi, tile_stage_bytes() / (int(sizeof(double2)) << wi), kTileWaves, s);
+                    build_tiles(op, E, wi, tile_stage_bytes(op.ntab) / (int(sizeof(double2)) << wi), kTileWaves, s);
             }
         }
     }
