@@ -158,18 +158,30 @@ struct DevOp {
     DevTiles tiles[kWidths];
 };
 
-// Row-block schedule: a plain grid stride over the row blocks.  The
-// dispatcher deals workgroups to the eight XCDs round robin, so all XCDs share
-// one moving window of rows, which the memory-side Infinity Cache serves.  A
-// schedule giving each XCD a contiguous eighth of the rows (one L2 per region)
-// measured slower on every solver kernel (a BiCGStab iteration 0.65 vs
-// 0.56 ms at one system, SpMV 0.78 vs 0.68 ms at 16).
+// Row-block schedule: a grid stride over the row blocks, whose workgroups
+// take the row blocks of a stride in windows: each full group of 64
+// workgroups takes 64 consecutive row blocks, eight consecutive ones per XCD
+// (the dispatcher deals workgroup i to XCD i % 8), so neighbouring rows
+// share an XCD's L2 while all XCDs work on one moving window of rows, which
+// the memory-side Infinity Cache serves.  window_block() is the workgroup of
+// the plain grid stride whose row blocks this one takes (a bijection on the
+// grid); a workgroup's block partials go to that slot, so every partial holds
+// the rows it would hold in the plain grid stride, added in the same order,
+// and no sum changes.  Measured on a 1.45 M-edge mesh against the plain grid
+// stride: the width-2 backward sweep 10.46 -> 10.00 us per launch and a
+// BiCGStab iteration 1-3.5 % shorter at every width.  A schedule giving each
+// XCD a contiguous eighth of the rows (one L2 per region) measured slower on
+// every solver kernel (a BiCGStab iteration 0.65 vs 0.56 ms at one system).
 struct RowRange {
     int first, end, step;
 };
+__device__ inline int window_block() {
+    const int b = blockIdx.x, g64 = int(gridDim.x) / 64 * 64;
+    return b < g64 ? b / 64 * 64 + b % kXcds * 8 + b / kXcds % 8 : b;
+}
 __device__ inline RowRange row_range(int nrb) {
     RowRange r;
-    r.first = blockIdx.x;
+    r.first = window_block();
     r.end = nrb;
     r.step = gridDim.x;
     return r;
@@ -189,24 +201,11 @@ __device__ inline RowRange xcd_part(int nrb) {
     return r;
 }
 
-// Row blocks of a sweep workgroup: the grid stride, or `chunk` consecutive
-// row blocks per workgroup (the grid then holds ceil(nrb / chunk) workgroups).
-// In the grid stride, each full group of 64 workgroups takes 64 consecutive
-// row blocks of a stride, eight consecutive ones per XCD (workgroup i runs
-// on XCD i % 8): neighbouring rows share an XCD's L2 while every XCD works on
-// one window of rows.  A permutation of the workgroups (the sweeps reduce
-// nothing, so no sum changes): on a 1.45 M-edge mesh the width-2 backward
-// sweep 10.46 -> 10.00 us per launch and a BiCGStab iteration 1-3.5 % shorter
-// at every width, against the plain grid stride.
+// Row blocks of a sweep workgroup: the windowed grid stride, or `chunk`
+// consecutive row blocks per workgroup (the grid then holds
+// ceil(nrb / chunk) workgroups).
 __device__ inline RowRange row_range_chunk(int nrb, int chunk) {
-    if (chunk <= 1) {
-        const int b = blockIdx.x, g64 = int(gridDim.x) / 64 * 64;
-        RowRange r;
-        r.first = b < g64 ? b / 64 * 64 + b % kXcds * 8 + b / kXcds % 8 : b;
-        r.end = nrb;
-        r.step = gridDim.x;
-        return r;
-    }
+    if (chunk <= 1) return row_range(nrb);
     RowRange r;
     r.first = blockIdx.x * chunk;
     r.end = min(nrb, r.first + chunk);
@@ -238,8 +237,9 @@ constexpr int kFinBlock = 1024;  // threads of a finishing kernel
 constexpr size_t kPartSNorm = size_t(3) * kMaxSys * kMaxBlocks;
 
 // Block stage: sums NQ per-thread values over the threads of each system in
-// the block and stores them block-major, part[blockIdx * P + p] with
-// P = NQ * NS pairs, p = q * NS + sys.  All threads must call it.
+// the block and stores them block-major, part[window_block() * P + p] with
+// P = NQ * NS pairs, p = q * NS + sys (the slot of the rows row_range gave
+// the block).  All threads must call it.
 template <int NS, int NQ>
 __device__ inline void block_partials(const double (&v)[NQ], double* part) {
     constexpr int P = NQ * NS;
@@ -256,7 +256,7 @@ __device__ inline void block_partials(const double (&v)[NQ], double* part) {
     if (tid < P) {
         double acc = 0.0;
         for (int w = 0; w < kBlock / 64; ++w) acc += sm[w][tid];
-        part[size_t(blockIdx.x) * P + tid] = acc;
+        part[size_t(window_block()) * P + tid] = acc;
     }
 }
 
