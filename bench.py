@@ -546,6 +546,11 @@ def c3_leg(args):
         lev, rad = sizes[i % len(sizes)]
         wls.append(configs.random3d(300 + i, max_level=lev, n_freq=16, half_width=100e3,
                                     station_radius=rad, maxiter=args.maxiter))
+    # Largest meshes first (finest level, then widest refinement radius):
+    # jobs are dealt to the streams round robin and each stream runs its
+    # jobs in order, so the four largest start together and every stream
+    # ends on small ones.
+    wls.sort(key=lambda wl: (wl.mesh.max_level, wl.mesh.station_refine_radius), reverse=True)
     n_streams = 4
     contexts = [Context(0) for _ in range(n_streams)]
     t = time.time()
