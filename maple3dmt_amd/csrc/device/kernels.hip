@@ -80,13 +80,15 @@ __device__ inline void gather_packed(const unsigned* __restrict__ pk, const V* _
     off = __builtin_amdgcn_readfirstlane(off);
     w = __builtin_amdgcn_readfirstlane(w);
     const unsigned cm = (1u << cbits) - 1u;
-    for (int k = k0; k < w; k += U * STEP) {
-        unsigned wd[U];
+    // The words of the next U entries load together with this batch's
+    // gathers (one memory round trip per batch after the first, not two).
+    unsigned wd[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = k + u * STEP;
-            wd[u] = kk < w ? pk[off + kk * kSlice + lane] : 0u;
-        }
+    for (int u = 0; u < U; ++u) {
+        const int kk = k0 + u * STEP;
+        wd[u] = kk < w ? pk[off + kk * kSlice + lane] : 0u;
+    }
+    for (int k = k0; k < w; k += U * STEP) {
         cvec<V> xv[U];
         V a[U];
 #pragma unroll
@@ -99,11 +101,20 @@ __device__ inline void gather_packed(const unsigned* __restrict__ pk, const V* _
                 a[u] = tab[wd[u] >> cbits];
             }
         }
+        const int kn = k + U * STEP;
+        unsigned nw[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = kn + u * STEP;
+            nw[u] = kk < w ? pk[off + kk * kSlice + lane] : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             ar += a[u] * xv[u].x;
             ai += a[u] * xv[u].y;
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) wd[u] = nw[u];
     }
 }
 
