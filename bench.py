@@ -152,7 +152,7 @@ def kernel_only(args):
     ms4, by4 = model.device.bench_kernel(4, omegas[:args.tail_width], reps=args.reps)
     iters = {}
     for kind, tag in ((2, "fp64"), (3, "c64")):
-        for k in (1, 2, 4, 16):
+        for k in (1, 2, 4, 8, 16):
             t, b = model.device.bench_kernel(kind, omegas[:k], reps=max(1, args.reps // 4))
             iters[f"{tag}_ns{k}"] = {"ms": t, "alg_GBs": b / t / 1e6}
             log(f"[kernel-only] BiCGStab iteration {tag} x{k}: {t:.3f} ms, {b / t / 1e6:.0f} GB/s")
