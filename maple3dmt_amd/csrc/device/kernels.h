@@ -70,21 +70,27 @@ struct Launch64 {
     static void phi0(hipStream_t s, int nn, const int* nt_ptr, const int* nt_edge,
                      const double* nt_val, const unsigned char* dead, const double* omega,
                      const double2* b, double2* phi0);
-    static void dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, double pcg_tol, const int* nt_ptr,
-                        const int* nt_edge, const double* nt_val, const double* ms,
-                        const unsigned char* bc_node, const double* ldiag, const double2* E,
-                        const double2* phi0, double2* r, double2* z, double2* p, double2* phi,
-                        SysState* st, double* part);
-    static void dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
-                      const double2* p, double2* ap, SysState* st, double* part);
-    static void dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
-                          const double2* ap, double2* phi, double2* r, SysState* st, double* part);
+};
+
+// The DivCorr PCG (divcorr_poisson_, forward_solver_fv.cpp:623-722) in value
+// type V: double, or float for the mixed-precision path's interleaved
+// corrections (complex64 node vectors, FP32 Laplacian; reductions in double).
+// The field E and phi0 stay complex128 in both.
+template <int NS, typename V>
+struct LaunchDc {
+    using C = cvec<V>;
+    static void divj(hipStream_t s, int nn, int it, double tol, int cg_max, double pcg_tol, const int* nt_ptr,
+                     const int* nt_edge, const double* nt_val, const double* ms, const unsigned char* bc_node,
+                     const V* ldiag, const double2* E, const double2* phi0, C* r, C* z, C* p, C* phi,
+                     SysState* st, double* part);
+    static void lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const V* lval, const C* p, C* ap,
+                   SysState* st, double* part);
+    static void update(hipStream_t s, int nn, const V* ldiag, const C* p, const C* ap, C* phi, C* r,
+                       SysState* st, double* part);
     // p = D^-1 r + beta p (the Jacobi-preconditioned residual formed in place).
-    static void dc_p(hipStream_t s, int nn, const double* ldiag, const double2* r, double2* p,
-                     const SysState* st);
-    static void dc_correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
-                           const unsigned char* bc_edge, const double2* phi, double2* E,
-                           const SysState* st);
+    static void dir(hipStream_t s, int nn, const V* ldiag, const C* r, C* p, const SysState* st);
+    static void correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
+                        const unsigned char* bc_edge, const C* phi, double2* E, const SysState* st);
 };
 
 void launch_bicg_begin(hipStream_t s, SysState* st, unsigned mask, int maxiter, double tol);
@@ -108,6 +114,8 @@ template <typename V>
 void launch_regroup(hipStream_t s, int rows, int na, int nb, const int* map, cvec<V>* wide,
                     cvec<V>* narrow, bool to_narrow);
 void launch_dc_begin(hipStream_t s, SysState* st, unsigned mask);
+// dst[i] = float(src[i]).
+void launch_to_f32(hipStream_t s, size_t n, const double* src, float* dst);
 
 }  // namespace dev
 }  // namespace m3d

@@ -1380,15 +1380,29 @@ __global__ void k_dc_begin(SysState* st, unsigned mask) {
     st[s].pcg_status = kIdle;
 }
 
+// The DivCorr PCG runs in V: double everywhere, float (complex64 vectors,
+// FP32 Laplacian values) for the interleaved corrections of the mixed
+// precision path.  Reductions accumulate in double in both; with V = double
+// every expression below is the FP64 one, operation for operation.
+template <typename V>
+__device__ inline cvec<V> mkv(V a, V b) {
+    cvec<V> r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+
 // divJ = G^T (Me_sigma E) - phi0, zero on Dirichlet nodes, ||divJ||; PCG start
-// r = divJ, z = r / diag, p = z, phi = 0, rz = (r, z) (:636-677).
-template <int NS>
+// r = divJ, z = r / diag, p = z, phi = 0, rz = (r, z) (:636-677).  divJ is
+// formed in double from the FP64 field; its norm (the DivCorr convergence
+// measure) is that of the double value, the PCG start is divJ rounded to V.
+template <int NS, typename V>
 __global__ void k_dc_divj(int nn, const int* __restrict__ nt_ptr, const int* __restrict__ nt_edge,
                           const double* __restrict__ nt_val, const double* __restrict__ ms,
-                          const unsigned char* __restrict__ bc_node, const double* __restrict__ ldiag,
+                          const unsigned char* __restrict__ bc_node, const V* __restrict__ ldiag,
                           const double2* __restrict__ E, const double2* __restrict__ phi0,
-                          double2* __restrict__ r, double2* __restrict__ z, double2* __restrict__ p,
-                          double2* __restrict__ phi, const SysState* __restrict__ st, double* part) {
+                          cvec<V>* __restrict__ r, cvec<V>* __restrict__ z, cvec<V>* __restrict__ p,
+                          cvec<V>* __restrict__ phi, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = (dc_mask<NS>(st, false) >> sys) & 1u;
@@ -1414,14 +1428,15 @@ __global__ void k_dc_divj(int nn, const int* __restrict__ nt_ptr, const int* __r
             a.y -= f.y;
         }
         if (bc_node[v]) a = mk(0.0, 0.0);
-        r[i] = a;
-        const double d = ldiag[v];
-        const double2 zi = (fabs(d) > 1e-30) ? mk(a.x / d, a.y / d) : a;
+        const cvec<V> av = to_v<V>(a);
+        r[i] = av;
+        const V d = ldiag[v];
+        const cvec<V> zi = (fabs(d) > V(1e-30)) ? mkv<V>(av.x / d, av.y / d) : av;
         z[i] = zi;
         p[i] = zi;
-        phi[i] = mk(0.0, 0.0);
+        phi[i] = mkv<V>(V(0), V(0));
         acc[0] += cabs2(a);
-        const double2 rz = cconjmul(a, zi);
+        const double2 rz = cconjmul(to_d<V>(av), to_d<V>(zi));
         acc[1] += rz.x;
         acc[2] += rz.y;
     }
@@ -1458,19 +1473,19 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_divj_fin(int dc_iter, double d
 // entries are added in increasing k, as in k_dc_lp's per-lane loop, so the
 // sums are the same bit for bit.  A lane whose system is not running (run =
 // false) passes its entries on and gathers nothing.
-template <int NS>
-__device__ inline double2 dc_row_lanes(int v, const int* __restrict__ lptr, const int* __restrict__ lcol,
-                                       const double* __restrict__ lval, const double2* __restrict__ p, int sys,
+template <int NS, typename V>
+__device__ inline cvec<V> dc_row_lanes(int v, const int* __restrict__ lptr, const int* __restrict__ lcol,
+                                       const V* __restrict__ lval, const cvec<V>* __restrict__ p, int sys,
                                        bool run) {
     constexpr int U = 8;
     static_assert(NS % U == 0, "a batch stays within the row's lanes");
     const int gl = (threadIdx.x & 63) - sys;
     const int k0 = lptr[v], k1 = lptr[v + 1];
-    double2 a = mk(0.0, 0.0);
+    cvec<V> a = mkv<V>(V(0), V(0));
     for (int kb = k0; kb < k1; kb += NS) {
         const int k = kb + sys;
         int c = 0;
-        double l = 0.0;
+        V l = V(0);
         if (k < k1) {
             c = lcol[k];
             l = lval[k];
@@ -1478,16 +1493,16 @@ __device__ inline double2 dc_row_lanes(int v, const int* __restrict__ lptr, cons
         const int n = min(NS, k1 - kb);
         for (int j = 0; j < n; j += U) {
             int cc[U];
-            double ll[U];
+            V ll[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 cc[u] = lane_get(c, gl + j + u);
                 ll[u] = lane_get(l, gl + j + u);
             }
-            double2 pv[U];
+            cvec<V> pv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                pv[u] = mk(0.0, 0.0);
+                pv[u] = mkv<V>(V(0), V(0));
                 if (run && j + u < n) pv[u] = p[size_t(cc[u]) * NS + sys];
             }
 #pragma unroll
@@ -1504,10 +1519,10 @@ __device__ inline double2 dc_row_lanes(int v, const int* __restrict__ lptr, cons
 
 // Ap = L p, (p, Ap); alpha = rz / pAp (:681-692).  LANES: the entries of a
 // row shared among its lanes (dc_row_lanes).
-template <int NS, bool LANES>
+template <int NS, bool LANES, typename V>
 __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restrict__ lcol,
-                        const double* __restrict__ lval, const double2* __restrict__ p,
-                        double2* __restrict__ ap, const SysState* __restrict__ st, double* part) {
+                        const V* __restrict__ lval, const cvec<V>* __restrict__ p,
+                        cvec<V>* __restrict__ ap, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = (dc_mask<NS>(st, true) >> sys) & 1u;
@@ -1519,26 +1534,26 @@ __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restr
         if (v >= nn || !(run || LANES)) continue;
         const size_t i = size_t(v) * NS + sys;
         if constexpr (LANES) {
-            const double2 a = dc_row_lanes<NS>(v, lptr, lcol, lval, p, sys, run);
+            const cvec<V> a = dc_row_lanes<NS, V>(v, lptr, lcol, lval, p, sys, run);
             if (!run) continue;
             ap[i] = a;
-            const double2 d = cconjmul(p[i], a);
+            const double2 d = cconjmul(to_d<V>(p[i]), to_d<V>(a));
             acc[0] += d.x;
             acc[1] += d.y;
             continue;
         }
         // Entries in increasing k, indices and values of 8 entries loaded
         // before their p entries.
-        double2 a = mk(0.0, 0.0);
+        cvec<V> a = mkv<V>(V(0), V(0));
         const int k1 = lptr[v + 1];
         for (int k = lptr[v]; k < k1; k += 8) {
             int c[8];
-            double l[8];
-            double2 pv[8];
+            V l[8];
+            cvec<V> pv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 c[u] = 0;
-                l[u] = 0.0;
+                l[u] = V(0);
                 if (k + u < k1) {
                     c[u] = lcol[k + u];
                     l[u] = lval[k + u];
@@ -1556,7 +1571,7 @@ __global__ void k_dc_lp(int nn, const int* __restrict__ lptr, const int* __restr
             }
         }
         ap[i] = a;
-        const double2 d = cconjmul(p[i], a);
+        const double2 d = cconjmul(to_d<V>(p[i]), to_d<V>(a));
         acc[0] += d.x;
         acc[1] += d.y;
     }
@@ -1580,14 +1595,14 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_lp_fin(SysState* st, const dou
 
 // phi += alpha p, r -= alpha Ap, ||r||; z = r / diag, (r, z);
 // stop when ||r|| < 1e-10 ||divJ||, else beta = rz_new / (rz + 1e-30) (:693-711).
-template <int NS>
-__global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const double2* __restrict__ p,
-                            const double2* __restrict__ ap, double2* __restrict__ phi,
-                            double2* __restrict__ r, const SysState* __restrict__ st, double* part) {
+template <int NS, typename V>
+__global__ void k_dc_update(int nn, const V* __restrict__ ldiag, const cvec<V>* __restrict__ p,
+                            const cvec<V>* __restrict__ ap, cvec<V>* __restrict__ phi,
+                            cvec<V>* __restrict__ r, const SysState* __restrict__ st, double* part) {
     const int tid = threadIdx.x, sys = tid % NS;
     const int rpb = rows_per_block<NS>();
     const bool run = (dc_mask<NS>(st, true) >> sys) & 1u;
-    const double2 al = st[sys].pcg_alpha;
+    const V ax = V(st[sys].pcg_alpha.x), ay = V(st[sys].pcg_alpha.y);
     double acc[3] = {0.0, 0.0, 0.0};
     const int nrb = (nn + rpb - 1) / rpb;
     const RowRange R = row_range(nrb);
@@ -1595,22 +1610,24 @@ __global__ void k_dc_update(int nn, const double* __restrict__ ldiag, const doub
         const int v = rb * rpb + tid / NS;
         if (v >= nn || !run) continue;
         const size_t i = size_t(v) * NS + sys;
-        const double2 pi = p[i], api = ap[i];
-        const double2 ap_ = cmul(al, pi), aap = cmul(al, api);
-        double2 f = phi[i];
+        const cvec<V> pi = p[i], api = ap[i];
+        const cvec<V> ap_ = mkv<V>(ax * pi.x - ay * pi.y, ax * pi.y + ay * pi.x);
+        const cvec<V> aap = mkv<V>(ax * api.x - ay * api.y, ax * api.y + ay * api.x);
+        cvec<V> f = phi[i];
         f.x += ap_.x;
         f.y += ap_.y;
         phi[i] = f;
-        double2 ri = r[i];
+        cvec<V> ri = r[i];
         ri.x -= aap.x;
         ri.y -= aap.y;
         r[i] = ri;
         // z = D^-1 r enters the (r, z) product only: k_dc_p forms it again
         // from r with the same division instead of a stored copy.
-        const double d = ldiag[v];
-        const double2 zi = (fabs(d) > 1e-30) ? mk(ri.x / d, ri.y / d) : ri;
-        acc[0] += cabs2(ri);
-        const double2 rz = cconjmul(ri, zi);
+        const V d = ldiag[v];
+        const cvec<V> zi = (fabs(d) > V(1e-30)) ? mkv<V>(ri.x / d, ri.y / d) : ri;
+        const double2 rd = to_d<V>(ri);
+        acc[0] += cabs2(rd);
+        const double2 rz = cconjmul(rd, to_d<V>(zi));
         acc[1] += rz.x;
         acc[2] += rz.y;
     }
@@ -1635,32 +1652,32 @@ __global__ __launch_bounds__(kFinBlock) void k_dc_update_fin(SysState* st, const
     S.rz = rzn;
 }
 
-// p = z + beta p.
-template <int NS>
 // p = z + beta p with z = D^-1 r formed here (the Jacobi preconditioner of
 // k_dc_update, same division).
-__global__ void k_dc_p(int nn, const double* __restrict__ ldiag, const double2* __restrict__ r,
-                       double2* __restrict__ p, const SysState* __restrict__ st) {
+template <int NS, typename V>
+__global__ void k_dc_p(int nn, const V* __restrict__ ldiag, const cvec<V>* __restrict__ r,
+                       cvec<V>* __restrict__ p, const SysState* __restrict__ st) {
     const size_t total = size_t(nn) * NS;
     const unsigned run = dc_mask<NS>(st, true);
     for (size_t t = blockIdx.x * size_t(blockDim.x) + threadIdx.x; t < total;
          t += size_t(gridDim.x) * blockDim.x) {
         const int sys = int(t % NS);
         if (!((run >> sys) & 1u)) continue;
-        const double2 b = st[sys].pcg_beta;
-        const double2 bp = cmul(b, p[t]);
-        const double2 ri = r[t];
-        const double d = ldiag[t / NS];
-        const double2 zi = (fabs(d) > 1e-30) ? mk(ri.x / d, ri.y / d) : ri;
-        p[t] = mk(zi.x + bp.x, zi.y + bp.y);
+        const V bx = V(st[sys].pcg_beta.x), by = V(st[sys].pcg_beta.y);
+        const cvec<V> pt = p[t];
+        const cvec<V> bp = mkv<V>(bx * pt.x - by * pt.y, bx * pt.y + by * pt.x);
+        const cvec<V> ri = r[t];
+        const V d = ldiag[t / NS];
+        const cvec<V> zi = (fabs(d) > V(1e-30)) ? mkv<V>(ri.x / d, ri.y / d) : ri;
+        p[t] = mkv<V>(zi.x + bp.x, zi.y + bp.y);
     }
 }
 
 // E -= G phi on edges outside the DivCorr Dirichlet set (:714-720).
-template <int NS>
+template <int NS, typename V>
 __global__ void k_dc_correct(int npad, const int* __restrict__ g_col, const double* __restrict__ g_val,
                              const unsigned char* __restrict__ bc_edge,
-                             const double2* __restrict__ phi, double2* __restrict__ E,
+                             const cvec<V>* __restrict__ phi, double2* __restrict__ E,
                              const SysState* __restrict__ st) {
     const size_t total = size_t(npad) * NS;
     const unsigned run = dc_mask<NS>(st, false);
@@ -1670,7 +1687,7 @@ __global__ void k_dc_correct(int npad, const int* __restrict__ g_col, const doub
         if (!((run >> sys) & 1u) || bc_edge[row]) continue;
         const int a = g_col[2 * row], b = g_col[2 * row + 1];
         if (a < 0) continue;  // padding row
-        const double2 pa = phi[size_t(a) * NS + sys], pb = phi[size_t(b) * NS + sys];
+        const double2 pa = to_d<V>(phi[size_t(a) * NS + sys]), pb = to_d<V>(phi[size_t(b) * NS + sys]);
         const double ga = g_val[2 * row], gb = g_val[2 * row + 1];
         double2 gr;
         gr.x = ga * pa.x + gb * pb.x;
@@ -2093,50 +2110,56 @@ void Launch64<NS>::phi0(hipStream_t s, int nn, const int* nt_ptr, const int* nt_
     hipLaunchKernelGGL((k_phi0<NS>), dim3(grid_for(size_t(nn) * NS, kStreamCap)), dim3(kBlock), 0, s,
                        nn, nt_ptr, nt_edge, nt_val, dead, omega, b, phi0);
 }
-template <int NS>
-void Launch64<NS>::dc_divj(hipStream_t s, int nn, int it, double tol, int cg_max, double pcg_tol, const int* nt_ptr,
-                           const int* nt_edge, const double* nt_val, const double* ms,
-                           const unsigned char* bc_node, const double* ldiag, const double2* E,
-                           const double2* phi0, double2* r, double2* z, double2* p, double2* phi,
-                           SysState* st, double* part) {
+template <int NS, typename V>
+void LaunchDc<NS, V>::divj(hipStream_t s, int nn, int it, double tol, int cg_max, double pcg_tol,
+                           const int* nt_ptr, const int* nt_edge, const double* nt_val, const double* ms,
+                           const unsigned char* bc_node, const V* ldiag, const double2* E, const double2* phi0,
+                           C* r, C* z, C* p, C* phi, SysState* st, double* part) {
     const int g = row_grid<NS>(nn);
-    hipLaunchKernelGGL((k_dc_divj<NS>), dim3(g), dim3(kBlock), 0, s, nn, nt_ptr, nt_edge, nt_val, ms,
+    hipLaunchKernelGGL((k_dc_divj<NS, V>), dim3(g), dim3(kBlock), 0, s, nn, nt_ptr, nt_edge, nt_val, ms,
                        bc_node, ldiag, E, phi0, r, z, p, phi, st, part);
     hipLaunchKernelGGL((k_dc_divj_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, it, tol, cg_max, pcg_tol, st, part, g);
 }
-template <int NS>
-void Launch64<NS>::dc_lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const double* lval,
-                         const double2* p, double2* ap, SysState* st, double* part) {
+template <int NS, typename V>
+void LaunchDc<NS, V>::lp(hipStream_t s, int nn, const int* lptr, const int* lcol, const V* lval, const C* p,
+                         C* ap, SysState* st, double* part) {
     const int g = row_grid<NS>(nn);
     // Entries shared among a row's lanes from 8 systems (bitwise the same
-    // sums; the 16-system product on a 483 K-node Laplacian 120 -> 108 us).
+    // sums; the 16-system FP64 product on a 483 K-node Laplacian 120 -> 108 us).
     if constexpr (NS >= 8)
-        hipLaunchKernelGGL((k_dc_lp<NS, true>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st,
+        hipLaunchKernelGGL((k_dc_lp<NS, true, V>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st,
                            part);
     else
-        hipLaunchKernelGGL((k_dc_lp<NS, false>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st,
+        hipLaunchKernelGGL((k_dc_lp<NS, false, V>), dim3(g), dim3(kBlock), 0, s, nn, lptr, lcol, lval, p, ap, st,
                            part);
     hipLaunchKernelGGL((k_dc_lp_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
-template <int NS>
-void Launch64<NS>::dc_update(hipStream_t s, int nn, const double* ldiag, const double2* p,
-                             const double2* ap, double2* phi, double2* r, SysState* st, double* part) {
+template <int NS, typename V>
+void LaunchDc<NS, V>::update(hipStream_t s, int nn, const V* ldiag, const C* p, const C* ap, C* phi, C* r,
+                             SysState* st, double* part) {
     const int g = row_grid<NS>(nn);
-    hipLaunchKernelGGL((k_dc_update<NS>), dim3(g), dim3(kBlock), 0, s, nn, ldiag, p, ap, phi, r, st, part);
+    hipLaunchKernelGGL((k_dc_update<NS, V>), dim3(g), dim3(kBlock), 0, s, nn, ldiag, p, ap, phi, r, st, part);
     hipLaunchKernelGGL((k_dc_update_fin<NS>), dim3(1), dim3(kFinBlock), 0, s, st, part, g);
 }
-template <int NS>
-void Launch64<NS>::dc_p(hipStream_t s, int nn, const double* ldiag, const double2* r, double2* p,
-                        const SysState* st) {
-    hipLaunchKernelGGL((k_dc_p<NS>), dim3(grid_for(size_t(nn) * NS, kStreamCap)), dim3(kBlock), 0, s,
+template <int NS, typename V>
+void LaunchDc<NS, V>::dir(hipStream_t s, int nn, const V* ldiag, const C* r, C* p, const SysState* st) {
+    hipLaunchKernelGGL((k_dc_p<NS, V>), dim3(grid_for(size_t(nn) * NS, kStreamCap)), dim3(kBlock), 0, s,
                        nn, ldiag, r, p, st);
 }
-template <int NS>
-void Launch64<NS>::dc_correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
-                              const unsigned char* bc_edge, const double2* phi, double2* E,
-                              const SysState* st) {
-    hipLaunchKernelGGL((k_dc_correct<NS>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
+template <int NS, typename V>
+void LaunchDc<NS, V>::correct(hipStream_t s, int npad, const int* g_col, const double* g_val,
+                              const unsigned char* bc_edge, const C* phi, double2* E, const SysState* st) {
+    hipLaunchKernelGGL((k_dc_correct<NS, V>), dim3(grid_for(size_t(npad) * NS, kStreamCap)),
                        dim3(kBlock), 0, s, npad, g_col, g_val, bc_edge, phi, E, st);
+}
+
+__global__ void k_to_f32(size_t n, const double* __restrict__ src, float* __restrict__ dst) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = float(src[i]);
+}
+void launch_to_f32(hipStream_t s, size_t n, const double* src, float* dst) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, kStreamCap)), dim3(kBlock), 0, s, n, src, dst);
 }
 
 template <typename V>
@@ -2183,7 +2206,9 @@ void launch_dc_begin(hipStream_t s, SysState* st, unsigned mask) {
 #define M3D_INST(NS)                     \
     template struct Launch<NS, double>;  \
     template struct Launch<NS, float>;   \
-    template struct Launch64<NS>;
+    template struct Launch64<NS>;        \
+    template struct LaunchDc<NS, double>; \
+    template struct LaunchDc<NS, float>;
 M3D_INST(1)
 M3D_INST(2)
 M3D_INST(4)

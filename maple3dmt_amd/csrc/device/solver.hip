@@ -19,6 +19,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "abi_common.h"
@@ -310,6 +311,10 @@ struct m3d_operator {
     DBuf<int> nt_ptr, nt_edge, g_col, l_ptr, l_colidx;
     DBuf<double> nt_val, g_val, ms, l_vals, l_diag;
     DBuf<unsigned char> dc_bc_edge, dc_dead, dc_bc_node;
+    // FP32 copies of the Laplacian for the FP32 PCG, made on first use after
+    // the values change (l32_fresh false).
+    DBuf<float> l_vals32, l_diag32;
+    bool l32_fresh = false;
     // Sigma update: edge-side mass structure (caller edge order), the
     // Laplacian entries' edges (device rows) and which edges its pattern holds.
     bool mass_struct = false, sigma_update_ok = false;
@@ -642,6 +647,89 @@ template <int NS>
 void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, unsigned mask,
              const m3d_solve_params& prm);
 
+// The DivCorr steps between the bursts of the mixed-precision path run their
+// PCG in FP32 (complex64 node vectors, FP32 Laplacian, double reductions)
+// when its tolerance is at least 1e-5, where FP32 rounding (6e-8) stays far
+// below the stop; M3D_DIVCORR_FP32=0 (read at each call) keeps them in FP64.
+// The FP64 paths, the post-hoc correction and m3d_divcorr (1e-10) always run
+// in FP64.
+constexpr double kDcFp32MinTol = 1e-5;
+
+bool dc_in_fp32(const m3d_solve_params& prm) {
+    if (prm.precision != M3D_PREC_MIXED || !(prm.divcorr_pcg_tol >= kDcFp32MinTol)) return false;
+    const char* e = getenv("M3D_DIVCORR_FP32");
+    return !(e && e[0] == '0');
+}
+
+void ensure_laplacian32(m3d_operator& op) {
+    if (op.l32_fresh) return;
+    hipStream_t s = op.ctx->stream;
+    // Reallocated only when the sizes change, which only set_divcorr does
+    // (it clears the captured graphs that hold the old addresses).
+    if (op.l_vals32.n != op.l_vals.n) op.l_vals32.alloc(op.l_vals.n);
+    if (op.l_diag32.n != op.l_diag.n) op.l_diag32.alloc(op.l_diag.n);
+    launch_to_f32(s, op.l_vals.n, op.l_vals.p, op.l_vals32.p);
+    launch_to_f32(s, op.l_diag.n, op.l_diag.p, op.l_diag32.p);
+    HIPCHK(hipGetLastError());
+    op.l32_fresh = true;
+}
+
+// The DivCorr loop of divcorr_poisson_ (forward_solver_fv.cpp:623-722) with
+// its PCG in V; the node vectors of W hold cvec<V> (float2 in the first half
+// of each complex128 buffer for V = float).
+template <int NS, typename V>
+void divcorr_pcg(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, unsigned mask,
+                 const m3d_solve_params& prm, const V* lval, const V* ldiag) {
+    hipStream_t s = op.ctx->stream;
+    using D = LaunchDc<NS, V>;
+    using C = cvec<V>;
+    C* dr = reinterpret_cast<C*>(W.dr.p);
+    C* dz = reinterpret_cast<C*>(W.dz.p);
+    C* dp = reinterpret_cast<C*>(W.dp.p);
+    C* dphi = reinterpret_cast<C*>(W.dphi.p);
+    C* dap = reinterpret_cast<C*>(W.dap.p);
+    const int cg_max = std::max(500, op.nn);
+    const double pcg_tol = prm.divcorr_pcg_tol > 0 ? prm.divcorr_pcg_tol : 1e-10;
+    launch_dc_begin(s, W.st.p, mask);
+    for (int it = 0; it < prm.div_corr_iters; ++it) {
+        D::divj(s, op.nn, it, prm.div_corr_tol, cg_max, pcg_tol, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p,
+                op.ms.p, op.dc_bc_node.p, ldiag, E, phi0, dr, dz, dp, dphi, W.st.p, W.part.p);
+        HIPCHK(hipGetLastError());
+        std::vector<SysState> h = read_states(op, W.st);
+        bool any_dc = false, any_pcg = false;
+        for (int i = 0; i < NS; ++i) {
+            if (h[i].dc_status == kRunning) any_dc = true;
+            if (h[i].pcg_status == kRunning) any_pcg = true;
+        }
+        if (!any_dc) break;
+        int chunk = 8;
+        auto pcg_iter = [&] {
+            D::lp(s, op.nn, op.l_ptr.p, op.l_colidx.p, lval, dp, dap, W.st.p, W.part.p);
+            D::update(s, op.nn, ldiag, dp, dap, dphi, dr, W.st.p, W.part.p);
+            D::dir(s, op.nn, ldiag, dr, dp, W.st.p);
+        };
+        while (any_pcg) {
+            if (use_graphs()) {
+                const int kind = std::is_same<V, float>::value ? 101 : 100;
+                hipGraphExec_t ex = W.graphs.get(s, W.dp.p, kind, kGraphPcgIters, [&] {
+                    for (int k = 0; k < kGraphPcgIters; ++k) pcg_iter();
+                });
+                HIPCHK(hipGraphLaunch(ex, s));
+            } else {
+                for (int k = 0; k < chunk; ++k) pcg_iter();
+                HIPCHK(hipGetLastError());
+                chunk = std::min(chunk * 2, 64);
+            }
+            h = read_states(op, W.st);
+            any_pcg = false;
+            for (int i = 0; i < NS; ++i)
+                if (h[i].pcg_status == kRunning) any_pcg = true;
+        }
+        D::correct(s, op.npad, op.g_col.p, op.g_val.p, op.dc_bc_edge.p, dphi, E, W.st.p);
+        HIPCHK(hipGetLastError());
+    }
+}
+
 // DivCorr of the masked systems of a width-NA batch at width NB: E (and
 // phi0) move into the NB workspace, are corrected there and E moves back.
 template <int NA, int NB>
@@ -683,48 +771,11 @@ void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, u
         return;
     }
     ensure_dc_vectors<NS>(op, W);
-    hipStream_t s = op.ctx->stream;
-    using L = Launch64<NS>;
-    const int cg_max = std::max(500, op.nn);
-    const double pcg_tol = prm.divcorr_pcg_tol > 0 ? prm.divcorr_pcg_tol : 1e-10;
-    launch_dc_begin(s, W.st.p, mask);
-    for (int it = 0; it < prm.div_corr_iters; ++it) {
-        L::dc_divj(s, op.nn, it, prm.div_corr_tol, cg_max, pcg_tol, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p,
-                   op.ms.p, op.dc_bc_node.p, op.l_diag.p, E, phi0, W.dr.p, W.dz.p, W.dp.p, W.dphi.p,
-                   W.st.p, W.part.p);
-        HIPCHK(hipGetLastError());
-        std::vector<SysState> h = read_states(op, W.st);
-        bool any_dc = false, any_pcg = false;
-        for (int i = 0; i < NS; ++i) {
-            if (h[i].dc_status == kRunning) any_dc = true;
-            if (h[i].pcg_status == kRunning) any_pcg = true;
-        }
-        if (!any_dc) break;
-        int chunk = 8;
-        auto pcg_iter = [&] {
-            L::dc_lp(s, op.nn, op.l_ptr.p, op.l_colidx.p, op.l_vals.p, W.dp.p, W.dap.p, W.st.p,
-                     W.part.p);
-            L::dc_update(s, op.nn, op.l_diag.p, W.dp.p, W.dap.p, W.dphi.p, W.dr.p, W.st.p, W.part.p);
-            L::dc_p(s, op.nn, op.l_diag.p, W.dr.p, W.dp.p, W.st.p);
-        };
-        while (any_pcg) {
-            if (use_graphs()) {
-                hipGraphExec_t ex = W.graphs.get(s, W.dp.p, 100, kGraphPcgIters, [&] {
-                    for (int k = 0; k < kGraphPcgIters; ++k) pcg_iter();
-                });
-                HIPCHK(hipGraphLaunch(ex, s));
-            } else {
-                for (int k = 0; k < chunk; ++k) pcg_iter();
-                HIPCHK(hipGetLastError());
-                chunk = std::min(chunk * 2, 64);
-            }
-            h = read_states(op, W.st);
-            any_pcg = false;
-            for (int i = 0; i < NS; ++i)
-                if (h[i].pcg_status == kRunning) any_pcg = true;
-        }
-        L::dc_correct(s, op.npad, op.g_col.p, op.g_val.p, op.dc_bc_edge.p, W.dphi.p, E, W.st.p);
-        HIPCHK(hipGetLastError());
+    if (dc_in_fp32(prm)) {
+        ensure_laplacian32(op);
+        divcorr_pcg<NS, float>(op, W, E, phi0, mask, prm, op.l_vals32.p, op.l_diag32.p);
+    } else {
+        divcorr_pcg<NS, double>(op, W, E, phi0, mask, prm, op.l_vals.p, op.l_diag.p);
     }
 }
 
@@ -1563,8 +1614,8 @@ size_t operator_bytes(const m3d_operator& op) {
     return tiles + nbytes(op.d_perm, op.l_col, op.u_col, op.hang, op.slc, op.l_val, op.u_val, op.diag_s, op.me,
                   op.l_val32, op.u_val32, op.diag_s32, op.me32, op.l_pk, op.u_pk, op.vtab, op.vtab32, op.bc,
                   op.constrained, op.nt_ptr, op.nt_edge, op.g_col, op.l_ptr, op.l_colidx, op.nt_val,
-                  op.g_val, op.ms, op.l_vals, op.l_diag, op.dc_bc_edge, op.dc_dead, op.dc_bc_node,
-                  op.m_adj_ptr, op.m_adj_cell, op.l_edge, op.d_err, op.m_area, op.m_len, op.m_massless,
+                  op.g_val, op.ms, op.l_vals, op.l_diag, op.l_vals32, op.l_diag32, op.dc_bc_edge,
+                  op.dc_dead, op.dc_bc_node, op.m_adj_ptr, op.m_adj_cell, op.l_edge, op.d_err, op.m_area, op.m_len, op.m_massless,
                   op.m_dead, op.dc_in_pattern, op.s_cell_ptr, op.s_cell_edges, op.s_edge_ptr,
                   op.s_edge_cells, op.s_weight, op.s_sigma, op.s_omega, op.order[0], op.order[1],
                   op.order[2], op.order[3], op.order[4], op.order[5]);
@@ -1644,6 +1695,7 @@ void set_divcorr(m3d_operator& op, const m3d_divcorr_desc& d) {
     op.l_colidx.upload(std::vector<int>(d.L_colidx, d.L_colidx + lnnz), s);
     op.l_vals.upload(std::vector<double>(d.L_values, d.L_values + lnnz), s);
     op.l_diag.upload(std::vector<double>(d.L_diag, d.L_diag + nn), s);
+    op.l32_fresh = false;
     // For sigma updates on the device: the edges that make up the pattern of
     // L (weight >= 1e-30, as div_laplacian keeps them) and, per off-diagonal
     // entry, the one edge joining its two nodes.  A pattern that is not of
@@ -1719,6 +1771,7 @@ void update_sigma(m3d_operator& op, const double* d_sigma) {
     launch_sigma_laplacian(s, op.nn, op.nt_ptr.p, op.nt_edge.p, op.nt_val.p, op.ms.p, op.l_ptr.p, op.l_colidx.p,
                            op.l_edge.p, op.g_col.p, op.g_val.p, op.dc_bc_node.p, op.l_vals.p, op.l_diag.p);
     HIPCHK(hipGetLastError());
+    op.l32_fresh = false;
     if (op.nc == op.mass_nc && op.s_sigma.p)
         HIPCHK(hipMemcpyAsync(op.s_sigma.p, d_sigma, size_t(op.nc) * sizeof(double), hipMemcpyDeviceToDevice, s));
     int err = 0;
