@@ -14,6 +14,7 @@
 // every forward solve from zero (forward_solver_fv.cpp:353).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -152,68 +153,99 @@ __global__ __launch_bounds__(256) void k_conv_fwd(int batch, int dil,
     }
 }
 
-// Input gradient of a 32-channel layer: dz = dA * 1[act > 0] (when act is
-// given) gathered at v - d*tap; B operand lane l reads W[tap][ci = l%32][co
-// k-octet].  16-wide MFMA steps, the taps pipelined as in the forward pass.
-__global__ __launch_bounds__(256) void k_conv_bwd_data(int batch, int dil,
-                                                       const unsigned short* __restrict__ dout,
-                                                       const unsigned short* __restrict__ act,
-                                                       const unsigned short* __restrict__ wb,
-                                                       unsigned short* __restrict__ din) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + wave;
-    if (row >= batch * G * G) return;
-    const int b = row / (G * G), z = (row / G) % G, y = row % G;
-    const int m = lane & 31, q = lane >> 5;
-    f16 acc;
-    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-    constexpr int NK = HID / 16;
-    s8 a[kTapStages][NK], g[kTapStages][NK], w[kTapStages][NK];
-    auto fetch = [&](int tap, s8 (&A)[NK], s8 (&M)[NK], s8 (&W)[NK]) {
-        int dz, dy, dx;
-        tap_offsets(tap, dz, dy, dx);
-        const int zz = z - dz * dil, yy = y - dy * dil, xx = m - dx * dil;
-        const bool ok = zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G;
-        const size_t vsrc = ((size_t(b) * G + (ok ? zz : 0)) * G + (ok ? yy : 0)) * G + (ok ? xx : 0);
-        const unsigned short* src = dout + vsrc * HID;
-        const unsigned short* msk = act ? act + vsrc * HID : nullptr;
-        const unsigned short* wsrc = wb + (size_t(tap) * HID + m) * HID;  // [tap][ci][co]
-#pragma unroll
-        for (int k = 0; k < NK; ++k) {
-            s8 t = {0, 0, 0, 0, 0, 0, 0, 0};
-            s8 u = {1, 1, 1, 1, 1, 1, 1, 1};  // no mask: every entry passes
-            if (ok) {
-                t = *reinterpret_cast<const s8*>(src + 16 * k + 8 * q);
-                if (msk) u = *reinterpret_cast<const s8*>(msk + 16 * k + 8 * q);
-            }
-            A[k] = t;
-            M[k] = u;
-            W[k] = *reinterpret_cast<const s8*>(wsrc + 16 * k + 8 * q);
-        }
-    };
-#pragma unroll
-    for (int t = 0; t < kTapStages - 1; ++t) fetch(t, a[t], g[t], w[t]);
-#pragma unroll
-    for (int tap = 0; tap < 27; ++tap) {
-        if (tap + kTapStages - 1 < 27) {
-            const int nx = (tap + kTapStages - 1) % kTapStages;
-            fetch(tap + kTapStages - 1, a[nx], g[nx], w[nx]);
-        }
-        const int cur = tap % kTapStages;
-#pragma unroll
-        for (int k = 0; k < NK; ++k) {
-            s8 av = a[cur][k];
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (bf2f(static_cast<unsigned short>(g[cur][k][j])) <= 0.f) av[j] = 0;
-            acc = mfma16(av, w[cur][k], acc);
-        }
+// Forward conv of a 32-channel layer with the layer's weights in LDS: a
+// workgroup of kConvWaves wavefronts copies W (27 taps x 32 x 32 BF16, 54 KB)
+// into LDS once and its wavefronts then walk x-rows as k_conv_fwd does,
+// reading the B operands from LDS instead of each wavefront reading all 54 KB
+// from the caches.  16-byte units are stored XOR-swizzled within their
+// 64-byte (tap, co) row, unit c at c ^ swz(co), so the 16 lanes that read one
+// unit of 16 consecutive co rows hit distinct banks.
+constexpr int kConvWaves = 8;
+constexpr int kConvWBytes = 27 * HID * HID * 2;
+
+__device__ inline int wswz(int r) { return (((r >> 1) ^ (r >> 3)) & 1) | ((r >> 1) & 2); }
+
+__device__ inline void stage_weights(const unsigned short* __restrict__ w, s8* __restrict__ lds) {
+    constexpr int units = kConvWBytes / 16;  // 4 units per 32-channel row
+    for (int i = threadIdx.x; i < units; i += blockDim.x) {
+        const int r = i >> 2, c = i & 3;
+        lds[r * 4 + (c ^ wswz(r & 31))] = reinterpret_cast<const s8*>(w)[i];
     }
-    const size_t vrow = ((size_t(b) * G + z) * G + y) * G;
+}
+
+__device__ inline s8 lds_unit(const s8* __restrict__ lds, int tap, int r, int c) {
+    return lds[(tap * HID + r) * 4 + (c ^ wswz(r))];
+}
+
+// BWD: the input gradient din[v, ci] = sum_{tap, co} dz[v - d*tap, co] *
+// W[tap][co][ci] of a 32-channel layer, from the transposed weights
+// wb[tap][ci][co] and an already masked dz, stored as BF16 without bias.
+template <bool BWD, bool RELU, bool OUTF32>
+__global__ __launch_bounds__(kConvWaves * 64) void k_conv_fwd_w(int batch, int dil,
+                                                                const unsigned short* __restrict__ in,
+                                                                const unsigned short* __restrict__ wt,
+                                                                const float* __restrict__ bias,
+                                                                unsigned short* __restrict__ out_bf,
+                                                                float* __restrict__ out_f, int cout_store) {
+    extern __shared__ s8 wl[];
+    stage_weights(wt, wl);
+    __syncthreads();
+    constexpr int NK = HID / 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = lane & 31, q = lane >> 5;
+    const int rows = batch * G * G;
+    const float bc = BWD ? 0.f : bias[m];
+    const int sg = BWD ? -1 : 1;
+    for (int row = blockIdx.x * kConvWaves + wave; row < rows; row += gridDim.x * kConvWaves) {
+        const int b = row / (G * G), z = (row / G) % G, y = row % G;
+        f16 acc;
+        for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+        s8 a[kTapStages][NK];
+        auto fetch = [&](int tap, s8 (&A)[NK]) {
+            int dz, dy, dx;
+            tap_offsets(tap, dz, dy, dx);
+            const int zz = z + sg * dz * dil, yy = y + sg * dy * dil, xx = m + sg * dx * dil;
+            const bool ok = zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G;
+            const size_t vin = ((size_t(b) * G + (ok ? zz : 0)) * G + (ok ? yy : 0)) * G + (ok ? xx : 0);
+            const unsigned short* src = in + vin * HID;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int x = 8 * (r >> 2) + 4 * q + (r & 3);
-        din[(vrow + x) * HID + m] = f2bf(acc[r]);
+            for (int k = 0; k < NK; ++k) {
+                s8 t = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (ok) t = *reinterpret_cast<const s8*>(src + 16 * k + 8 * q);
+                A[k] = t;
+            }
+        };
+        // B operands one tap ahead from LDS; a scheduling barrier per tap
+        // keeps the compiler from hoisting every tap's LDS reads (which
+        // would take 216 registers).
+        s8 wv[2][NK];
+#pragma unroll
+        for (int t = 0; t < kTapStages - 1; ++t) fetch(t, a[t]);
+#pragma unroll
+        for (int k = 0; k < NK; ++k) wv[0][k] = lds_unit(wl, 0, m, 2 * k + q);
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            if (tap + kTapStages - 1 < 27) fetch(tap + kTapStages - 1, a[(tap + kTapStages - 1) % kTapStages]);
+            if (tap + 1 < 27) {
+#pragma unroll
+                for (int k = 0; k < NK; ++k) wv[(tap + 1) & 1][k] = lds_unit(wl, tap + 1, m, 2 * k + q);
+            }
+#pragma unroll
+            for (int k = 0; k < NK; ++k) acc = mfma16(a[tap % kTapStages][k], wv[tap & 1][k], acc);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const size_t vrow = ((size_t(b) * G + z) * G + y) * G;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int x = 8 * (r >> 2) + 4 * q + (r & 3);
+            float v = acc[r] + bc;
+            if (RELU) v = v > 0.f ? v : 0.f;
+            if (OUTF32) {
+                if (m < cout_store) out_f[(vrow + x) * cout_store + m] = v;
+            } else {
+                out_bf[(vrow + x) * HID + m] = f2bf(v);
+            }
+        }
     }
 }
 
@@ -325,31 +357,90 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows
     }
 }
 
-__global__ void k_sum_chunks(int nchunk, int count, const float* __restrict__ part, float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+// out[i] = sum_c part[c][i] over nchunk partial sets of count values.  A
+// 256-thread block takes 32 consecutive outputs; its eight groups of 32
+// threads each add every eighth partial set, and the groups' sums are added
+// in LDS.
+__global__ __launch_bounds__(256) void k_sum_chunks(int nchunk, int count, const float* __restrict__ part,
+                                                    float* __restrict__ out) {
+    __shared__ float sm[8][32];
+    const int j = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int i = blockIdx.x * 32 + j;
     float s = 0.f;
-    for (int c = 0; c < nchunk; ++c) s += part[size_t(c) * count + i];
-    out[i] = s;
+    if (i < count)
+        for (int c = g; c < nchunk; c += 8) s += part[size_t(c) * count + i];
+    sm[g][j] = s;
+    __syncthreads();
+    if (g == 0 && i < count) {
+        float t = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t += sm[k][j];
+        out[i] = t;
+    }
 }
 
-// Bias gradient partials: per block sum of dz over a voxel range.
-__global__ void k_bias_grad(int nvox, const unsigned short* __restrict__ dout,
-                            const unsigned short* __restrict__ act, float* __restrict__ part) {
-    __shared__ float sm[256];
-    const int co = threadIdx.x & 31, sub = threadIdx.x >> 5;  // 8 voxel lanes per channel
-    float s = 0.f;
-    for (int v = blockIdx.x * 8 + sub; v < nvox; v += gridDim.x * 8) {
-        unsigned short g = dout[size_t(v) * HID + co];
-        if (act && bf2f(act[size_t(v) * HID + co]) <= 0.f) g = 0;
-        s += bf2f(g);
+// Bias gradient partials: per block the sum of dz (masked by act > 0) over
+// its voxels, 32 channels.  Thread t loads channels 8 (t % 4) .. + 7 of one
+// voxel (16-byte loads), 64 voxels per block step; the 64 voxel lanes are
+// added in LDS.
+constexpr int kBiasBlocks = 512;
+__global__ __launch_bounds__(256) void k_bias_grad(int nvox, const unsigned short* __restrict__ dout,
+                                                   const unsigned short* __restrict__ act, float* __restrict__ part) {
+    __shared__ float sm[64][33];
+    const int o = threadIdx.x & 3, vl = threadIdx.x >> 2;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int v = blockIdx.x * 64 + vl; v < nvox; v += gridDim.x * 64) {
+        const s8 g = *reinterpret_cast<const s8*>(dout + size_t(v) * HID + 8 * o);
+        s8 a = {1, 1, 1, 1, 1, 1, 1, 1};
+        if (act) a = *reinterpret_cast<const s8*>(act + size_t(v) * HID + 8 * o);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (!act || bf2f(static_cast<unsigned short>(a[j])) > 0.f) acc[j] += bf2f(static_cast<unsigned short>(g[j]));
     }
-    sm[threadIdx.x] = s;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sm[vl][8 * o + j] = acc[j];
+    __syncthreads();
+    // 32 channels x 8 groups of 8 voxel lanes, then the 8 group sums.
+    const int c = threadIdx.x & 31, gq = threadIdx.x >> 5;
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t += sm[gq * 8 + k][c];
+    __syncthreads();
+    sm[gq][c] = t;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float u = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) u += sm[k][threadIdx.x];
+        part[blockIdx.x * 32 + threadIdx.x] = u;
+    }
+}
+
+// out[c] = sum over nb partials of 32 channels: one block of 1024 threads,
+// 32 groups each adding every 32nd partial, then the groups in LDS.
+__global__ __launch_bounds__(1024) void k_sum_bias(int nb, const float* __restrict__ part, float* __restrict__ out) {
+    __shared__ float sm[32][33];
+    const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+    float s = 0.f;
+    for (int i = g; i < nb; i += 32) s += part[i * 32 + c];
+    sm[g][c] = s;
     __syncthreads();
     if (threadIdx.x < 32) {
         float t = 0.f;
-        for (int k = 0; k < 8; ++k) t += sm[k * 32 + threadIdx.x];
-        part[blockIdx.x * 32 + threadIdx.x] = t;
+        for (int k = 0; k < 32; ++k) t += sm[k][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
+// dz = dA * 1[act > 0] in place (BF16, 8 channels per thread).
+__global__ void k_mask_relu(size_t units, unsigned short* __restrict__ g, const unsigned short* __restrict__ act) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < units; i += size_t(gridDim.x) * blockDim.x) {
+        s8 v = reinterpret_cast<s8*>(g)[i];
+        const s8 a = reinterpret_cast<const s8*>(act)[i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (bf2f(static_cast<unsigned short>(a[j])) <= 0.f) v[j] = 0;
+        reinterpret_cast<s8*>(g)[i] = v;
     }
 }
 
@@ -532,12 +623,14 @@ void forward(m3d_net& n, int batch, const float* d_in, float* d_out) {
     const dim3 grid((batch * G * G + 3) / 4), blk(256);
     hipLaunchKernelGGL((k_conv_fwd<CIN0, true, false>), grid, blk, 0, s, batch, n.dil[0], n.x0.p, n.wt[0].p,
                        n.b[0].p, n.act[0].p, nullptr, HID);
-    hipLaunchKernelGGL((k_conv_fwd<HID, true, false>), grid, blk, 0, s, batch, n.dil[1], n.act[0].p, n.wt[1].p,
-                       n.b[1].p, n.act[1].p, nullptr, HID);
-    hipLaunchKernelGGL((k_conv_fwd<HID, true, false>), grid, blk, 0, s, batch, n.dil[2], n.act[1].p, n.wt[2].p,
-                       n.b[2].p, n.act[2].p, nullptr, HID);
-    hipLaunchKernelGGL((k_conv_fwd<HID, false, true>), grid, blk, 0, s, batch, n.dil[3], n.act[2].p, n.wt[3].p,
-                       n.b[3].p, nullptr, d_out, COUT);
+    // 32-channel layers: weights in LDS, two workgroups per CU.
+    const dim3 gw(std::min((batch * G * G + kConvWaves - 1) / kConvWaves, 512)), bw(kConvWaves * 64);
+    hipLaunchKernelGGL((k_conv_fwd_w<false, true, false>), gw, bw, kConvWBytes, s, batch, n.dil[1], n.act[0].p,
+                       n.wt[1].p, n.b[1].p, n.act[1].p, nullptr, HID);
+    hipLaunchKernelGGL((k_conv_fwd_w<false, true, false>), gw, bw, kConvWBytes, s, batch, n.dil[2], n.act[1].p,
+                       n.wt[2].p, n.b[2].p, n.act[2].p, nullptr, HID);
+    hipLaunchKernelGGL((k_conv_fwd_w<false, false, true>), gw, bw, kConvWBytes, s, batch, n.dil[3], n.act[2].p,
+                       n.wt[3].p, n.b[3].p, nullptr, d_out, COUT);
     chk(hipGetLastError(), "conv forward");
 }
 
@@ -556,9 +649,17 @@ float backward(m3d_net& n, int batch, const float* d_in, const float* d_tgt) {
     // dz of layer l is grad (masked by act[l] for ReLU layers).
     const unsigned short* g = n.dout.p;
     unsigned short* bufs[2] = {n.grad_a.p, n.grad_b.p};
+    const dim3 gw(std::min((batch * G * G + kConvWaves - 1) / kConvWaves, 512)), bw(kConvWaves * 64);
     for (int l = NL - 1; l >= 0; --l) {
-        const unsigned short* a_out = (l < NL - 1) ? n.act[l].p : nullptr;  // ReLU mask
         const unsigned short* a_in = (l == 0) ? n.x0.p : n.act[l - 1].p;
+        // ReLU layers: dz = dA * 1[act > 0], formed once in place (dA is not
+        // needed afterwards), so the kernels below read dz unmasked.
+        if (l < NL - 1) {
+            const size_t units = size_t(nvox) * HID / 8;
+            hipLaunchKernelGGL(k_mask_relu, dim3(std::min<size_t>((units + 255) / 256, 4096)), dim3(256), 0, s, units,
+                               const_cast<unsigned short*>(g), n.act[l].p);
+        }
+        const unsigned short* a_out = nullptr;
         const int c = m3d_net::cin(l);
         if (c == CIN0)
             hipLaunchKernelGGL((k_conv_bwd_w<CIN0>), dim3(nchunk, 27), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
@@ -567,13 +668,14 @@ float backward(m3d_net& n, int batch, const float* d_in, const float* d_tgt) {
             hipLaunchKernelGGL((k_conv_bwd_w<HID>), dim3(nchunk, 27), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
                                a_out, n.part.p);
         const int cnt = 27 * HID * c;
-        hipLaunchKernelGGL(k_sum_chunks, dim3((cnt + 255) / 256), dim3(256), 0, s, nchunk, cnt, n.part.p,
+        hipLaunchKernelGGL(k_sum_chunks, dim3((cnt + 31) / 32), dim3(256), 0, s, nchunk, cnt, n.part.p,
                            n.gw[l].p);
-        hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(256), 0, s, nvox, g, a_out, n.part.p);
-        hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(32), 0, s, 256, 32, n.part.p, n.gb[l].p);
+        hipLaunchKernelGGL(k_bias_grad, dim3(kBiasBlocks), dim3(256), 0, s, nvox, g, a_out, n.part.p);
+        hipLaunchKernelGGL(k_sum_bias, dim3(1), dim3(1024), 0, s, kBiasBlocks, n.part.p, n.gb[l].p);
         if (l > 0) {
             unsigned short* dst = bufs[l & 1];
-            hipLaunchKernelGGL(k_conv_bwd_data, grid, blk, 0, s, batch, n.dil[l], g, a_out, n.wb[l].p, dst);
+            hipLaunchKernelGGL((k_conv_fwd_w<true, false, false>), gw, bw, kConvWBytes, s, batch, n.dil[l], g,
+                               n.wb[l].p, nullptr, dst, nullptr, HID);
             g = dst;
         }
     }
@@ -647,7 +749,7 @@ int m3d_net_create(m3d_context* ctx, int max_batch, const int* dilations, unsign
         n->grad_b.alloc(nv * HID);
         n->dout.alloc(nv * HID);
         n->out.alloc(nv * COUT);
-        n->part.alloc(size_t(64) * 27 * HID * HID + 256 * 32);
+        n->part.alloc(size_t(64) * 27 * HID * HID + kBiasBlocks * 32);
         n->lpart.alloc(512);
         n->gscale.alloc(1);
         upload_bf16(*n);
