@@ -249,111 +249,124 @@ __global__ __launch_bounds__(kConvWaves * 64) void k_conv_fwd_w(int batch, int d
     }
 }
 
-// Weight gradient partials: block (chunk, tap) sums dz[v, co] * in[v + d*tap, ci]
-// over the rows of its chunk; M = co (32), N = ci (CIN, zero-padded to 32),
-// K = voxels.  The contraction runs over voxels, the strided dimension of
-// channels-last activations, so each wavefront stages one x-row of dz
-// (masked by act > 0) and of the shifted input through LDS: 16-byte global
-// loads, written transposed into wave-private tiles [channel][x] (row stride
-// kTileStride keeps every 8-voxel operand 16-byte aligned), then read back
-// as the operands of two 16-deep MFMAs.  A tile belongs to one wavefront and
-// LDS serves a wavefront's accesses in order, so no block barrier is needed
-// between rows (the wavefronts of a block may take different row counts).
+// Weight gradient partials: block (chunk, dz dy) sums dz[v, co] * in[v + d*tap, ci]
+// over the rows of its chunk for the three taps (dz, dy, dx = -1, 0, 1);
+// M = co (32), N = ci (CIN, zero-padded to 32), K = voxels.  The
+// contraction runs over voxels, the strided dimension of channels-last
+// activations, so each wavefront stages one x-row of dz (already masked by
+// the ReLU) and the row (z + dz d, y + dy d) of the input, shifted by dx d
+// for each of the three taps, through LDS: 16-byte global loads, written
+// transposed into wave-private tiles [channel][x] (row stride kTileStride
+// keeps every 8-voxel operand 16-byte aligned), then read back as the
+// operands of 16-deep MFMAs; the dz operands are read once for the three
+// taps.  A tile belongs to one wavefront and LDS serves a wavefront's
+// accesses in order, so no block barrier is needed between rows (the
+// wavefronts of a block may take different row counts).
 constexpr int kTileStride = 40;
 
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows_per_chunk,
                                                     const unsigned short* __restrict__ in,
                                                     const unsigned short* __restrict__ dout,
-                                                    const unsigned short* __restrict__ act,
                                                     float* __restrict__ part) {
-    __shared__ float red[4][16][64];
-    __shared__ __attribute__((aligned(16))) unsigned short tdz[4][HID * kTileStride];
-    __shared__ __attribute__((aligned(16))) unsigned short tin[4][HID * kTileStride];
+    // Per wavefront a dz tile and three input tiles; after the rows the same
+    // LDS holds the wavefronts' accumulators for the block sum (40 KB, three
+    // blocks per CU).
+    constexpr int kT = HID * kTileStride;
+    __shared__ __attribute__((aligned(16))) unsigned short lds[4 * 4 * kT];
+    auto tdz = reinterpret_cast<unsigned short(*)[kT]>(lds);
+    auto tin = reinterpret_cast<unsigned short(*)[3][kT]>(lds + 4 * kT);
+    auto red = reinterpret_cast<float(*)[16][64]>(lds);
+    static_assert(4 * 16 * 64 * sizeof(float) <= sizeof(lds), "the block sum fits the tiles' LDS");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int chunk = blockIdx.x, tap = blockIdx.y;
+    const int chunk = blockIdx.x, dzy = blockIdx.y;  // dzy = 3 (dz + 1) + (dy + 1)
     const int m = lane & 31, q = lane >> 5;
-    int dz, dy, dx;
-    tap_offsets(tap, dz, dy, dx);
+    const int dz = dzy / 3 - 1, dy = dzy % 3 - 1;
     unsigned short* Tdz = tdz[wave];
-    unsigned short* Tin = tin[wave];
-    // Input channels CIN..31 of the B tile stay zero.
-    for (int k = lane; k < HID * kTileStride; k += 64) Tin[k] = 0;
-    f16 acc;
-    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    // Input channels CIN..31 of the B tiles stay zero.
+    for (int k = lane; k < 3 * kT; k += 64) tin[wave][k / kT][k % kT] = 0;
+    f16 acc[3];
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < 16; ++i) acc[d][i] = 0.f;
     const int rows = batch * G * G;
     const int r0 = chunk * rows_per_chunk, r1 = min(rows, r0 + rows_per_chunk);
-    for (int row = r0 + wave; row < r1; row += 4) {
+    // Global loads of a row: lane l takes voxels x, x + 1 (x = 2 (l / 4)) of
+    // dz channels c0 .. c0 + 7 (c0 = 8 (l % 4)); lanes below NP take a voxel
+    // pair and channel octet of the input row for each of the three shifts.
+    // The next row's loads are issued before this row's tiles are read.
+    constexpr int NO = CIN / 8;  // channel octets per voxel
+    constexpr int NP = 16 * NO;  // (voxel pair, octet) items of the input
+    const int xg = 2 * (lane >> 2), cg = (lane & 3) * 8;
+    const int xi = 2 * (lane / NO), ci0 = (lane % NO) * 8;
+    s8 gdz[2], gin[3][2];
+    auto load = [&](int row) {
         const int b = row / (G * G), z = (row / G) % G, y = row % G;
         const size_t vrow = ((size_t(b) * G + z) * G + y) * G;
         const int zz = z + dz * dil, yy = y + dy * dil;
         const bool okzy = zz >= 0 && zz < G && yy >= 0 && yy < G;
         const size_t vsrc_row = ((size_t(b) * G + (okzy ? zz : 0)) * G + (okzy ? yy : 0)) * G;
-        __builtin_amdgcn_wave_barrier();
-        // dz tile: lane l takes voxels x, x + 1 (x = 2 (l / 4)) of channels
-        // c0 .. c0 + 7 (c0 = 8 (l % 4)) and writes each channel's pair as one
-        // 4-byte word of the transposed tile.
-        {
-            const int x = 2 * (lane >> 2), c0 = (lane & 3) * 8;
-            s8 g[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) gdz[h] = *reinterpret_cast<const s8*>(dout + (vrow + xg + h) * HID + cg);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                g[h] = *reinterpret_cast<const s8*>(dout + (vrow + x + h) * HID + c0);
-                if (act) {
-                    const s8 a = *reinterpret_cast<const s8*>(act + (vrow + x + h) * HID + c0);
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        if (bf2f(static_cast<unsigned short>(a[j])) <= 0.f) g[h][j] = 0;
-                }
+                const int xs = xi + h + (d - 1) * dil;
+                s8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (lane < NP && okzy && xs >= 0 && xs < G)
+                    v = *reinterpret_cast<const s8*>(in + (vsrc_row + xs) * CIN + ci0);
+                gin[d][h] = v;
             }
+    };
+    if (r0 + wave < r1) load(r0 + wave);
+    for (int row = r0 + wave; row < r1; row += 4) {
+        __builtin_amdgcn_wave_barrier();
+        // Transposed tiles: each channel's voxel pair is one 4-byte word.
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                *reinterpret_cast<unsigned*>(Tdz + (c0 + j) * kTileStride + x) =
-                    unsigned(static_cast<unsigned short>(g[0][j])) |
-                    (unsigned(static_cast<unsigned short>(g[1][j])) << 16);
-        }
-        // Input tile: Tin[ci][x] = in[x + dx * dil][ci], zero outside the
-        // grid; the same pairing over the CIN / 8 channel octets.
-        {
-            constexpr int NO = CIN / 8;        // channel octets per voxel
-            constexpr int NP = 16 * NO;        // (voxel pair, octet) items
-            if (lane < NP) {
-                const int x = 2 * (lane / NO), c0 = (lane % NO) * 8;
-                s8 t[2];
+        for (int j = 0; j < 8; ++j)
+            *reinterpret_cast<unsigned*>(Tdz + (cg + j) * kTileStride + xg) =
+                unsigned(static_cast<unsigned short>(gdz[0][j])) |
+                (unsigned(static_cast<unsigned short>(gdz[1][j])) << 16);
+        if (lane < NP) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int xs = x + h + dx * dil;
-                    s8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-                    if (okzy && xs >= 0 && xs < G) v = *reinterpret_cast<const s8*>(in + (vsrc_row + xs) * CIN + c0);
-                    t[h] = v;
-                }
+            for (int d = 0; d < 3; ++d) {
+                unsigned short* Tin = tin[wave][d];
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    *reinterpret_cast<unsigned*>(Tin + (c0 + j) * kTileStride + x) =
-                        unsigned(static_cast<unsigned short>(t[0][j])) |
-                        (unsigned(static_cast<unsigned short>(t[1][j])) << 16);
+                    *reinterpret_cast<unsigned*>(Tin + (ci0 + j) * kTileStride + xi) =
+                        unsigned(static_cast<unsigned short>(gin[d][0][j])) |
+                        (unsigned(static_cast<unsigned short>(gin[d][1][j])) << 16);
             }
         }
+        if (row + 4 < r1) load(row + 4);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int s16 = 0; s16 < 2; ++s16) {
             const s8 a = *reinterpret_cast<const s8*>(Tdz + m * kTileStride + 16 * s16 + 8 * q);
-            const s8 w = *reinterpret_cast<const s8*>(Tin + m * kTileStride + 16 * s16 + 8 * q);
-            acc = mfma16(a, w, acc);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const s8 w = *reinterpret_cast<const s8*>(tin[wave][d] + m * kTileStride + 16 * s16 + 8 * q);
+                acc[d] = mfma16(a, w, acc[d]);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     }
-    for (int r = 0; r < 16; ++r) red[wave][r][lane] = acc[r];
-    __syncthreads();
-    if (wave == 0) {
-        for (int r = 0; r < 16; ++r) {
-            const float v = red[0][r][lane] + red[1][r][lane] + red[2][r][lane] + red[3][r][lane];
-            const int co = 8 * (r >> 2) + 4 * q + (r & 3);  // D row = M index = co
-            const int ci = m;                               // D column = N index = ci
-            if (ci < CIN) part[((size_t(chunk) * 27 + tap) * HID + co) * CIN + ci] = v;
+    __syncthreads();  // every wavefront is done with its tiles
+    for (int d = 0; d < 3; ++d) {
+        for (int r = 0; r < 16; ++r) red[wave][r][lane] = acc[d][r];
+        __syncthreads();
+        if (wave == 0) {
+            const int tap = dzy * 3 + d;
+            for (int r = 0; r < 16; ++r) {
+                const float v = red[0][r][lane] + red[1][r][lane] + red[2][r][lane] + red[3][r][lane];
+                const int co = 8 * (r >> 2) + 4 * q + (r & 3);  // D row = M index = co
+                const int ci = m;                               // D column = N index = ci
+                if (ci < CIN) part[((size_t(chunk) * 27 + tap) * HID + co) * CIN + ci] = v;
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -476,44 +489,78 @@ __global__ void k_loss(int nvox, const float* __restrict__ out, const float* __r
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-// Global gradient norm of all layers, one block: gscale[0] = min(1, clip / ||g||)
-// (1 when clip <= 0), summed in a fixed order.
-struct GradPtrs {
-    const float* w[4];
-    const float* b[4];
-    int nw[4];
+// Every parameter of the four layers in one index space: the weight arrays
+// of layers 0-3, then their bias arrays (off[k] .. off[k + 1] for array k).
+struct ParamSet {
+    float* p[8];
+    const float* g[8];
+    float* m[8];
+    float* v[8];
+    unsigned short* wt[4];  // BF16 weights [tap][co][ci]
+    unsigned short* wb[4];  // BF16 transposed weights [tap][ci][co] (32-channel inputs)
+    int cin[4];
+    int off[9];
 };
-__global__ __launch_bounds__(1024) void k_grad_scale(GradPtrs g, float clip, float* __restrict__ gscale) {
-    __shared__ float sm[1024];
+__device__ inline int param_array(const ParamSet& P, int i) {
+    int k = 0;
+    while (k < 7 && i >= P.off[k + 1]) ++k;
+    return k;
+}
+
+// Global gradient norm, first stage: kSqBlocks block partials of the sum of
+// squares over the parameter index space (fixed slots, fixed order).
+constexpr int kSqBlocks = 64;
+__global__ __launch_bounds__(256) void k_grad_sq(ParamSet P, float* __restrict__ part) {
+    __shared__ float sm[256];
     float s = 0.f;
-    for (int l = 0; l < 4; ++l) {
-        for (int i = threadIdx.x; i < g.nw[l]; i += blockDim.x) s += g.w[l][i] * g.w[l][i];
-        for (int i = threadIdx.x; i < 32; i += blockDim.x) s += g.b[l][i] * g.b[l][i];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < P.off[8]; i += kSqBlocks * 256) {
+        const int k = param_array(P, i);
+        const float g = P.g[k][i - P.off[k]];
+        s += g * g;
     }
     sm[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const float nrm = sqrtf(sm[0]);
-        gscale[0] = (clip > 0.f && nrm > clip) ? clip / nrm : 1.f;
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-// Adam on FP32 master parameters (gradient scaled by gscale[0]).
-__global__ void k_adam(int count, float lr, float b1, float b2, float eps, float c1, float c2,
-                       float* __restrict__ p, const float* __restrict__ g, const float* __restrict__ gscale,
-                       float* __restrict__ mo, float* __restrict__ ve) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const float gi = g[i] * gscale[0];
-    const float m = b1 * mo[i] + (1.f - b1) * gi;
-    const float v = b2 * ve[i] + (1.f - b2) * gi * gi;
-    mo[i] = m;
-    ve[i] = v;
-    p[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+// Adam on the FP32 master parameters of every layer in one launch, the
+// gradient scaled by min(1, clip / ||g||) (1 when clip <= 0; every block
+// adds the kSqBlocks partials in the same order), and the BF16 copies of the
+// updated weights that the conv kernels read.
+__global__ __launch_bounds__(256) void k_adam_all(ParamSet P, float lr, float b1, float b2, float eps, float c1,
+                                                  float c2, float clip, const float* __restrict__ sq) {
+    static_assert(kSqBlocks == 64, "one wavefront adds the partials");
+    __shared__ float sc;
+    if (threadIdx.x < 64) {
+        float t = sq[threadIdx.x];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if (threadIdx.x == 0) {
+            const float nrm = sqrtf(t);
+            sc = (clip > 0.f && nrm > clip) ? clip / nrm : 1.f;
+        }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= P.off[8]) return;
+    const int k = param_array(P, i), j = i - P.off[k];
+    const float gi = P.g[k][j] * sc;
+    const float m = b1 * P.m[k][j] + (1.f - b1) * gi;
+    const float v = b2 * P.v[k][j] + (1.f - b2) * gi * gi;
+    P.m[k][j] = m;
+    P.v[k][j] = v;
+    const float pn = P.p[k][j] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    P.p[k][j] = pn;
+    if (k < 4) {
+        const int cin = P.cin[k];
+        const int tap = j / (HID * cin), co = (j / cin) % HID, ci = j % cin;
+        const unsigned short h = f2bf(pn);
+        P.wt[k][j] = h;
+        if (cin == HID) P.wb[k][(size_t(tap) * HID + ci) * HID + co] = h;
+    }
 }
 
 __global__ void k_weights_bf16(int cin, const float* __restrict__ w, unsigned short* __restrict__ wt,
@@ -634,13 +681,14 @@ void forward(m3d_net& n, int batch, const float* d_in, float* d_out) {
     chk(hipGetLastError(), "conv forward");
 }
 
-// Loss and parameter gradients (into gw / gb) for one batch.
-float backward(m3d_net& n, int batch, const float* d_in, const float* d_tgt) {
+// Loss partials (lpart) and parameter gradients (into gw / gb) for one
+// batch, enqueued on the stream; read_loss waits for them.
+constexpr int kLossBlocks = 512;
+void backward(m3d_net& n, int batch, const float* d_in, const float* d_tgt) {
     const int nvox = batch * V;
     hipStream_t s = n.stream;
     forward(n, batch, d_in, n.out.p);
-    const int lblocks = 512;
-    hipLaunchKernelGGL(k_loss, dim3(lblocks), dim3(256), 0, s, nvox, n.out.p, d_tgt,
+    hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, s, nvox, n.out.p, d_tgt,
                        1.0f / float(size_t(nvox) * COUT), n.dout.p, n.lpart.p);
     const dim3 grid((batch * G * G + 3) / 4), blk(256);
     const int rows = batch * G * G;
@@ -659,18 +707,17 @@ float backward(m3d_net& n, int batch, const float* d_in, const float* d_tgt) {
             hipLaunchKernelGGL(k_mask_relu, dim3(std::min<size_t>((units + 255) / 256, 4096)), dim3(256), 0, s, units,
                                const_cast<unsigned short*>(g), n.act[l].p);
         }
-        const unsigned short* a_out = nullptr;
         const int c = m3d_net::cin(l);
         if (c == CIN0)
-            hipLaunchKernelGGL((k_conv_bwd_w<CIN0>), dim3(nchunk, 27), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
-                               a_out, n.part.p);
+            hipLaunchKernelGGL((k_conv_bwd_w<CIN0>), dim3(nchunk, 9), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
+                               n.part.p);
         else
-            hipLaunchKernelGGL((k_conv_bwd_w<HID>), dim3(nchunk, 27), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
-                               a_out, n.part.p);
+            hipLaunchKernelGGL((k_conv_bwd_w<HID>), dim3(nchunk, 9), blk, 0, s, batch, n.dil[l], rpc, a_in, g,
+                               n.part.p);
         const int cnt = 27 * HID * c;
         hipLaunchKernelGGL(k_sum_chunks, dim3((cnt + 31) / 32), dim3(256), 0, s, nchunk, cnt, n.part.p,
                            n.gw[l].p);
-        hipLaunchKernelGGL(k_bias_grad, dim3(kBiasBlocks), dim3(256), 0, s, nvox, g, a_out, n.part.p);
+        hipLaunchKernelGGL(k_bias_grad, dim3(kBiasBlocks), dim3(256), 0, s, nvox, g, nullptr, n.part.p);
         hipLaunchKernelGGL(k_sum_bias, dim3(1), dim3(1024), 0, s, kBiasBlocks, n.part.p, n.gb[l].p);
         if (l > 0) {
             unsigned short* dst = bufs[l & 1];
@@ -680,12 +727,17 @@ float backward(m3d_net& n, int batch, const float* d_in, const float* d_tgt) {
         }
     }
     chk(hipGetLastError(), "conv backward");
-    std::vector<float> lp(lblocks);
-    chk(hipMemcpyAsync(lp.data(), n.lpart.p, lblocks * sizeof(float), hipMemcpyDeviceToHost, s), "loss copy");
-    chk(hipStreamSynchronize(s), "sync");
+}
+
+// The loss of the last backward(): waits for the stream.
+float read_loss(m3d_net& n, int batch) {
+    std::vector<float> lp(kLossBlocks);
+    chk(hipMemcpyAsync(lp.data(), n.lpart.p, kLossBlocks * sizeof(float), hipMemcpyDeviceToHost, n.stream),
+        "loss copy");
+    chk(hipStreamSynchronize(n.stream), "sync");
     double tot = 0;
     for (float v : lp) tot += v;
-    return float(tot / (double(nvox) * COUT));
+    return float(tot / (double(batch) * V * COUT));
 }
 
 template <typename F>
@@ -751,7 +803,7 @@ int m3d_net_create(m3d_context* ctx, int max_batch, const int* dilations, unsign
         n->out.alloc(nv * COUT);
         n->part.alloc(size_t(64) * 27 * HID * HID + kBiasBlocks * 32);
         n->lpart.alloc(512);
-        n->gscale.alloc(1);
+        n->gscale.alloc(kSqBlocks);
         upload_bf16(*n);
         chk(hipStreamSynchronize(n->stream), "sync");
         *out = n.release();
@@ -818,7 +870,8 @@ int m3d_net_forward(m3d_net* n, int batch, const float* d_in, float* d_out) {
 int m3d_net_gradients(m3d_net* n, int batch, const float* d_in, const float* d_tgt, float* loss) {
     return nn_guard([&] {
         if (batch <= 0 || batch > n->max_batch) throw std::invalid_argument("batch out of range");
-        const float l = backward(*n, batch, d_in, d_tgt);
+        backward(*n, batch, d_in, d_tgt);
+        const float l = read_loss(*n, batch);
         if (loss) *loss = l;
     });
 }
@@ -850,26 +903,32 @@ int m3d_net_set_clip(m3d_net* n, float max_norm) {
 int m3d_net_train_step(m3d_net* n, int batch, const float* d_in, const float* d_tgt, float lr, float* loss) {
     return nn_guard([&] {
         if (batch <= 0 || batch > n->max_batch) throw std::invalid_argument("batch out of range");
-        const float l = backward(*n, batch, d_in, d_tgt);
+        backward(*n, batch, d_in, d_tgt);
         n->step++;
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float c1 = 1.f - std::pow(b1, float(n->step)), c2 = 1.f - std::pow(b2, float(n->step));
-        GradPtrs gp;
+        ParamSet P;
+        int off = 0;
         for (int k = 0; k < NL; ++k) {
-            gp.w[k] = n->gw[k].p;
-            gp.b[k] = n->gb[k].p;
-            gp.nw[k] = 27 * HID * m3d_net::cin(k);
+            const int c = m3d_net::cin(k);
+            P.p[k] = n->w[k].p; P.g[k] = n->gw[k].p; P.m[k] = n->mw[k].p; P.v[k] = n->vw[k].p;
+            P.p[NL + k] = n->b[k].p; P.g[NL + k] = n->gb[k].p; P.m[NL + k] = n->mb[k].p; P.v[NL + k] = n->vb[k].p;
+            P.wt[k] = n->wt[k].p;
+            P.wb[k] = k > 0 ? n->wb[k].p : nullptr;
+            P.cin[k] = c;
+            P.off[k] = off;
+            off += 27 * HID * c;
         }
-        hipLaunchKernelGGL(k_grad_scale, dim3(1), dim3(1024), 0, n->stream, gp, n->clip, n->gscale.p);
         for (int k = 0; k < NL; ++k) {
-            const int cnt = 27 * HID * m3d_net::cin(k);
-            hipLaunchKernelGGL(k_adam, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, cnt, lr, b1, b2, eps, c1, c2,
-                               n->w[k].p, n->gw[k].p, n->gscale.p, n->mw[k].p, n->vw[k].p);
-            hipLaunchKernelGGL(k_adam, dim3(1), dim3(32), 0, n->stream, HID, lr, b1, b2, eps, c1, c2, n->b[k].p,
-                               n->gb[k].p, n->gscale.p, n->mb[k].p, n->vb[k].p);
+            P.off[NL + k] = off;
+            off += HID;
         }
-        upload_bf16(*n);
-        chk(hipStreamSynchronize(n->stream), "sync");
+        P.off[2 * NL] = off;
+        hipLaunchKernelGGL(k_grad_sq, dim3(kSqBlocks), dim3(256), 0, n->stream, P, n->gscale.p);
+        hipLaunchKernelGGL(k_adam_all, dim3((off + 255) / 256), dim3(256), 0, n->stream, P, lr, b1, b2, eps, c1, c2,
+                           n->clip, n->gscale.p);
+        chk(hipGetLastError(), "adam");
+        const float l = read_loss(*n, batch);
         if (loss) *loss = l;
     });
 }
