@@ -536,9 +536,11 @@ def c3_leg(args):
     complex128 systems, solved by BiCGStab with interleaved DivCorr (the
     reference's adjoint method, bursts of 100 iterations, DivCorr PCG to
     1e-4 inside the bursts) in batches of 32 per model and phase, four
-    models at a time on four streams.  Each model
-    is released once its gradient is done, so at most four models' solve
-    workspaces are resident."""
+    models at a time on four streams.  The models are built on a context of
+    their own and taken, largest first, from one queue by four workers, each
+    moving the model it takes to its stream.  Each model is released once
+    its gradient is done, so at most four models' solve workspaces are
+    resident."""
     from maple3dmt_amd import PREC_MIXED, Context, configs, multimodel
     sizes = [(6, 80e3), (7, 40e3), (7, 60e3), (7, 80e3), (7, 100e3)]
     wls = []
@@ -547,18 +549,18 @@ def c3_leg(args):
         wls.append(configs.random3d(300 + i, max_level=lev, n_freq=16, half_width=100e3,
                                     station_radius=rad, maxiter=args.maxiter))
     # Largest meshes first (finest level, then widest refinement radius):
-    # jobs are dealt to the streams round robin and each stream runs its
-    # jobs in order, so the four largest start together and every stream
-    # ends on small ones.
+    # the four largest start together and a worker that finishes takes the
+    # next one, so the streams end on small ones.
     wls.sort(key=lambda wl: (wl.mesh.max_level, wl.mesh.station_refine_radius), reverse=True)
     n_streams = 4
+    build_ctx = Context(0)
     contexts = [Context(0) for _ in range(n_streams)]
     t = time.time()
     jobs = []
     for wl in wls:
         z, err = multimodel.halfspace_data(wl)
         jobs.append(multimodel.ModelJob(wl, z, err))
-    multimodel.prepare(jobs, contexts, n_threads=16)
+    multimodel.prepare(jobs, [build_ctx], n_threads=16)
     t_build = time.time() - t
     edges = [j.model.n for j in jobs]
     log(f"[c3] 16 models built in {t_build:.1f}s, edges {edges}")
@@ -567,7 +569,7 @@ def c3_leg(args):
         precision=prec, dc_interval=args.dc_interval)
     # Device-resident gradients: the station steps between the forward and
     # the adjoint batch run on the device.
-    wall = multimodel.run(jobs, n_streams, prm, prm, release=True, device=True)
+    wall = multimodel.run(jobs, n_streams, prm, prm, release=True, device=True, contexts=contexts)
     n_sys = multimodel.systems(jobs)
     iters = [int(sum(r["iterations"] for r in j.result.forward + j.result.adjoint)) for j in jobs]
     # A system is solved when it converged, or broke down (info 2) with its
@@ -595,7 +597,7 @@ def c3_leg(args):
            "dtype": "complex128 (f64)" if not prec else "complex64 inner, complex128 outer",
            "data": "synthetic (seeded random 3-D models, halfspace observations)"}
     log(f"[c3] {n_sys} systems in {wall:.1f}s, {n_solved} solved = {n_solved / wall:.1f} solved systems/s")
-    for c in contexts:
+    for c in contexts + [build_ctx]:
         c.close()
     return out
 

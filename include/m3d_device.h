@@ -77,6 +77,17 @@ typedef struct {
 int m3d_operator_create(m3d_context* ctx, const m3d_operator_desc* desc, m3d_operator** out);
 int m3d_operator_destroy(m3d_operator* op);
 
+/* Moves an operator to another context of the same device: waits for the
+ * operator's current stream, then runs every later call on ctx's stream
+ * (its workspaces, launch shapes and captured graphs stay; the old context
+ * is freed if it was destroyed and nothing else depends on it).  No
+ * counterpart in the reference, which has no device contexts: it is for a
+ * scheduler that builds many models on one context and hands each to
+ * whichever stream is free (the many-model batch that replaces the MPI
+ * frequency parallelism of freq_parallel.cpp).  Calls on the operator must
+ * not overlap the move. */
+int m3d_operator_set_context(m3d_operator* op, m3d_context* ctx);
+
 /* Edge mass Me(sigma) (operators.cpp:481-518), n values; call again after
  * every sigma update (ForwardSolverFV::update_sigma, :53-58). */
 int m3d_operator_set_mass(m3d_operator* op, const double* me);
@@ -423,6 +434,9 @@ typedef struct {
 /* Uploads the table; the set keeps its context alive until it is destroyed. */
 int m3d_stations_create(m3d_context* ctx, const m3d_station_desc* desc, m3d_stations** out);
 int m3d_stations_destroy(m3d_stations* st);
+/* m3d_operator_set_context for a station set (no reference counterpart;
+ * the same many-model scheduling). */
+int m3d_stations_set_context(m3d_stations* st, m3d_context* ctx);
 
 /* extract_impedance_ (:1490-1727) of nf frequencies: omega [nf] host,
  * station_sigma [ns] host (conductivity of each station cell, used by the

@@ -38,8 +38,14 @@ struct Fail : std::runtime_error {
 
 // Message slot of m3d_last_error (one per host thread).
 std::string& last_error();
-// Frees a destroyed context once nothing created on it is alive.
-void context_release(m3d_context* ctx);
+// Dependent-object bookkeeping of a context under one lock (objects may be
+// created, moved to another context and destroyed from several host
+// threads): attach counts an object in, detach counts it out and frees a
+// destroyed context once nothing created on it is alive, close marks the
+// context destroyed and frees it when nothing depends on it.
+void context_attach(m3d_context* ctx);
+void context_detach(m3d_context* ctx);
+void context_close(m3d_context* ctx);
 
 template <typename F>
 int guarded(F&& f) {

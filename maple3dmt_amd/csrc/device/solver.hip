@@ -36,11 +36,33 @@ std::string& last_error() {
     return msg;
 }
 
-void context_release(m3d_context* ctx) {
+std::mutex& context_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+void context_free_if_unused(m3d_context* ctx) {  // with the lock held
     if (!ctx->closed || ctx->operators > 0) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+void context_attach(m3d_context* ctx) {
+    std::lock_guard<std::mutex> g(context_mutex());
+    ctx->operators++;
+}
+
+void context_detach(m3d_context* ctx) {
+    std::lock_guard<std::mutex> g(context_mutex());
+    ctx->operators--;
+    context_free_if_unused(ctx);
+}
+
+void context_close(m3d_context* ctx) {
+    std::lock_guard<std::mutex> g(context_mutex());
+    ctx->closed = true;
+    context_free_if_unused(ctx);
 }
 
 }  // namespace abi
@@ -1815,8 +1837,7 @@ int m3d_context_create(int device, m3d_context** out) {
 int m3d_context_destroy(m3d_context* ctx) {
     return guarded([&] {
         if (!ctx || ctx->closed) return;
-        ctx->closed = true;
-        context_release(ctx);
+        context_close(ctx);
     });
 }
 
@@ -1838,7 +1859,7 @@ int m3d_operator_create(m3d_context* ctx, const m3d_operator_desc* desc, m3d_ope
         op->ctx = ctx;
         build_operator(*op, *desc);
         *out = op.release();
-        ctx->operators++;
+        context_attach(ctx);
     });
 }
 
@@ -1849,8 +1870,22 @@ int m3d_operator_destroy(m3d_operator* op) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         delete op;
-        ctx->operators--;
-        context_release(ctx);
+        context_detach(ctx);
+    });
+}
+
+int m3d_operator_set_context(m3d_operator* op, m3d_context* ctx) {
+    return guarded([&] {
+        if (!op || !ctx) throw Fail(M3D_ERR_ARG, "null argument");
+        if (ctx->closed) throw Fail(M3D_ERR_STATE, "context already destroyed");
+        m3d_context* old = op->ctx;
+        if (ctx == old) return;
+        if (ctx->device != old->device) throw Fail(M3D_ERR_ARG, "the context is on another device");
+        HIPCHK(hipSetDevice(old->device));
+        HIPCHK(hipStreamSynchronize(old->stream));
+        context_attach(ctx);
+        op->ctx = ctx;
+        context_detach(old);
     });
 }
 

@@ -331,7 +331,7 @@ int m3d_stations_create(m3d_context* ctx, const m3d_station_desc* d, m3d_station
         st->coef.alloc(fs * kCoef);
         st->skip.alloc(fs);
         HIPCHK(hipStreamSynchronize(s));
-        ctx->operators++;
+        context_attach(ctx);
         *out = st.release();
     });
 }
@@ -343,8 +343,22 @@ int m3d_stations_destroy(m3d_stations* st) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         delete st;
-        ctx->operators--;
-        context_release(ctx);
+        context_detach(ctx);
+    });
+}
+
+int m3d_stations_set_context(m3d_stations* st, m3d_context* ctx) {
+    return guarded([&] {
+        if (!st || !ctx) throw Fail(M3D_ERR_ARG, "null argument");
+        if (ctx->closed) throw Fail(M3D_ERR_STATE, "context already destroyed");
+        m3d_context* old = st->ctx;
+        if (ctx == old) return;
+        if (ctx->device != old->device) throw Fail(M3D_ERR_ARG, "the context is on another device");
+        HIPCHK(hipSetDevice(old->device));
+        HIPCHK(hipStreamSynchronize(old->stream));
+        context_attach(ctx);
+        st->ctx = ctx;
+        context_detach(old);
     });
 }
 

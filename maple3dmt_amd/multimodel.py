@@ -16,6 +16,7 @@ frequency runs the per-frequency loop of InversionFV::compute_gradient_
 from __future__ import annotations
 
 import os
+import queue
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -72,32 +73,55 @@ def prepare(jobs: Sequence[ModelJob], contexts: Sequence[Context], n_threads: Op
 
 def run(jobs: Sequence[ModelJob], n_workers: int, params: Optional[SolveParams] = None,
         adjoint_params: Optional[SolveParams] = None, release: bool = False,
-        device: bool = False) -> float:
+        device: bool = False, contexts: Optional[Sequence[Context]] = None) -> float:
     """Gradient of every job; with one worker all jobs run in order on the
     calling thread, else worker w runs, in order, the jobs whose context
     index is w modulo n_workers (so every job runs, and a context's jobs stay
-    on one thread).  device: the device-resident gradient
-    (inversion.gradient_device) instead of the host-station one.  With
-    release, each model (and its device memory) is closed once its gradient
-    is done.  Returns the wall time in seconds."""
+    on one thread).  With contexts (one per worker; n_workers must equal
+    their number) the workers instead take the jobs in list order from one
+    queue, and each job's model moves to the taking worker's context first
+    (Model.set_context), so a worker that finishes early takes the next job
+    whatever context the model was built on; build the models on a context
+    no worker uses, so that the move does not wait for another worker's
+    solves.  device: the device-resident gradient (inversion.gradient_device)
+    instead of the host-station one.  With release, each model (and its
+    device memory) is closed once its gradient is done.  Returns the wall
+    time in seconds."""
     if n_workers < 1:
         raise ValueError("n_workers must be at least 1")
+    if contexts is not None and len(contexts) != n_workers:
+        raise ValueError("dynamic scheduling takes one context per worker")
     if any(job.worker < 0 or job.model is None for job in jobs):
         raise ValueError("every job must be prepared (prepare()) before run()")
     errors: List[BaseException] = []
     grad = inversion.gradient_device if device else inversion.gradient
+    pending: "queue.Queue[ModelJob]" = queue.Queue()
+    if contexts is not None:
+        for job in jobs:
+            pending.put(job)
+
+    def solve(job: ModelJob):
+        t = time.perf_counter()
+        job.result = grad(job.model, job.workload.freqs, job.z_obs, job.z_err, params, adjoint_params)
+        job.seconds = time.perf_counter() - t
+        if release:
+            job.model.close()
 
     def work(w: int):
         try:
+            if contexts is not None:
+                while not errors:
+                    try:
+                        job = pending.get_nowait()
+                    except queue.Empty:
+                        return
+                    job.worker = w
+                    job.model.set_context(contexts[w])
+                    solve(job)
+                return
             for job in jobs:
-                if job.worker % n_workers != w:
-                    continue
-                t = time.perf_counter()
-                job.result = grad(job.model, job.workload.freqs, job.z_obs, job.z_err, params,
-                                  adjoint_params)
-                job.seconds = time.perf_counter() - t
-                if release:
-                    job.model.close()
+                if job.worker % n_workers == w:
+                    solve(job)
         except BaseException as e:  # re-raised in the caller
             errors.append(e)
 

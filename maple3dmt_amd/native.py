@@ -151,6 +151,7 @@ DEVICE_SYMBOLS = [
     "m3d_operator_update_sigma", "m3d_operator_update_sigma_device", "m3d_perturbation_rhs_device",
     "m3d_stations_create", "m3d_stations_destroy", "m3d_impedance_device",
     "m3d_weighted_residual_device", "m3d_adjoint_rhs_device", "m3d_sensitivity_device_noalloc",
+    "m3d_operator_set_context", "m3d_stations_set_context",
 ]
 HOST_SYMBOLS = [
     "m3dh_last_error", "m3dh_model_create", "m3dh_model_destroy", "m3dh_get_array",
@@ -189,6 +190,7 @@ def lib() -> C.CDLL:
     L.m3d_context_stream.restype = vp
     L.m3d_operator_create.argtypes = [vp, p(OperatorDescC), p(vp)]
     L.m3d_operator_destroy.argtypes = [vp]
+    L.m3d_operator_set_context.argtypes = [vp, vp]
     L.m3d_operator_set_mass.argtypes = [vp, vp]
     L.m3d_operator_set_divcorr.argtypes = [vp, p(DivcorrDescC)]
     L.m3d_solve_batch.argtypes = [vp, i, i, vp, vp, vp, vp, p(SolveParamsC), p(SolveResultC)]
@@ -210,6 +212,7 @@ def lib() -> C.CDLL:
     L.m3d_operator_autotune.argtypes = [vp, i, vp, i, vp]
     L.m3d_stations_create.argtypes = [vp, p(StationDescC), p(vp)]
     L.m3d_stations_destroy.argtypes = [vp]
+    L.m3d_stations_set_context.argtypes = [vp, vp]
     L.m3d_impedance_device.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_weighted_residual_device.argtypes = [vp, i, vp, vp, vp, vp, vp]
     L.m3d_adjoint_rhs_device.argtypes = [vp, i, vp, vp]
@@ -694,6 +697,20 @@ class Model:
         else:
             st.sigma = np.ascontiguousarray(self.array("st_sigma"), dtype=np.float64)
         return st
+
+    def set_context(self, ctx: Context) -> None:
+        """Moves the device operator and the device station set to ctx (same
+        device): waits for the current stream, then every later call runs on
+        ctx's stream with the same workspaces, launch shapes and graphs."""
+        if self.ctx is None:
+            raise NativeError("model has no device operator (created without a Context)")
+        if ctx is self.ctx:
+            return
+        _check_dev(lib().m3d_operator_set_context(self.device.handle, ctx.handle))
+        st = self._dev_stations
+        if st is not None and st.handle is not None:
+            _check_dev(lib().m3d_stations_set_context(st.handle, ctx.handle))
+        self.ctx = ctx
 
     def _drop_stations(self) -> None:
         st = getattr(self, "_dev_stations", None)

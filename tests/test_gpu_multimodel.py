@@ -52,6 +52,35 @@ def test_concurrent_models_match_sequential():
         c.close()
 
 
+def test_dynamic_schedule_matches_sequential():
+    """Models built on one context and taken from a shared queue by two
+    workers on contexts of their own (each model moved to the stream of the
+    worker that takes it) give exactly the gradients of running them one by
+    one.  The build context is destroyed before the run: it stays alive
+    until the last model has moved off it."""
+    build = Context(0)
+    workers = [Context(0) for _ in range(2)]
+    seq = _jobs()
+    multimodel.prepare(seq, workers[:1])
+    multimodel.run(seq, 1)
+    dyn = _jobs()
+    multimodel.prepare(dyn, [build])
+    build.close()
+    multimodel.run(dyn, 2, contexts=workers)
+    assert {j.worker for j in dyn} <= {0, 1}
+    for a, b in zip(seq, dyn):
+        assert b.model.ctx is workers[b.worker]
+        assert [r["iterations"] for r in a.result.forward] == [r["iterations"] for r in b.result.forward]
+        assert [r["iterations"] for r in a.result.adjoint] == [r["iterations"] for r in b.result.adjoint]
+        assert a.result.phi == b.result.phi
+        assert np.array_equal(a.result.grad, b.result.grad)
+        assert np.array_equal(a.result.z_pred, b.result.z_pred)
+    for j in seq + dyn:
+        j.model.close()
+    for c in workers:
+        c.close()
+
+
 def test_release_after_gradient_keeps_results():
     """run(..., release=True) closes each model once its gradient is done
     (the device memory goes back) and leaves the gradients those of a run
