@@ -264,6 +264,15 @@ __global__ __launch_bounds__(kConvWaves * 64) void k_conv_fwd_w(int batch, int d
 // wavefronts of a block may take different row counts).
 constexpr int kTileStride = 40;
 
+// Tile position of voxel x (even) in channel row r: the four 16-byte units
+// of a row (8 voxels each) are XOR-swizzled by (r / 8) % 4, so the 4-byte
+// writes of the four channel octets a wavefront transposes at once fall on
+// different banks (a row stride of 40 shorts alone maps them onto the same
+// ones), while every 8-voxel operand stays one aligned 16-byte unit.
+__device__ inline int tile_at(int r, int x) {
+    return r * kTileStride + 8 * ((x >> 3) ^ ((r >> 3) & 3)) + (x & 7);
+}
+
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows_per_chunk,
                                                     const unsigned short* __restrict__ in,
@@ -324,7 +333,7 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows
         // Transposed tiles: each channel's voxel pair is one 4-byte word.
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            *reinterpret_cast<unsigned*>(Tdz + (cg + j) * kTileStride + xg) =
+            *reinterpret_cast<unsigned*>(Tdz + tile_at(cg + j, xg)) =
                 unsigned(static_cast<unsigned short>(gdz[0][j])) |
                 (unsigned(static_cast<unsigned short>(gdz[1][j])) << 16);
         if (lane < NP) {
@@ -333,7 +342,7 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows
                 unsigned short* Tin = tin[wave][d];
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    *reinterpret_cast<unsigned*>(Tin + (ci0 + j) * kTileStride + xi) =
+                    *reinterpret_cast<unsigned*>(Tin + tile_at(ci0 + j, xi)) =
                         unsigned(static_cast<unsigned short>(gin[d][0][j])) |
                         (unsigned(static_cast<unsigned short>(gin[d][1][j])) << 16);
             }
@@ -344,10 +353,10 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(int batch, int dil, int rows
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int s16 = 0; s16 < 2; ++s16) {
-            const s8 a = *reinterpret_cast<const s8*>(Tdz + m * kTileStride + 16 * s16 + 8 * q);
+            const s8 a = *reinterpret_cast<const s8*>(Tdz + tile_at(m, 16 * s16 + 8 * q));
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const s8 w = *reinterpret_cast<const s8*>(tin[wave][d] + m * kTileStride + 16 * s16 + 8 * q);
+                const s8 w = *reinterpret_cast<const s8*>(tin[wave][d] + tile_at(m, 16 * s16 + 8 * q));
                 acc[d] = mfma16(a, w, acc[d]);
             }
         }
