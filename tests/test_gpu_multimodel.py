@@ -81,6 +81,35 @@ def test_dynamic_schedule_matches_sequential():
         c.close()
 
 
+def test_set_context_errors_and_same_context():
+    """Moving an operator or station set: a null context is an argument
+    error, a context already destroyed (kept alive by the objects still on
+    it) a state error, the current context a no-op.  Moving the last object
+    off a destroyed context frees it; the moved model then solves on its new
+    stream."""
+    from maple3dmt_amd.native import lib
+    a, c = Context(0), Context(0)
+    j1, j2 = _jobs()[:2]
+    multimodel.prepare([j1], [a])
+    multimodel.prepare([j2], [c])
+    m1, m2 = j1.model, j2.model
+    st = m1.device_stations(len(j1.workload.freqs))
+    assert lib().m3d_operator_set_context(m1.device.handle, None) == 1      # M3D_ERR_ARG
+    assert lib().m3d_stations_set_context(st.handle, None) == 1
+    ch = c.handle
+    c.close()                                   # destroyed; m2's operator keeps it alive
+    assert lib().m3d_operator_set_context(m1.device.handle, ch) == 3        # M3D_ERR_STATE
+    assert lib().m3d_stations_set_context(st.handle, ch) == 3
+    m1.set_context(a)                           # its own context: nothing to do
+    m2.set_context(a)                           # last object off c: c is freed
+    assert m2.ctx is a
+    for j in (j1, j2):
+        E, Z, res = j.model.solve_frequencies(j.workload.freqs)
+        assert all(solved(r) for r in res)
+        j.model.close()
+    a.close()
+
+
 def test_release_after_gradient_keeps_results():
     """run(..., release=True) closes each model once its gradient is done
     (the device memory goes back) and leaves the gradients those of a run
