@@ -279,7 +279,11 @@ int m3d_operator_trim(m3d_operator* op);
 
 /* Measurement (no reference counterpart): average duration in ms of one
  * launch of a kernel class over `reps` launches, timed with HIP events on the
- * context stream, and its algorithmic HBM bytes per launch.
+ * context stream, and its algorithmic HBM bytes per launch: alg_bytes counts
+ * every off-diagonal coupling as its value and its int32 column (w_v + 4
+ * bytes), the diagonal pair and each vector once; stored_bytes (may be NULL)
+ * the same with the bytes per coupling of the format the launch reads (one
+ * 4-byte word in the packed blocks and the row tiles).
  * kind 0: batched SpMV y = A x; kind 1: one multicolour SSOR apply;
  * kind 2: one FP64 BiCGStab iteration (b = 1, tolerance 0, so every system
  * keeps running); kind 3: the same iteration in complex64; kind 4: one
@@ -287,7 +291,7 @@ int m3d_operator_trim(m3d_operator* op);
  * apply: every colour but the last, whose backward step is written by its
  * forward launch when it has no upper couplings). */
 int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
-                     double* avg_ms, double* alg_bytes);
+                     double* avg_ms, double* alg_bytes, double* stored_bytes);
 
 /* Edge-side data of the edge mass, fixed per mesh, for sigma updates on the
  * device.  build_edge_mass (operators.cpp:481-518) gives

@@ -2189,7 +2189,7 @@ void divcorr_host_impl(m3d_operator& op, int nsys, const double* omega, double2*
 // 8 + 4 otherwise.
 template <int NS>
 void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int reps, double* avg_ms,
-                double* bytes) {
+                double* bytes, double* stored_bytes) {
     Work64<NS>& W = ready_workspace<NS>(op);
     auto& w = vec64<NS>(op, W);
     hipStream_t s = op.ctx->stream;
@@ -2285,30 +2285,37 @@ void bench_impl(m3d_operator& op, int kind, int nsys, const double* omega, int r
     for (int r = 0; r < op.colour_start[bwd_last + 1]; ++r)
         nfree_bwd += (op.perm[r] >= 0 && !op.h_constrained[op.perm[r]]) ? 1 : 0;
     const double m = double(op.nnz), n = double(nfree), k = double(nsys);
-    // Algorithmic bytes: values + columns of the m off-diagonal nonzeros, the
-    // (S_ii, Me_i) diagonal pair, and each vector touched once.
-    // One 4-byte word per coupling in the packed blocks and in the row tiles.
+    // Algorithmic bytes: value + column of the m off-diagonal nonzeros
+    // (w_v + 4 bytes each), the (S_ii, Me_i) diagonal pair, and each vector
+    // touched once.  stored_bytes: the same with the bytes per coupling of
+    // the format the launch reads, one 4-byte word in the packed blocks and
+    // the row tiles.
     const LaunchShape& shp = op.shape[kind == 3 ? 1 : 0][width_index(NS)];
     const bool pk = op.ntab && (shp.packed || (kind == 0 && shp.tiled && op.tiles[width_index(NS)].ntiles));
-    auto ent = [&](double wv) { return pk ? 4.0 : wv + 4.0; };
-    auto spmv_b = [&](double wv, double wc) { return m * ent(wv) + 2.0 * n * wv + 2.0 * k * n * wc; };
-    auto ssor_b = [&](double wv, double wc) { return m * ent(wv) + 4.0 * n * wv + 4.0 * k * n * wc; };
-    // One right-preconditioned BiCGStab iteration: 2 SpMV, 2 SSOR and 17
-    // vector passes of the AXPY / dot work.
-    auto iter_b = [&](double wv, double wc) {
-        return 2.0 * spmv_b(wv, wc) + 2.0 * ssor_b(wv, wc) + 17.0 * k * n * wc;
-    };
-    // Backward sweep: the upper half of the couplings, the diagonal pair, y
-    // in and z out of the rows it visits, per launch (one colour on average).
     const double nb = double(nfree_bwd);
-    auto bwd_b = [&](double wv, double wc) {
-        return (0.5 * m * ent(wv) + 2.0 * nb * wv + 2.0 * k * nb * wc) / (bwd_last + 1);
+    auto count = [&](bool stored) {
+        auto ent = [&](double wv) { return stored && pk ? 4.0 : wv + 4.0; };
+        auto spmv_b = [&](double wv, double wc) { return m * ent(wv) + 2.0 * n * wv + 2.0 * k * n * wc; };
+        auto ssor_b = [&](double wv, double wc) { return m * ent(wv) + 4.0 * n * wv + 4.0 * k * n * wc; };
+        // One right-preconditioned BiCGStab iteration: 2 SpMV, 2 SSOR and
+        // 17 vector passes of the AXPY / dot work.
+        auto iter_b = [&](double wv, double wc) {
+            return 2.0 * spmv_b(wv, wc) + 2.0 * ssor_b(wv, wc) + 17.0 * k * n * wc;
+        };
+        // Backward sweep: the upper half of the couplings, the diagonal
+        // pair, y in and z out of the rows it visits, per launch (one colour
+        // on average).
+        auto bwd_b = [&](double wv, double wc) {
+            return (0.5 * m * ent(wv) + 2.0 * nb * wv + 2.0 * k * nb * wc) / (bwd_last + 1);
+        };
+        return kind == 0   ? spmv_b(8, 16)
+               : kind == 1 ? ssor_b(8, 16)
+               : kind == 2 ? iter_b(8, 16)
+               : kind == 3 ? iter_b(4, 8)
+                           : bwd_b(8, 16);
     };
-    *bytes = kind == 0   ? spmv_b(8, 16)
-             : kind == 1 ? ssor_b(8, 16)
-             : kind == 2 ? iter_b(8, 16)
-             : kind == 3 ? iter_b(4, 8)
-                         : bwd_b(8, 16);
+    *bytes = count(false);
+    if (stored_bytes) *stored_bytes = count(true);
 }
 
 bool valid_shape(int width, const LaunchShape& sh) {
@@ -2332,12 +2339,12 @@ double time_iteration(m3d_operator& op, int p, int slot, const double* omega, in
     const int kind = p == 0 ? 2 : 3, k = 1 << slot;
     double ms = 0.0, bytes = 0.0;
     switch (k) {
-        case 1: bench_impl<1>(op, kind, k, omega, reps, &ms, &bytes); break;
-        case 2: bench_impl<2>(op, kind, k, omega, reps, &ms, &bytes); break;
-        case 4: bench_impl<4>(op, kind, k, omega, reps, &ms, &bytes); break;
-        case 8: bench_impl<8>(op, kind, k, omega, reps, &ms, &bytes); break;
-        case 16: bench_impl<16>(op, kind, k, omega, reps, &ms, &bytes); break;
-        default: bench_impl<32>(op, kind, k, omega, reps, &ms, &bytes); break;
+        case 1: bench_impl<1>(op, kind, k, omega, reps, &ms, &bytes, nullptr); break;
+        case 2: bench_impl<2>(op, kind, k, omega, reps, &ms, &bytes, nullptr); break;
+        case 4: bench_impl<4>(op, kind, k, omega, reps, &ms, &bytes, nullptr); break;
+        case 8: bench_impl<8>(op, kind, k, omega, reps, &ms, &bytes, nullptr); break;
+        case 16: bench_impl<16>(op, kind, k, omega, reps, &ms, &bytes, nullptr); break;
+        default: bench_impl<32>(op, kind, k, omega, reps, &ms, &bytes, nullptr); break;
     }
     return ms;
 }
@@ -2468,19 +2475,19 @@ int m3d_ssor_apply(m3d_operator* op, int nsys, const double* omega, const double
 }
 
 int m3d_bench_kernel(m3d_operator* op, int kind, int nsys, const double* omega, int reps,
-                     double* avg_ms, double* alg_bytes) {
+                     double* avg_ms, double* alg_bytes, double* stored_bytes) {
     return guarded([&] {
         check_ready(op);
         if (!omega || !avg_ms || !alg_bytes || reps <= 0 || kind < 0 || kind > 4)
             throw Fail(M3D_ERR_ARG, "invalid benchmark arguments");
         HIPCHK(hipSetDevice(op->ctx->device));
         for_width(nsys, [&](int k) {
-            if (k == 1) bench_impl<1>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
-            else if (k == 2) bench_impl<2>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
-            else if (k <= 4) bench_impl<4>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
-            else if (k <= 8) bench_impl<8>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
-            else if (k <= 16) bench_impl<16>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
-            else bench_impl<32>(*op, kind, k, omega, reps, avg_ms, alg_bytes);
+            if (k == 1) bench_impl<1>(*op, kind, k, omega, reps, avg_ms, alg_bytes, stored_bytes);
+            else if (k == 2) bench_impl<2>(*op, kind, k, omega, reps, avg_ms, alg_bytes, stored_bytes);
+            else if (k <= 4) bench_impl<4>(*op, kind, k, omega, reps, avg_ms, alg_bytes, stored_bytes);
+            else if (k <= 8) bench_impl<8>(*op, kind, k, omega, reps, avg_ms, alg_bytes, stored_bytes);
+            else if (k <= 16) bench_impl<16>(*op, kind, k, omega, reps, avg_ms, alg_bytes, stored_bytes);
+            else bench_impl<32>(*op, kind, k, omega, reps, avg_ms, alg_bytes, stored_bytes);
         });
     });
 }

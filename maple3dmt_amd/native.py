@@ -203,7 +203,7 @@ def lib() -> C.CDLL:
     L.m3d_operator_format.argtypes = [vp, p(i), p(i)]
     L.m3d_operator_device_bytes.argtypes = [vp, p(C.c_longlong), p(C.c_longlong)]
     L.m3d_operator_trim.argtypes = [vp]
-    L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d)]
+    L.m3d_bench_kernel.argtypes = [vp, i, i, vp, i, p(d), p(d), p(d)]
     L.m3d_sensitivity_device.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_sensitivity_device_noalloc.argtypes = [vp, i, vp, vp, vp, vp]
     L.m3d_perturbation_rhs_device.argtypes = [vp, i, vp, vp, vp, vp]
@@ -416,16 +416,18 @@ class DeviceOperator:
         _check_dev(lib().m3d_operator_permutation(self.handle, _ptr(perm), _ptr(cs)))
         return perm, cs
 
-    def bench_kernel(self, kind: int, omegas, reps: int = 20):
+    def bench_kernel(self, kind: int, omegas, reps: int = 20, stored: bool = False):
         """(avg ms per launch, algorithmic bytes per launch) of kind 0 = SpMV,
         1 = SSOR apply, 2 = one FP64 BiCGStab iteration, 3 = one complex64
         BiCGStab iteration, 4 = one backward-sweep launch (mean over the
-        colours), timed with HIP events on the library stream."""
+        colours), timed with HIP events on the library stream; the bytes
+        count a coupling as its value and int32 column.  With stored, a third
+        value: the bytes with the stored format's bytes per coupling."""
         om = np.ascontiguousarray(omegas, dtype=np.float64)
-        ms, by = C.c_double(), C.c_double()
+        ms, by, sb = C.c_double(), C.c_double(), C.c_double()
         _check_dev(lib().m3d_bench_kernel(self.handle, kind, om.shape[0], _ptr(om), reps,
-                                          C.byref(ms), C.byref(by)))
-        return ms.value, by.value
+                                          C.byref(ms), C.byref(by), C.byref(sb)))
+        return (ms.value, by.value, sb.value) if stored else (ms.value, by.value)
 
     def update_sigma_device(self, d_sigma: int) -> None:
         """m3d_operator_update_sigma_device: sigma [cells] in device memory."""
