@@ -14,6 +14,9 @@ under "variants", the same batch solved
   fp64_interleaved           with E_sec by BiCGStab bursts of 100 iterations
                              each followed by DivCorr (solve_with_divcorr_)
   mixed_interleaved          both: the fast path
+  mixed_interleaved_tol1e-9  the fast path at relative residual 1e-9
+  mixed_interleaved_dc32     the fast path with its DivCorr PCG between bursts
+                             in FP32 (M3D_DIVCORR_FP32=1; FP64 by default)
   mixed_interleaved_learned  the fast path started from the trained network's
                              e0 (inference and prolongation inside the timed
                              region), when its weights exist
@@ -345,6 +348,13 @@ def main():
         # path's Z is converged to 1e-6 on the level-6 parity models (the
         # reference's 1e-7 leaves up to 1e-4 there at 19-100 Hz).
         legs["mixed_interleaved_tol1e-9"] = leg("mixed+interleaved tol 1e-9", params(PREC_MIXED, True, 1e-9))
+        # The fast path with the DivCorr PCG between bursts in FP32 (opt-in:
+        # the specified precision split keeps the potential solve in FP64).
+        os.environ["M3D_DIVCORR_FP32"] = "1"
+        try:
+            legs["mixed_interleaved_dc32"] = leg("mixed+interleaved, FP32 DivCorr PCG", params(PREC_MIXED, True))
+        finally:
+            del os.environ["M3D_DIVCORR_FP32"]
         if not args.no_learned and os.path.exists(args.weights):
             from maple3dmt_amd.learned import Box, CoarseNet, initial_guess
             wts = np.load(args.weights)

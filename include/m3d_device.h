@@ -194,10 +194,11 @@ typedef struct {
                                 with interleaved_forward); 0 = the reference's 1e-10
                                 (forward_solver_fv.cpp:700-702).  The FORWARD
                                 post-hoc correction and m3d_divcorr always use 1e-10.
-                                With precision MIXED and a tolerance >= 1e-5 this PCG
-                                runs in FP32 (complex64 node vectors, FP32 Laplacian,
-                                double reductions; E and the stops stay FP64); the
-                                environment variable M3D_DIVCORR_FP32=0 keeps it FP64. */
+                                This PCG runs in FP64.  With the environment variable
+                                M3D_DIVCORR_FP32=1, precision MIXED and a tolerance
+                                >= 1e-5 it runs in FP32 instead (complex64 node
+                                vectors, FP32 Laplacian, double reductions; E and
+                                the stops stay FP64). */
 } m3d_solve_params;
 
 /* Per-system outcome; mirrors BiCGStabResult (bicgstab.h:23-28) plus what the

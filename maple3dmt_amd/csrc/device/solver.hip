@@ -669,18 +669,18 @@ template <int NS>
 void divcorr(m3d_operator& op, Work64<NS>& W, double2* E, const double2* phi0, unsigned mask,
              const m3d_solve_params& prm);
 
-// The DivCorr steps between the bursts of the mixed-precision path run their
-// PCG in FP32 (complex64 node vectors, FP32 Laplacian, double reductions)
-// when its tolerance is at least 1e-5, where FP32 rounding (6e-8) stays far
-// below the stop; M3D_DIVCORR_FP32=0 (read at each call) keeps them in FP64.
-// The FP64 paths, the post-hoc correction and m3d_divcorr (1e-10) always run
-// in FP64.
+// The divergence-correction potential solve stays FP64.  On request
+// (M3D_DIVCORR_FP32=1, read at each call) the DivCorr steps between the
+// bursts of the mixed-precision path run their PCG in FP32 (complex64 node
+// vectors, FP32 Laplacian, double reductions) when its tolerance is at least
+// 1e-5, where FP32 rounding (6e-8) stays far below the stop.  The FP64 paths,
+// the post-hoc correction and m3d_divcorr (1e-10) always run in FP64.
 constexpr double kDcFp32MinTol = 1e-5;
 
 bool dc_in_fp32(const m3d_solve_params& prm) {
     if (prm.precision != M3D_PREC_MIXED || !(prm.divcorr_pcg_tol >= kDcFp32MinTol)) return false;
     const char* e = getenv("M3D_DIVCORR_FP32");
-    return !(e && e[0] == '0');
+    return e && e[0] == '1';
 }
 
 void ensure_laplacian32(m3d_operator& op) {

@@ -1,9 +1,9 @@
 """The FP32 DivCorr PCG of the mixed-precision path against the FP64 one.
 
-With precision = mixed and a DivCorr PCG tolerance of at least 1e-5 (the fast
-path's 1e-4), the DivCorr steps between the interleaved bursts run their PCG
-in FP32 (complex64 node vectors, FP32 Laplacian, double reductions);
-M3D_DIVCORR_FP32=0 keeps them in FP64.  The field E, the divergence that
+With M3D_DIVCORR_FP32=1, precision = mixed and a DivCorr PCG tolerance of at
+least 1e-5 (the fast path's 1e-4), the DivCorr steps between the interleaved
+bursts run their PCG in FP32 (complex64 node vectors, FP32 Laplacian, double
+reductions); by default they stay FP64.  The field E, the divergence that
 stops DivCorr and the true residual that stops the solve stay FP64, so the
 result meets the same tolerance: the impedances of both forms agree within
 the 1e-5 parity bar (the solve tolerance decides their difference, not the
