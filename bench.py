@@ -350,11 +350,15 @@ def main():
         legs["mixed_interleaved_tol1e-9"] = leg("mixed+interleaved tol 1e-9", params(PREC_MIXED, True, 1e-9))
         # The fast path with the DivCorr PCG between bursts in FP32 (opt-in:
         # the specified precision split keeps the potential solve in FP64).
+        saved = os.environ.get("M3D_DIVCORR_FP32")
         os.environ["M3D_DIVCORR_FP32"] = "1"
         try:
             legs["mixed_interleaved_dc32"] = leg("mixed+interleaved, FP32 DivCorr PCG", params(PREC_MIXED, True))
         finally:
-            del os.environ["M3D_DIVCORR_FP32"]
+            if saved is None:
+                del os.environ["M3D_DIVCORR_FP32"]
+            else:
+                os.environ["M3D_DIVCORR_FP32"] = saved
         if not args.no_learned and os.path.exists(args.weights):
             from maple3dmt_amd.learned import Box, CoarseNet, initial_guess
             wts = np.load(args.weights)
